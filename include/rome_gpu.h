@@ -1,0 +1,163 @@
+/*
+ * rome_gpu.h -- C-ABI of the MI355X-native MAP E-step + Fourier backprojection
+ * engine (librome_gpu.so).  Plain C types only; every pointer is a host pointer
+ * unless the function name says "device".  Functions return 0 on success and a
+ * negative code on failure; rome_last_error() then holds the message.  A context
+ * is thread-compatible (one host thread at a time), never thread-safe.
+ *
+ * The reference (ROME 1.1.2) has no plugin API: its E-step is a set of functions in
+ * namespace Map3dOptimizer_old / Map2dOptimizer_old reading namespace globals.  Each
+ * entry point below names the reference interface it replaces; a drop-in caller sets
+ * the same state the reference keeps in those globals and calls rome_expectation()
+ * where the reference calls expectationSomeParticles() for a pool of images.
+ */
+#ifndef ROME_GPU_H
+#define ROME_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rome_ctx rome_ctx;
+
+enum {
+    ROME_OK = 0,
+    ROME_ERR_ARG = -1,        /* bad argument / not configured */
+    ROME_ERR_HIP = -2,        /* HIP runtime error */
+    ROME_ERR_NUMERIC = -3,    /* numeric guard (zero significant samples, zero weight sum) */
+    ROME_ERR_UNSUPPORTED = -4 /* configuration outside what the engine implements */
+};
+
+/* Iteration-level configuration: the globals of src/map_optimizer_base.h:57-136 that
+ * the E-step reads (current_size, coarse_size, adaptive_oversampling/fraction,
+ * sigma2_fudge, do_ctf_correction, refs_are_ctf_corrected, do_norm_correction, do_map)
+ * plus the sampling sizes of HealpixSampler (src/sampling.h:113-134). */
+typedef struct {
+    int ref_dim;               /* 3: rome_map3d path, 2: rome_map2d path */
+    int ori_size, current_size, coarse_size;
+    int nr_classes, nr_groups;
+    int adaptive_oversampling; /* 0 or 1 */
+    int nr_dir, nr_psi, nr_trans;  /* coarse grid; nr_dir = 1 in 2D */
+    double adaptive_fraction, sigma2_fudge, sigma2_offset, avg_norm_correction;
+    int do_ctf_correction, refs_are_ctf_corrected, do_norm_correction, do_map;
+} rome_config;
+
+/* One pool of prepared images, the state ParticleModel::setFourierTransforms leaves
+ * (src/exp_model.h:397-639): masked / unmasked Fourier transforms windowed to
+ * current_size (half-plane, n = i*(cur/2+1)+j), CTF image, highres_Xi2, power spectrum,
+ * rounded old offsets, group (0-based), NORM.  FP64 host arrays, SoA like the reference. */
+typedef struct {
+    int n;
+    const double *fmask_re, *fmask_im, *fnomask_re, *fnomask_im, *fctf; /* [n][cur*(cur/2+1)] */
+    const double* highres_Xi2;  /* [n] */
+    const double* power_img;    /* [n][ori/2+1] */
+    const double *old_offx, *old_offy, *norm; /* [n] */
+    const int* group;           /* [n] */
+} rome_images;
+
+/* Device-resident variant: interleaved FP32 complex / FP32 CTF, FP64 scalars. */
+typedef struct {
+    int n;
+    const void* fmask;   /* float2 [n][cur*(cur/2+1)] */
+    const void* fnomask; /* float2 [n][cur*(cur/2+1)] */
+    const float* fctf;   /* [n][cur*(cur/2+1)] */
+    const double* highres_Xi2;
+    const double* power_img;
+    const double *old_offx, *old_offy, *norm;
+    const int* group;
+} rome_device_images;
+
+/* Per-image results (host, any pointer may be NULL).  meta columns follow the STAR
+ * labels the reference writes (src/metadata.h:55): PMAX CLASS ROT TILT PSI XOFF YOFF
+ * NORM DLL.  argmax = (class, dir, psi, over_rot, trans, over_trans) of the last pass. */
+typedef struct {
+    double* meta;            /* [n][9] */
+    int* nr_sign;            /* [n] NR_SIGN of the coarse pass */
+    int* argmax;             /* [n][6] */
+    double* min_diff2;       /* [n] last pass */
+    double* sum_weight;      /* [n] */
+    double* sig_weight;      /* [n] */
+    double* min_diff2_coarse;  /* [n] */
+    double* sum_weight_coarse; /* [n] */
+    double* sig_weight_coarse; /* [n] */
+    double* wsum_norm_correction; /* [n] */
+} rome_results;
+
+/* Weighted sums of MLModel (src/ml_model.h) accumulated over rome_expectation calls. */
+typedef struct {
+    double* wsum_sigma2_noise;   /* [G][ori/2+1] */
+    double* wsum_sumw_group;     /* [G] */
+    double* wsum_pdf_class;      /* [K] */
+    double* wsum_pdf_direction;  /* [K][nr_dir] */
+    double* wsum_prior_offx;     /* [K] (2D) */
+    double* wsum_prior_offy;     /* [K] (2D) */
+    double* wsum_scalars;        /* [4] sigma2_offset, LL, ave_Pmax, avg_norm_correction */
+} rome_sums;
+
+/* ---- context ---- */
+int rome_create(int device, rome_ctx** out);
+void rome_destroy(rome_ctx* ctx);
+const char* rome_last_error(const rome_ctx* ctx);
+
+/* Map3dOptimizer_old::prepareExpData sizes (src/map3d_optimizer.cpp:615-714). */
+int rome_configure(rome_ctx* ctx, const rome_config* cfg);
+
+/* SamplingGrid tables (src/sampling.cpp:834-845, 927-958), perturbation already applied:
+ * 3D pose_coarse [dir][psi][3] (rot, tilt, psi deg), pose_fine [dir][psi][8][3];
+ * 2D pose_coarse [psi], pose_fine [psi][2].  Shifts [T] / [T][4] in pixels (perturbed);
+ * trans_x/y [T] unperturbed (pdf_offset, src/map3d_optimizer.cpp:1704-1716).
+ * pose_fine / shift_fine may be NULL when adaptive_oversampling == 0. */
+int rome_set_sampling(rome_ctx* ctx, const double* pose_coarse, const double* pose_fine,
+                      const double* shift_coarse_x, const double* shift_coarse_y,
+                      const double* shift_fine_x, const double* shift_fine_y,
+                      const double* trans_x, const double* trans_y);
+
+/* MLModel state read by the E-step: sigma2_noise [G][ori/2+1], pdf_class [K],
+ * pdf_direction [K][nr_dir], prior_offset{x,y}_class [K]. */
+int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_class,
+                   const double* pdf_direction, const double* prior_offx, const double* prior_offy);
+
+/* MyProjector::data of one class after computeFourierTransformMap
+ * (src/reconstruct.cpp:106-223): complex interleaved FP64 [pad][pad][pad/2+1] (3D) or
+ * [pad][pad/2+1] (2D), pad = 2*(2*min(cur/2,ori/2)+1)+1. */
+int rome_set_reference(rome_ctx* ctx, int iclass, const double* grid);
+
+/* MLModel::resetZero of the E-step sums + zeroed backprojector grids (MAPModel). */
+int rome_reset_sums(rome_ctx* ctx);
+
+/* Map3dOptimizer_old::expectationSomeParticles (src/map3d_optimizer.cpp:883-953) or
+ * Map2dOptimizer_old::expectationSomeParticles (src/map2d_optimizer.cpp:802-854), with
+ * storeWeightedSums, for one pool of images ("iteration >= 2" semantics: all classes). */
+int rome_expectation(rome_ctx* ctx, const rome_images* images, rome_results* results);
+int rome_expectation_device(rome_ctx* ctx, const rome_device_images* images, rome_results* results);
+
+/* MLModel weighted sums (src/ml_model.cpp:91-114 layout). */
+int rome_get_sums(rome_ctx* ctx, rome_sums* out);
+
+/* MyBackProjector data / weight of one class (src/reconstruct.h:546-717), FP64 host. */
+int rome_get_backprojector(rome_ctx* ctx, int iclass, double* data, double* weight);
+
+/* Parity hooks: coarse-pass diff2 of the last pool [n][K][dir][psi][T] (NaN where the
+ * reference stores no entry), and the fine-pass list as (key, diff2) with
+ * key = ((((img*K + k)*D + dir)*PSI + psi)*T + t)*NOR*NOT + over_rot*NOT + over_trans. */
+int rome_get_coarse_diff2(rome_ctx* ctx, double* out);
+long long rome_get_fine_count(rome_ctx* ctx);
+int rome_get_fine_list(rome_ctx* ctx, long long* keys, double* diff2);
+
+/* MyProjector::projectOneTile / rotate2D of arbitrary rotations (A row-major 3x3 each)
+ * into full half-planes of size `size`: out [nrot][size*(size/2+1)]. */
+int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9,
+                 double* out_re, double* out_im);
+
+/* ---- host sampling helpers (HealpixSampler, C1) ---- */
+int rome_sampling_pose3d(int healpix_order, int oversampling, double perturbation, double* out);
+int rome_sampling_pose2d(double psi_step, int oversampling, double perturbation, double* out);
+int rome_sampling_translations(double offset_range, double offset_step, double* tx, double* ty);
+int rome_sampling_shifts(double offset_range, double offset_step, int oversampling, double perturbation,
+                         double* out_x, double* out_y);
+int rome_healpix_pix2ang(int order, int pix, double* z, double* phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

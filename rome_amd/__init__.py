@@ -1,0 +1,263 @@
+"""rome_amd -- MI355X-native MAP E-step + Fourier backprojection engine.
+
+Python binding of librome_gpu.so (C-ABI declared in include/rome_gpu.h).  The HIP
+library is the only implementation: importing works without it, but every call
+raises if the library was not built (``python -m rome_amd.build``).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librome_gpu.so")
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_longlong)
+
+META_FIELDS = ("PMAX", "CLASS", "ROT", "TILT", "PSI", "XOFF", "YOFF", "NORM", "DLL")
+
+EXPORTED = (
+    "rome_create", "rome_destroy", "rome_last_error", "rome_configure", "rome_set_sampling", "rome_set_model",
+    "rome_set_reference", "rome_reset_sums", "rome_expectation", "rome_expectation_device", "rome_get_sums",
+    "rome_get_backprojector", "rome_get_coarse_diff2", "rome_get_fine_count", "rome_get_fine_list", "rome_project",
+    "rome_sampling_pose3d", "rome_sampling_pose2d", "rome_sampling_translations", "rome_sampling_shifts",
+    "rome_healpix_pix2ang",
+)
+
+
+class RomeError(RuntimeError):
+    pass
+
+
+class rome_config(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ref_dim", "ori_size", "current_size", "coarse_size", "nr_classes",
+                                        "nr_groups", "adaptive_oversampling", "nr_dir", "nr_psi", "nr_trans")] + \
+               [(n, C.c_double) for n in ("adaptive_fraction", "sigma2_fudge", "sigma2_offset",
+                                           "avg_norm_correction")] + \
+               [(n, C.c_int) for n in ("do_ctf_correction", "refs_are_ctf_corrected", "do_norm_correction",
+                                        "do_map")]
+
+
+class rome_images(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(n, _dp) for n in (
+        "fmask_re", "fmask_im", "fnomask_re", "fnomask_im", "fctf", "highres_Xi2", "power_img", "old_offx",
+        "old_offy", "norm")] + [("group", _ip)]
+
+
+class rome_device_images(C.Structure):
+    _fields_ = [("n", C.c_int), ("fmask", C.c_void_p), ("fnomask", C.c_void_p), ("fctf", C.c_void_p),
+                ("highres_Xi2", C.c_void_p), ("power_img", C.c_void_p), ("old_offx", C.c_void_p),
+                ("old_offy", C.c_void_p), ("norm", C.c_void_p), ("group", C.c_void_p)]
+
+
+class rome_results(C.Structure):
+    _fields_ = [("meta", _dp), ("nr_sign", _ip), ("argmax", _ip)] + [(n, _dp) for n in (
+        "min_diff2", "sum_weight", "sig_weight", "min_diff2_coarse", "sum_weight_coarse", "sig_weight_coarse",
+        "wsum_norm_correction")]
+
+
+class rome_sums(C.Structure):
+    _fields_ = [(n, _dp) for n in ("wsum_sigma2_noise", "wsum_sumw_group", "wsum_pdf_class", "wsum_pdf_direction",
+                                   "wsum_prior_offx", "wsum_prior_offy", "wsum_scalars")]
+
+
+def lib():
+    """Load librome_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RomeError(f"{LIB_PATH} not found: build it with `python -m rome_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.rome_last_error.restype = C.c_char_p
+        L.rome_get_fine_count.restype = C.c_longlong
+        L.rome_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        for name in EXPORTED:
+            getattr(L, name)  # every declared symbol must resolve
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+# ---------------- host sampling (HealpixSampler, C1) ----------------
+def pose_table3d(order, oversampling, perturbation):
+    nd, npsi = 12 * 4 ** order, 6 * 2 ** order
+    nover = 8 if oversampling else 1
+    out = np.zeros((nd, npsi, nover, 3))
+    if lib().rome_sampling_pose3d(order, oversampling, C.c_double(perturbation), _p(out)) < 0:
+        raise RomeError("rome_sampling_pose3d failed")
+    return out if oversampling else out[:, :, 0, :]
+
+
+def pose_table2d(psi_step, oversampling, perturbation):
+    npsi = int(np.ceil(360.0 / psi_step))
+    nover = 2 if oversampling else 1
+    out = np.zeros((npsi, nover))
+    if lib().rome_sampling_pose2d(C.c_double(psi_step), oversampling, C.c_double(perturbation), _p(out)) < 0:
+        raise RomeError("rome_sampling_pose2d failed")
+    return out if oversampling else out[:, 0]
+
+
+def translations(offset_range, offset_step):
+    n = lib().rome_sampling_translations(C.c_double(offset_range), C.c_double(offset_step), None, None)
+    tx, ty = np.zeros(n), np.zeros(n)
+    lib().rome_sampling_translations(C.c_double(offset_range), C.c_double(offset_step), _p(tx), _p(ty))
+    return tx, ty
+
+
+def shift_table(offset_range, offset_step, oversampling, perturbation):
+    T = len(translations(offset_range, offset_step)[0])
+    nover = 4 if oversampling else 1
+    x, y = np.zeros((T, nover)), np.zeros((T, nover))
+    lib().rome_sampling_shifts(C.c_double(offset_range), C.c_double(offset_step), oversampling,
+                               C.c_double(perturbation), _p(x), _p(y))
+    return x, y
+
+
+def healpix_pix2ang(order, pix):
+    z, phi = C.c_double(), C.c_double()
+    if lib().rome_healpix_pix2ang(order, pix, C.byref(z), C.byref(phi)) != 0:
+        raise RomeError("bad HEALPix pixel")
+    return z.value, phi.value
+
+
+class Engine:
+    """One device context: the E-step of rome_map3d / rome_map2d on one GPU."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().rome_create(device, C.byref(self._h))
+        if rc != 0:
+            raise RomeError(f"rome_create failed ({rc}): no usable GPU")
+        self.cfg = None
+
+    def close(self):
+        if self._h:
+            lib().rome_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RomeError(f"{what} failed ({rc}): {lib().rome_last_error(self._h).decode()}")
+
+    # ---- iteration setup
+    def configure(self, **kw):
+        c = rome_config(**kw)
+        self._check(lib().rome_configure(self._h, C.byref(c)), "rome_configure")
+        self.cfg = dict(kw)
+        self.is3d = c.ref_dim == 3
+        self.K, self.D, self.PSI, self.T = c.nr_classes, c.nr_dir, c.nr_psi, c.nr_trans
+        self.nor = (8 if self.is3d else 2) if c.adaptive_oversampling else 1
+        self.nots = 4 if c.adaptive_oversampling else 1
+        r_max = min(c.current_size // 2, c.ori_size // 2)
+        self.pad = 2 * (2 * r_max + 1) + 1
+        padF = self.pad // 2 + 1
+        self.grid_n = self.pad * self.pad * padF if self.is3d else self.pad * padF
+        self.oriF = c.ori_size // 2 + 1
+        self.G = c.nr_groups
+        self.F2 = c.current_size * (c.current_size // 2 + 1)
+
+    def set_sampling(self, pose_coarse, pose_fine, shift_coarse_x, shift_coarse_y, shift_fine_x, shift_fine_y,
+                     trans_x, trans_y):
+        keep = [None if a is None else _f64(a) for a in (pose_coarse, pose_fine, shift_coarse_x, shift_coarse_y,
+                                                         shift_fine_x, shift_fine_y, trans_x, trans_y)]
+        self._check(lib().rome_set_sampling(self._h, *[_p(a) for a in keep]), "rome_set_sampling")
+
+    def set_model(self, sigma2_noise, pdf_class, pdf_direction, prior_offx, prior_offy):
+        keep = [_f64(a) for a in (sigma2_noise, pdf_class, pdf_direction, prior_offx, prior_offy)]
+        self._check(lib().rome_set_model(self._h, *[_p(a) for a in keep]), "rome_set_model")
+
+    def set_reference(self, iclass, grid):
+        g = _f64(grid)
+        if g.size != 2 * self.grid_n:
+            raise RomeError(f"reference grid has {g.size} values, expected {2 * self.grid_n}")
+        self._check(lib().rome_set_reference(self._h, iclass, _p(g)), "rome_set_reference")
+
+    def reset_sums(self):
+        self._check(lib().rome_reset_sums(self._h), "rome_reset_sums")
+
+    # ---- E-step
+    def _results(self, n):
+        r = {"meta": np.zeros((n, 9)), "nr_sign": np.zeros(n, np.int32), "argmax": np.zeros((n, 6), np.int32)}
+        for k in ("min_diff2", "sum_weight", "sig_weight", "min_diff2_coarse", "sum_weight_coarse",
+                  "sig_weight_coarse", "wsum_norm_correction"):
+            r[k] = np.zeros(n)
+        cr = rome_results(meta=_p(r["meta"]), nr_sign=_p(r["nr_sign"], C.c_int), argmax=_p(r["argmax"], C.c_int),
+                          **{k: _p(r[k]) for k in ("min_diff2", "sum_weight", "sig_weight", "min_diff2_coarse",
+                                                   "sum_weight_coarse", "sig_weight_coarse",
+                                                   "wsum_norm_correction")})
+        return r, cr
+
+    def expectation(self, b):
+        """b: dict of host arrays (fmask_re, fmask_im, fnomask_re, fnomask_im, fctf, highres_Xi2, power_img,
+        old_offx, old_offy, norm, group)."""
+        n = int(b["n"])
+        keep = {k: _f64(b[k]) for k in ("fmask_re", "fmask_im", "fnomask_re", "fnomask_im", "fctf", "highres_Xi2",
+                                        "power_img", "old_offx", "old_offy", "norm")}
+        grp = np.ascontiguousarray(b["group"], dtype=np.int32)
+        im = rome_images(n=n, group=_p(grp, C.c_int), **{k: _p(v) for k, v in keep.items()})
+        r, cr = self._results(n)
+        self._check(lib().rome_expectation(self._h, C.byref(im), C.byref(cr)), "rome_expectation")
+        return r
+
+    def expectation_device(self, n, fmask, fnomask, fctf, highres_Xi2, power_img, old_offx, old_offy, norm, group,
+                           results=True):
+        """Device pointers (ints) of FP32 complex / FP32 / FP64 / int32 arrays already in HBM."""
+        im = rome_device_images(n=n, fmask=fmask, fnomask=fnomask, fctf=fctf, highres_Xi2=highres_Xi2,
+                                power_img=power_img, old_offx=old_offx, old_offy=old_offy, norm=norm, group=group)
+        if results:
+            r, cr = self._results(n)
+            self._check(lib().rome_expectation_device(self._h, C.byref(im), C.byref(cr)), "rome_expectation_device")
+            return r
+        self._check(lib().rome_expectation_device(self._h, C.byref(im), None), "rome_expectation_device")
+        return None
+
+    def get_sums(self):
+        s = {"wsum_sigma2_noise": np.zeros((self.G, self.oriF)), "wsum_sumw_group": np.zeros(self.G),
+             "wsum_pdf_class": np.zeros(self.K), "wsum_pdf_direction": np.zeros((self.K, self.D)),
+             "wsum_prior_offx": np.zeros(self.K), "wsum_prior_offy": np.zeros(self.K),
+             "wsum_scalars": np.zeros(4)}
+        cs = rome_sums(**{k: _p(v) for k, v in s.items()})
+        self._check(lib().rome_get_sums(self._h, C.byref(cs)), "rome_get_sums")
+        return s
+
+    def get_backprojector(self, iclass):
+        d = np.zeros(2 * self.grid_n)
+        w = np.zeros(self.grid_n)
+        self._check(lib().rome_get_backprojector(self._h, iclass, _p(d), _p(w)), "rome_get_backprojector")
+        return d, w
+
+    def get_coarse_diff2(self, n):
+        out = np.zeros((n, self.K, self.D, self.PSI, self.T))
+        self._check(lib().rome_get_coarse_diff2(self._h, _p(out)), "rome_get_coarse_diff2")
+        return out
+
+    def get_fine_list(self):
+        cnt = lib().rome_get_fine_count(self._h)
+        keys = np.zeros(max(cnt, 1), np.int64)
+        d2 = np.zeros(max(cnt, 1))
+        self._check(lib().rome_get_fine_list(self._h, _p(keys, C.c_longlong), _p(d2)), "rome_get_fine_list")
+        return keys[:cnt], d2[:cnt]
+
+    def project(self, iclass, size, A):
+        A = _f64(A).reshape(-1, 9)
+        n = size * (size // 2 + 1)
+        re = np.zeros((A.shape[0], n))
+        im = np.zeros((A.shape[0], n))
+        self._check(lib().rome_project(self._h, iclass, size, A.shape[0], _p(A), _p(re), _p(im)), "rome_project")
+        return re, im

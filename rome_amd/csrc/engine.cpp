@@ -1,0 +1,997 @@
+// Host orchestration of the MI355X E-step (C++ above the C-ABI).  Mirrors the call
+// order of Map3dOptimizer_old::expectationSomeParticles (src/map3d_optimizer.cpp:883-953):
+//   coarse pass  : preShiftedImages -> getAllSquaredDifferencesCoarse -> convert -> significance
+//   fine pass    : getAllSquaredDifferencesFine -> convert -> significance
+//   updateOtherParams + backProjection (one fused kernel), storeWeightedSums.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/rome_gpu.h"
+#include "kernels.h"
+#include "rome_common.h"
+
+namespace rome {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = std::max<size_t>(count, 1);
+        return e;
+    }
+    hipError_t upload(const T* h, size_t count, hipStream_t s) {
+        hipError_t e = reserve(count);
+        if (e != hipSuccess) return e;
+        if (count == 0) return hipSuccess;
+        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+struct Error {
+    int code;
+};
+
+}  // namespace rome
+
+using namespace rome;
+
+struct rome_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    rome_config cfg{};
+    bool configured = false, have_sampling = false, have_model = false;
+    std::vector<char> have_ref;
+    bool fct_dirty = true;
+
+    // derived sizes
+    int is3d = 1, K = 0, D = 0, PSI = 0, T = 0, R = 0, Mtot = 0, Mpad = 0;
+    int NORf = 1, NOTf = 1, oriF = 0, cs = 0, fs = 0, F2f = 0;
+    int pad = 0, padF = 0, cgrid = 0, r_max = 0;
+    size_t grid_n = 0;
+    int Pc = 0, Pc_pad = 0, Pf = 0, TW = 1;
+    bool two_pass = true, post_enabled = true;
+
+    // host copies of tables
+    std::vector<double> pose_c, pose_f, sxc, syc, sxf, syf, tx, ty;
+    std::vector<double> sigma2_h, pdf_class_h, pdf_dir_h, prior_x_h, prior_y_h;
+    std::vector<int> f_nfine_h, f_xy_h;
+
+    // device persistent
+    DevBuf<float2> grids;
+    DevBuf<double> Ai_c, Ai_f, sintab, costab, d_sxc, d_syc, d_sxf, d_syf, d_tx, d_ty;
+    DevBuf<int> c_nfine, c_ires, c_xy, f_nfine, f_ires, f_xy;
+    DevBuf<float2> Sc, Sf;
+    DevBuf<double> sigma2, pdf_dir, lpdf_dir, prior_x, prior_y, logsigma2;
+    DevBuf<unsigned char> valid;
+    DevBuf<float> FcT;
+    // sums
+    DevBuf<float> bp_data, bp_weight;
+    DevBuf<double> wsum_sigma2, wsum_pdf_class, wsum_pdf_dir, wsum_sumw_group, wsum_prior_x, wsum_prior_y, wsum_scalars;
+    // per batch
+    DevBuf<float2> in_fmask, in_fnomask;
+    DevBuf<float> in_fctf;
+    DevBuf<double> in_hxi2, in_power, in_offx, in_offy, in_norm;
+    DevBuf<int> in_group;
+    DevBuf<float> WcT, MC2c, At, E, Minvf, Ctff, MC2bp;
+    DevBuf<double> C2c, C2f;
+    DevBuf<float2> Xf, NMf;
+    DevBuf<double> c_dmin, c_sumw, c_maxw, c_thr, f_dmin, f_sumw, f_maxw, f_thr, wnorm, meta;
+    DevBuf<long long> c_amax, f_amax;
+    DevBuf<int> c_nsig, c_status, f_status;
+    DevBuf<uint32_t> sigmask;
+    DevBuf<int> ntask, nblk;
+    DevBuf<long long> task_off, blk_off;
+    DevBuf<int> task_img, task_m, task_nt, blk_task, blk_t, ulist, m2u, ucount;
+    DevBuf<long long> task_blk0;
+    DevBuf<unsigned char> used;
+    DevBuf<float2> Pfine;
+    DevBuf<double> D2;
+    DevBuf<int2> amax_mt;
+    // last batch bookkeeping (parity hooks)
+    int last_n = 0;
+    long long last_blocks = 0, last_tasks = 0;
+};
+
+namespace {
+
+int fail(rome_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess)                                                                          \
+            return fail(ctx, ROME_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),       \
+                        __FILE__, __LINE__);                                                           \
+    } while (0)
+
+inline int pack_xy(int x, int y) { return (int)(((uint32_t)(y & 0xffff) << 16) | (uint32_t)(x & 0xffff)); }
+
+// Mresol of src/map_model.cpp:545-575
+int mresol(int size, int i, int j) {
+    const int Fsize = size / 2 + 1;
+    const int ip = (i < Fsize) ? i : i - size;
+    const int ires = (int)round(sqrt((double)(ip * ip + j * j)));
+    return (ires < Fsize && !(j == 0 && ip < 0)) ? ires : -1;
+}
+
+void rot6(const double A[9], double* out) {
+    // Ainv = 2 * A^T restricted to the (x, y) columns (src/reconstruct.cpp:456-466)
+    out[0] = 2.0 * A[0];
+    out[1] = 2.0 * A[3];
+    out[2] = 2.0 * A[1];
+    out[3] = 2.0 * A[4];
+    out[4] = 2.0 * A[2];
+    out[5] = 2.0 * A[5];
+}
+
+int upload_poses(rome_ctx* ctx) {
+    const int nc = ctx->R, nf = ctx->R * ctx->NORf;
+    std::vector<double> a(6 * (size_t)nc), b(6 * (size_t)std::max(nf, 1));
+    double A[9];
+    for (int r = 0; r < nc; ++r) {
+        if (ctx->is3d) euler_matrix(ctx->pose_c[3 * r], ctx->pose_c[3 * r + 1], ctx->pose_c[3 * r + 2], A);
+        else euler_matrix(0, 0, ctx->pose_c[r], A);
+        rot6(A, &a[6 * r]);
+    }
+    if (ctx->two_pass) {
+        for (int r = 0; r < nf; ++r) {
+            if (ctx->is3d) euler_matrix(ctx->pose_f[3 * r], ctx->pose_f[3 * r + 1], ctx->pose_f[3 * r + 2], A);
+            else euler_matrix(0, 0, ctx->pose_f[r], A);
+            rot6(A, &b[6 * r]);
+        }
+    } else {
+        b = a;  // single pass: the "fine" stage reuses the pass's own rotations
+    }
+    HIPCHK(ctx->Ai_c.upload(a.data(), a.size(), ctx->stream));
+    HIPCHK(ctx->Ai_f.upload(b.data(), b.size(), ctx->stream));
+    return ROME_OK;
+}
+
+int build_phase_tables(rome_ctx* ctx) {
+    const rome_config& c = ctx->cfg;
+    HIPCHK(ctx->Sc.reserve((size_t)ctx->T * ctx->Pc));
+    HIPCHK(launch_phase_table(ctx->stream, ctx->d_sxc.p, ctx->d_syc.p, ctx->T, ctx->c_xy.p, ctx->Pc, c.ori_size,
+                              ctx->sintab.p, ctx->costab.p, ctx->Sc.p));
+    const int nsf = ctx->T * ctx->NOTf;
+    HIPCHK(ctx->Sf.reserve((size_t)nsf * ctx->Pf));
+    const double* sx = ctx->two_pass ? ctx->d_sxf.p : ctx->d_sxc.p;
+    const double* sy = ctx->two_pass ? ctx->d_syf.p : ctx->d_syc.p;
+    HIPCHK(launch_phase_table(ctx->stream, sx, sy, nsf, ctx->f_xy.p, ctx->Pf, c.ori_size, ctx->sintab.p, ctx->costab.p,
+                              ctx->Sf.p));
+    return ROME_OK;
+}
+
+int ensure_ready(rome_ctx* ctx) {
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (!ctx->have_sampling) return fail(ctx, ROME_ERR_ARG, "rome_set_sampling has not been called");
+    if (!ctx->have_model) return fail(ctx, ROME_ERR_ARG, "rome_set_model has not been called");
+    for (int k = 0; k < ctx->K; ++k)
+        if (!ctx->have_ref[k]) return fail(ctx, ROME_ERR_ARG, "reference of class %d not set", k);
+    if (ctx->fct_dirty) {
+        HIPCHK(ctx->FcT.reserve((size_t)2 * ctx->Pc_pad * ctx->Mpad));
+        HIPCHK(hipMemsetAsync(ctx->FcT.p, 0, (size_t)2 * ctx->Pc_pad * ctx->Mpad * sizeof(float), ctx->stream));
+        const int my_r_max = std::min(ctx->r_max, ctx->cs / 2);
+        HIPCHK(launch_project_coarse(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d,
+                                     ctx->Ai_c.p, ctx->R, ctx->Mtot, ctx->c_xy.p, ctx->Pc, ctx->Pc_pad, my_r_max,
+                                     ctx->Mpad, ctx->FcT.p));
+        ctx->fct_dirty = false;
+    }
+    return ROME_OK;
+}
+
+WeightParams weight_params(rome_ctx* ctx) {
+    WeightParams wp;
+    wp.K = ctx->K;
+    wp.D = ctx->D;
+    wp.PSI = ctx->PSI;
+    wp.R = ctx->R;
+    wp.Mtot = ctx->Mtot;
+    wp.T = ctx->T;
+    wp.pdf_dir = ctx->pdf_dir.p;
+    wp.lpdf_dir = ctx->lpdf_dir.p;
+    wp.trans_x = ctx->d_tx.p;
+    wp.trans_y = ctx->d_ty.p;
+    wp.old_offx = ctx->in_offx.p;
+    wp.old_offy = ctx->in_offy.p;
+    wp.prior_x = ctx->prior_x.p;
+    wp.prior_y = ctx->prior_y.p;
+    wp.sigma2_offset = ctx->cfg.sigma2_offset;
+    wp.adaptive_fraction = ctx->cfg.adaptive_fraction;
+    return wp;
+}
+
+// The E-step on inputs already in ctx->in_* (device).
+int run_estep(rome_ctx* ctx, int n, rome_results* res) {
+    int rc = ensure_ready(ctx);
+    if (rc) return rc;
+    const rome_config& c = ctx->cfg;
+    hipStream_t s = ctx->stream;
+    const int T = ctx->T, M = ctx->Mtot;
+    const int Ncols = n * T;
+    const int Npad = (Ncols + 127) / 128 * 128;
+    const int Kdim = 2 * ctx->Pc_pad;
+    // ---- image preparation
+    HIPCHK(ctx->WcT.reserve((size_t)Kdim * Npad));
+    HIPCHK(hipMemsetAsync(ctx->WcT.p, 0, (size_t)Kdim * Npad * sizeof(float), s));
+    HIPCHK(ctx->MC2c.reserve((size_t)n * ctx->Pc));
+    HIPCHK(ctx->C2c.reserve(n));
+    HIPCHK(ctx->C2f.reserve(n));
+    const size_t nPf = (size_t)n * ctx->Pf;
+    HIPCHK(ctx->Xf.reserve(nPf));
+    HIPCHK(ctx->Minvf.reserve(nPf));
+    HIPCHK(ctx->Ctff.reserve(nPf));
+    HIPCHK(ctx->NMf.reserve(nPf));
+    HIPCHK(ctx->MC2bp.reserve(nPf));
+    PrepArgs pa;
+    pa.fmask = ctx->in_fmask.p;
+    pa.fnomask = ctx->in_fnomask.p;
+    pa.fctf = ctx->in_fctf.p;
+    pa.group = ctx->in_group.p;
+    pa.hxi2 = ctx->in_hxi2.p;
+    pa.sigma2 = ctx->sigma2.p;
+    pa.ori_Fsize = ctx->oriF;
+    pa.F2f = ctx->F2f;
+    pa.sigma2_fudge = c.sigma2_fudge;
+    pa.ctf_in_diff = (c.do_ctf_correction && c.refs_are_ctf_corrected) ? 1 : 0;
+    pa.do_ctf = c.do_ctf_correction ? 1 : 0;
+    pa.do_map = c.do_map ? 1 : 0;
+    pa.have_fine = 1;
+    pa.Pc = ctx->Pc;
+    pa.Pc_pad = ctx->Pc_pad;
+    pa.T = T;
+    pa.Npad = Npad;
+    pa.c_nfine = ctx->c_nfine.p;
+    pa.c_ires = ctx->c_ires.p;
+    pa.Sc = ctx->Sc.p;
+    pa.Pf = ctx->Pf;
+    pa.f_nfine = ctx->f_nfine.p;
+    pa.f_ires = ctx->f_ires.p;
+    pa.WcT = ctx->WcT.p;
+    pa.MC2c = ctx->MC2c.p;
+    pa.C2c = ctx->C2c.p;
+    pa.Xf = ctx->Xf.p;
+    pa.Minvf = ctx->Minvf.p;
+    pa.Ctff = ctx->Ctff.p;
+    pa.C2f = ctx->C2f.p;
+    pa.NMf = ctx->NMf.p;
+    pa.MC2bp = ctx->MC2bp.p;
+    HIPCHK(launch_prep_images(s, n, pa));
+    // ---- coarse cross term + norms
+    HIPCHK(ctx->At.reserve((size_t)n * M));
+    HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
+    HIPCHK(ctx->E.reserve((size_t)n * M * T));
+    GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, ctx->E.p, Kdim, ctx->Mpad, Npad, M, Ncols, T};
+    HIPCHK(launch_coarse_gemm(s, ga));
+    // ---- coarse weights and significance
+    for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_maxw, &ctx->c_thr, &ctx->f_dmin, &ctx->f_sumw, &ctx->f_maxw,
+                    &ctx->f_thr, &ctx->wnorm})
+        HIPCHK(b->reserve(n));
+    HIPCHK(ctx->meta.reserve((size_t)9 * n));
+    HIPCHK(ctx->c_amax.reserve(n));
+    HIPCHK(ctx->f_amax.reserve(n));
+    HIPCHK(ctx->c_nsig.reserve(n));
+    HIPCHK(ctx->c_status.reserve(n));
+    HIPCHK(ctx->f_status.reserve(n));
+    HIPCHK(ctx->sigmask.reserve((size_t)n * M * ctx->TW));
+    WeightParams wp = weight_params(ctx);
+    WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p, ctx->c_nsig.p,
+                     ctx->c_status.p};
+    HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->valid.p, ctx->C2c.p, wc, ctx->sigmask.p, ctx->TW));
+    // ---- tasks (image, rotation) of the fine pass
+    HIPCHK(ctx->ntask.reserve(n));
+    HIPCHK(ctx->nblk.reserve(n));
+    HIPCHK(ctx->task_off.reserve(n + 1));
+    HIPCHK(ctx->blk_off.reserve(n + 1));
+    HIPCHK(launch_task_build(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p,
+                             ctx->blk_off.p));
+    long long tot[2];
+    HIPCHK(hipMemcpyAsync(&tot[0], ctx->task_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&tot[1], ctx->blk_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+    std::vector<int> cstatus(n);
+    HIPCHK(hipMemcpyAsync(cstatus.data(), ctx->c_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i)
+        if (cstatus[i] == 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
+    const long long ntasks = tot[0], nblocks = tot[1];
+    ctx->last_n = n;
+    ctx->last_tasks = ntasks;
+    ctx->last_blocks = nblocks;
+    const int NOR = ctx->NORf, NOT = ctx->NOTf, NE = NOR * NOT;
+    HIPCHK(ctx->task_img.reserve(ntasks));
+    HIPCHK(ctx->task_m.reserve(ntasks));
+    HIPCHK(ctx->task_nt.reserve(ntasks));
+    HIPCHK(ctx->task_blk0.reserve(ntasks));
+    HIPCHK(ctx->blk_task.reserve(nblocks));
+    HIPCHK(ctx->blk_t.reserve(nblocks));
+    HIPCHK(ctx->used.reserve(M));
+    HIPCHK(hipMemsetAsync(ctx->used.p, 0, M, s));
+    HIPCHK(ctx->ulist.reserve(M));
+    HIPCHK(ctx->m2u.reserve(M));
+    HIPCHK(ctx->ucount.reserve(1));
+    HIPCHK(launch_task_emit(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->task_off.p, ctx->blk_off.p, ctx->task_img.p,
+                            ctx->task_m.p, ctx->task_blk0.p, ctx->task_nt.p, ctx->blk_task.p, ctx->blk_t.p, ctx->used.p,
+                            ctx->ulist.p, ctx->m2u.p, ctx->ucount.p));
+    int U = 0;
+    HIPCHK(hipMemcpyAsync(&U, ctx->ucount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(ctx->D2.reserve((size_t)nblocks * NE));
+    HIPCHK(ctx->Pfine.reserve((size_t)U * NOR * ctx->Pf));
+    HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d, ctx->Ai_f.p,
+                               ctx->R, NOR, ctx->ulist.p, U, ctx->f_xy.p, ctx->Pf, ctx->r_max, ctx->Pfine.p));
+    FineArgs fa;
+    fa.task_img = ctx->task_img.p;
+    fa.task_m = ctx->task_m.p;
+    fa.task_blk0 = ctx->task_blk0.p;
+    fa.task_nt = ctx->task_nt.p;
+    fa.blk_t = ctx->blk_t.p;
+    fa.m2u = ctx->m2u.p;
+    fa.Pfine = ctx->Pfine.p;
+    fa.Xf = ctx->Xf.p;
+    fa.Minvf = ctx->Minvf.p;
+    fa.Ctff = ctx->Ctff.p;
+    fa.Sf = ctx->Sf.p;
+    fa.C2f = ctx->C2f.p;
+    fa.D2 = ctx->D2.p;
+    fa.Pf = ctx->Pf;
+    const double* thr_p;
+    const double* sum_p;
+    const double* dmin_p;
+    WeightOutArgs wf{ctx->f_dmin.p, ctx->f_sumw.p, ctx->f_maxw.p, ctx->f_amax.p, ctx->f_thr.p, nullptr,
+                     ctx->f_status.p};
+    if (ctx->two_pass) {
+        HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa));
+        HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE,
+                                   wf));
+        thr_p = ctx->f_thr.p;
+        sum_p = ctx->f_sumw.p;
+        dmin_p = ctx->f_dmin.p;
+    } else {
+        // single pass: the pass's own significant entries feed the sums (2D, oversampling 0)
+        HIPCHK(launch_fill_d2(s, ctx->blk_task.p, ctx->blk_t.p, ctx->task_img.p, ctx->task_m.p, nblocks, ctx->E.p,
+                              ctx->C2c.p, M, T, ctx->D2.p));
+        thr_p = ctx->c_thr.p;
+        sum_p = ctx->c_sumw.p;
+        dmin_p = ctx->c_dmin.p;
+    }
+    // ---- updateOtherParams + backProjection
+    if (ctx->post_enabled) {
+        PostArgs pp;
+        pp.K = ctx->K;
+        pp.D = ctx->D;
+        pp.PSI = ctx->PSI;
+        pp.R = ctx->R;
+        pp.is3d = ctx->is3d;
+        pp.Pf = ctx->Pf;
+        pp.oriF = ctx->oriF;
+        pp.pad = ctx->pad;
+        pp.padF = ctx->padF;
+        pp.c = ctx->cgrid;
+        pp.r_max = ctx->r_max;
+        pp.grid_stride = ctx->grid_n;
+        pp.group = ctx->in_group.p;
+        pp.sumw = sum_p;
+        pp.thr = thr_p;
+        pp.dmin = dmin_p;
+        pp.task_off = ctx->task_off.p;
+        pp.task_m = ctx->task_m.p;
+        pp.task_nt = ctx->task_nt.p;
+        pp.task_blk0 = ctx->task_blk0.p;
+        pp.blk_t = ctx->blk_t.p;
+        pp.m2u = ctx->m2u.p;
+        pp.D2 = ctx->D2.p;
+        pp.pdf_dir = ctx->pdf_dir.p;
+        pp.trans_x = ctx->d_tx.p;
+        pp.trans_y = ctx->d_ty.p;
+        pp.shift_x = ctx->two_pass ? ctx->d_sxf.p : ctx->d_sxc.p;
+        pp.shift_y = ctx->two_pass ? ctx->d_syf.p : ctx->d_syc.p;
+        pp.old_offx = ctx->in_offx.p;
+        pp.old_offy = ctx->in_offy.p;
+        pp.prior_x = ctx->prior_x.p;
+        pp.prior_y = ctx->prior_y.p;
+        pp.sigma2_offset = c.sigma2_offset;
+        pp.Ai_fine = ctx->Ai_f.p;
+        pp.Pfine = ctx->Pfine.p;
+        pp.Xf = ctx->Xf.p;
+        pp.Ctff = ctx->Ctff.p;
+        pp.NMf = ctx->NMf.p;
+        pp.MC2bp = ctx->MC2bp.p;
+        pp.Sf = ctx->Sf.p;
+        pp.f_xy = ctx->f_xy.p;
+        pp.f_ires = ctx->f_ires.p;
+        pp.bp_data = ctx->bp_data.p;
+        pp.bp_weight = ctx->bp_weight.p;
+        pp.wsum_sigma2 = ctx->wsum_sigma2.p;
+        pp.wsum_pdf_class = ctx->wsum_pdf_class.p;
+        pp.wsum_pdf_dir = ctx->wsum_pdf_dir.p;
+        pp.wsum_sumw_group = ctx->wsum_sumw_group.p;
+        pp.wsum_prior_x = ctx->wsum_prior_x.p;
+        pp.wsum_prior_y = ctx->wsum_prior_y.p;
+        pp.wsum_scalars = ctx->wsum_scalars.p;
+        pp.wnorm = ctx->wnorm.p;
+        HIPCHK(launch_post(s, n, NOR, NOT, pp));
+    } else {
+        HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
+    }
+    // ---- metadata of the last pass
+    std::vector<long long> amax(n);
+    std::vector<double> maxw(n), sumw(n);
+    const long long* amax_d = ctx->two_pass ? ctx->f_amax.p : ctx->c_amax.p;
+    HIPCHK(hipMemcpyAsync(amax.data(), amax_d, n * sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(maxw.data(), ctx->two_pass ? ctx->f_maxw.p : ctx->c_maxw.p, n * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(sumw.data(), sum_p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    std::vector<double> offx(n), offy(n);
+    HIPCHK(hipMemcpyAsync(offx.data(), ctx->in_offx.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(offy.data(), ctx->in_offy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    std::vector<int> fstatus(n, 0);
+    std::vector<int2> mt(n);
+    if (ctx->two_pass) {
+        HIPCHK(ctx->amax_mt.reserve(n));
+        HIPCHK(launch_decode_argmax(s, ctx->f_amax.p, ctx->blk_off.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, n, NE,
+                                    ctx->amax_mt.p));
+        HIPCHK(hipMemcpyAsync(mt.data(), ctx->amax_mt.p, n * sizeof(int2), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(fstatus.data(), ctx->f_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> argmax(6 * (size_t)n, 0);
+    std::vector<double> meta(9 * (size_t)n, 0.);
+    for (int i = 0; i < n; ++i) {
+        if (!(sumw[i] > 0)) return fail(ctx, ROME_ERR_NUMERIC, "image %d: exp_sum_weight == 0", i);
+        int m, t, o = 0, ot = 0;
+        if (ctx->two_pass) {
+            m = mt[i].x;
+            t = mt[i].y;
+            o = (int)((amax[i] % NE) / NOT);
+            ot = (int)(amax[i] % NOT);
+        } else {
+            m = (int)(amax[i] / T);
+            t = (int)(amax[i] % T);
+        }
+        const int k = m / ctx->R, r = m % ctx->R, dir = r / ctx->PSI, psi = r % ctx->PSI;
+        int* am = &argmax[6 * i];
+        am[0] = k, am[1] = dir, am[2] = psi, am[3] = o, am[4] = t, am[5] = ot;
+        double* md = &meta[9 * i];
+        md[0] = maxw[i] / sumw[i];
+        md[1] = k + 1;
+        const bool fine = ctx->two_pass;
+        if (ctx->is3d) {
+            const double* pose = fine ? &ctx->pose_f[3 * ((size_t)r * NOR + o)] : &ctx->pose_c[3 * (size_t)r];
+            md[2] = pose[0], md[3] = pose[1], md[4] = pose[2];
+        } else {
+            double ps = fine ? ctx->pose_f[(size_t)r * NOR + o] : ctx->pose_c[r];
+            if (ps > 180.) ps -= 360.;
+            if (ps < -180.) ps += 360.;
+            md[2] = 0, md[3] = 0, md[4] = ps;
+        }
+        const std::vector<double>& sx = fine ? ctx->sxf : ctx->sxc;
+        const std::vector<double>& sy = fine ? ctx->syf : ctx->syc;
+        const int nots = fine ? NOT : 1;
+        md[5] = offx[i] + sx[(size_t)t * nots + ot];
+        md[6] = offy[i] + sy[(size_t)t * nots + ot];
+    }
+    HIPCHK(hipMemcpyAsync(ctx->meta.p, meta.data(), meta.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    // ---- storeWeightedSums
+    StoreArgs sa;
+    sa.n = n;
+    sa.curF = ctx->fs / 2 + 1;
+    sa.oriF = ctx->oriF;
+    sa.do_norm = c.do_norm_correction;
+    sa.avg_norm = c.avg_norm_correction;
+    sa.group = ctx->in_group.p;
+    sa.power = ctx->in_power.p;
+    sa.norm = ctx->in_norm.p;
+    sa.sumw = sum_p;
+    sa.dmin = dmin_p;
+    sa.logsigma2 = ctx->logsigma2.p;
+    sa.wnorm = ctx->wnorm.p;
+    sa.meta = ctx->meta.p;
+    sa.wsum_sigma2 = ctx->wsum_sigma2.p;
+    sa.wsum_scalars = ctx->wsum_scalars.p;
+    HIPCHK(launch_store_sums(s, sa));
+    if (res) {
+        auto d2h = [&](void* h, const void* d, size_t bytes) -> hipError_t {
+            return h ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+        };
+        HIPCHK(d2h(res->meta, ctx->meta.p, 9 * (size_t)n * sizeof(double)));
+        HIPCHK(d2h(res->nr_sign, ctx->c_nsig.p, n * sizeof(int)));
+        HIPCHK(d2h(res->min_diff2, dmin_p, n * sizeof(double)));
+        HIPCHK(d2h(res->sum_weight, sum_p, n * sizeof(double)));
+        HIPCHK(d2h(res->sig_weight, thr_p, n * sizeof(double)));
+        HIPCHK(d2h(res->min_diff2_coarse, ctx->c_dmin.p, n * sizeof(double)));
+        HIPCHK(d2h(res->sum_weight_coarse, ctx->c_sumw.p, n * sizeof(double)));
+        HIPCHK(d2h(res->sig_weight_coarse, ctx->c_thr.p, n * sizeof(double)));
+        HIPCHK(d2h(res->wsum_norm_correction, ctx->wnorm.p, n * sizeof(double)));
+        if (res->argmax) std::memcpy(res->argmax, argmax.data(), argmax.size() * sizeof(int));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    (void)fstatus;
+    return ROME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rome_create(int device, rome_ctx** out) {
+    if (!out) return ROME_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ROME_ERR_HIP;
+    if (device < 0 || device >= ndev) return ROME_ERR_ARG;
+    rome_ctx* ctx = new rome_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return ROME_ERR_HIP;
+    }
+    // 5000-entry sin/cos table of shiftImageInFourierTransform (src/image.h:1486-1510)
+    std::vector<double> st(5000), ct(5000);
+    const double step = 2 * 3.14159265358979323846 / 5000.0;
+    for (int i = 0; i < 5000; ++i) {
+        st[i] = sin((double)i * step);
+        ct[i] = cos((double)i * step);
+    }
+    if (ctx->sintab.upload(st.data(), 5000, ctx->stream) != hipSuccess ||
+        ctx->costab.upload(ct.data(), 5000, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        rome_destroy(ctx);
+        return ROME_ERR_HIP;
+    }
+    *out = ctx;
+    return ROME_OK;
+}
+
+void rome_destroy(rome_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* rome_last_error(const rome_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
+    if (!ctx || !cfg) return ROME_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    const rome_config& c = *cfg;
+    if (c.ref_dim != 2 && c.ref_dim != 3) return fail(ctx, ROME_ERR_ARG, "ref_dim must be 2 or 3");
+    if (c.adaptive_oversampling < 0 || c.adaptive_oversampling > 1)
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "adaptive_oversampling must be 0 or 1");
+    if (c.current_size <= 0 || c.current_size > c.ori_size || (c.current_size & 1) || c.coarse_size <= 0 ||
+        c.coarse_size > c.current_size || (c.coarse_size & 1))
+        return fail(ctx, ROME_ERR_ARG, "bad image sizes");
+    if (c.nr_classes <= 0 || c.nr_classes > kMaxK || c.nr_groups <= 0) return fail(ctx, ROME_ERR_ARG, "bad class/group count");
+    if (c.nr_trans <= 0 || c.nr_trans > kMaxT || c.nr_classes * c.nr_trans > kMaxKT)
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "nr_trans %d outside the supported range", c.nr_trans);
+    if (c.ori_size / 2 + 1 > kMaxShell) return fail(ctx, ROME_ERR_UNSUPPORTED, "ori_size too large");
+    if (c.ref_dim == 2 && c.nr_dir != 1) return fail(ctx, ROME_ERR_ARG, "2D requires nr_dir == 1");
+    ctx->cfg = c;
+    ctx->is3d = c.ref_dim == 3;
+    ctx->K = c.nr_classes;
+    ctx->D = c.nr_dir;
+    ctx->PSI = c.nr_psi;
+    ctx->T = c.nr_trans;
+    ctx->R = c.nr_dir * c.nr_psi;
+    ctx->Mtot = ctx->K * ctx->R;
+    ctx->Mpad = (ctx->Mtot + 127) / 128 * 128;
+    ctx->two_pass = c.adaptive_oversampling > 0;
+    ctx->post_enabled = !(ctx->is3d && !ctx->two_pass);  // 3D stores no fine weights without oversampling
+    ctx->NORf = ctx->two_pass ? (ctx->is3d ? 8 : 2) : 1;
+    ctx->NOTf = ctx->two_pass ? 4 : 1;
+    ctx->oriF = c.ori_size / 2 + 1;
+    ctx->fs = c.current_size;
+    ctx->cs = ctx->two_pass ? c.coarse_size : c.current_size;
+    ctx->F2f = ctx->fs * (ctx->fs / 2 + 1);
+    ctx->r_max = std::min(c.current_size / 2, c.ori_size / 2);
+    ctx->pad = 2 * (2 * ctx->r_max + 1) + 1;
+    ctx->padF = ctx->pad / 2 + 1;
+    ctx->cgrid = (int)((float)ctx->pad / 2.0);
+    ctx->grid_n = ctx->is3d ? (size_t)ctx->pad * ctx->pad * ctx->padF : (size_t)ctx->pad * ctx->padF;
+    ctx->TW = (ctx->T + 31) / 32;
+    // pixel sets: coarse = Mresol_coarse > 0 (Minv != 0), fine = Mresol_fine >= 0
+    std::vector<int> cn, ci, cxy, fn, fi, fxy;
+    {
+        const int sz = ctx->cs, F = sz / 2 + 1;
+        for (int i = 0; i < sz; ++i)
+            for (int j = 0; j < F; ++j) {
+                const int ires = mresol(sz, i, j);
+                if (ires <= 0) continue;
+                const int y = (i < F) ? i : i - sz;
+                const int row_f = y >= 0 ? y : y + ctx->fs;
+                cn.push_back(row_f * (ctx->fs / 2 + 1) + j);
+                ci.push_back(ires);
+                cxy.push_back(pack_xy(j, y));
+            }
+    }
+    {
+        const int sz = ctx->fs, F = sz / 2 + 1;
+        for (int i = 0; i < sz; ++i)
+            for (int j = 0; j < F; ++j) {
+                const int ires = mresol(sz, i, j);
+                if (ires < 0) continue;
+                const int y = (i < F) ? i : i - sz;
+                fn.push_back(i * F + j);
+                fi.push_back(ires);
+                fxy.push_back(pack_xy(j, y));
+            }
+    }
+    ctx->Pc = (int)cn.size();
+    ctx->Pc_pad = (ctx->Pc + 7) / 8 * 8;
+    ctx->Pf = (int)fn.size();
+    if (ctx->Pf > 16 * 1024) return fail(ctx, ROME_ERR_UNSUPPORTED, "current_size too large for the post kernel");
+    ctx->f_nfine_h = fn;
+    ctx->f_xy_h = fxy;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx->c_nfine.upload(cn.data(), cn.size(), s));
+    HIPCHK(ctx->c_ires.upload(ci.data(), ci.size(), s));
+    HIPCHK(ctx->c_xy.upload(cxy.data(), cxy.size(), s));
+    HIPCHK(ctx->f_nfine.upload(fn.data(), fn.size(), s));
+    HIPCHK(ctx->f_ires.upload(fi.data(), fi.size(), s));
+    HIPCHK(ctx->f_xy.upload(fxy.data(), fxy.size(), s));
+    HIPCHK(ctx->grids.reserve(ctx->grid_n * ctx->K));
+    HIPCHK(ctx->bp_data.reserve(2 * ctx->grid_n * ctx->K));
+    HIPCHK(ctx->bp_weight.reserve(ctx->grid_n * ctx->K));
+    HIPCHK(ctx->wsum_sigma2.reserve((size_t)c.nr_groups * ctx->oriF));
+    HIPCHK(ctx->wsum_pdf_class.reserve(ctx->K));
+    HIPCHK(ctx->wsum_pdf_dir.reserve((size_t)ctx->K * ctx->D));
+    HIPCHK(ctx->wsum_sumw_group.reserve(c.nr_groups));
+    HIPCHK(ctx->wsum_prior_x.reserve(ctx->K));
+    HIPCHK(ctx->wsum_prior_y.reserve(ctx->K));
+    HIPCHK(ctx->wsum_scalars.reserve(4));
+    ctx->have_ref.assign(ctx->K, 0);
+    ctx->configured = true;
+    ctx->have_sampling = ctx->have_model = false;
+    ctx->fct_dirty = true;
+    HIPCHK(hipStreamSynchronize(s));
+    return rome_reset_sums(ctx);
+}
+
+int rome_set_sampling(rome_ctx* ctx, const double* pose_coarse, const double* pose_fine, const double* shift_coarse_x,
+                      const double* shift_coarse_y, const double* shift_fine_x, const double* shift_fine_y,
+                      const double* trans_x, const double* trans_y) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (!pose_coarse || !shift_coarse_x || !shift_coarse_y || !trans_x || !trans_y)
+        return fail(ctx, ROME_ERR_ARG, "null sampling table");
+    if (ctx->two_pass && (!pose_fine || !shift_fine_x || !shift_fine_y))
+        return fail(ctx, ROME_ERR_ARG, "fine sampling tables required with oversampling");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int per = ctx->is3d ? 3 : 1;
+    ctx->pose_c.assign(pose_coarse, pose_coarse + (size_t)ctx->R * per);
+    if (ctx->two_pass) ctx->pose_f.assign(pose_fine, pose_fine + (size_t)ctx->R * ctx->NORf * per);
+    else ctx->pose_f.clear();
+    ctx->sxc.assign(shift_coarse_x, shift_coarse_x + ctx->T);
+    ctx->syc.assign(shift_coarse_y, shift_coarse_y + ctx->T);
+    if (ctx->two_pass) {
+        ctx->sxf.assign(shift_fine_x, shift_fine_x + (size_t)ctx->T * 4);
+        ctx->syf.assign(shift_fine_y, shift_fine_y + (size_t)ctx->T * 4);
+    }
+    ctx->tx.assign(trans_x, trans_x + ctx->T);
+    ctx->ty.assign(trans_y, trans_y + ctx->T);
+    int rc = upload_poses(ctx);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx->d_sxc.upload(ctx->sxc.data(), ctx->T, s));
+    HIPCHK(ctx->d_syc.upload(ctx->syc.data(), ctx->T, s));
+    if (ctx->two_pass) {
+        HIPCHK(ctx->d_sxf.upload(ctx->sxf.data(), ctx->sxf.size(), s));
+        HIPCHK(ctx->d_syf.upload(ctx->syf.data(), ctx->syf.size(), s));
+    }
+    HIPCHK(ctx->d_tx.upload(ctx->tx.data(), ctx->T, s));
+    HIPCHK(ctx->d_ty.upload(ctx->ty.data(), ctx->T, s));
+    rc = build_phase_tables(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->have_sampling = true;
+    ctx->fct_dirty = true;
+    return ROME_OK;
+}
+
+int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_class, const double* pdf_direction,
+                   const double* prior_offx, const double* prior_offy) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (!sigma2_noise || !pdf_class || !pdf_direction || !prior_offx || !prior_offy)
+        return fail(ctx, ROME_ERR_ARG, "null model array");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int G = ctx->cfg.nr_groups, K = ctx->K, D = ctx->D;
+    ctx->sigma2_h.assign(sigma2_noise, sigma2_noise + (size_t)G * ctx->oriF);
+    for (double v : ctx->sigma2_h)
+        if (!(v > 0.) || !std::isfinite(v)) return fail(ctx, ROME_ERR_NUMERIC, "sigma2_noise must be positive and finite");
+    ctx->pdf_class_h.assign(pdf_class, pdf_class + K);
+    ctx->pdf_dir_h.assign(pdf_direction, pdf_direction + (size_t)K * D);
+    ctx->prior_x_h.assign(prior_offx, prior_offx + K);
+    ctx->prior_y_h.assign(prior_offy, prior_offy + K);
+    std::vector<double> lp((size_t)K * D);
+    std::vector<unsigned char> valid((size_t)ctx->Mtot);
+    for (int k = 0; k < K; ++k)
+        for (int d = 0; d < D; ++d) {
+            const double p = pdf_direction[(size_t)k * D + d];
+            lp[(size_t)k * D + d] = p > 0 ? log(p) : -INFINITY;
+            const bool ok = pdf_class[k] > 0. && p > 0.;
+            for (int s = 0; s < ctx->PSI; ++s) valid[(size_t)k * ctx->R + d * ctx->PSI + s] = ok;
+        }
+    // logsigma2 per group (storeWeightedSums, src/map3d_optimizer.cpp:1998-2008): same loop order as the reference
+    std::vector<double> ls(G, 0.);
+    const int sz = ctx->fs, F = sz / 2 + 1;
+    for (int g = 0; g < G; ++g) {
+        double acc = 0.;
+        for (int i = 0; i < sz; ++i)
+            for (int j = 0; j < F; ++j) {
+                const int ires = mresol(sz, i, j);
+                if (ires > 0) acc += log(2. * 3.14159265358979323846 * ctx->sigma2_h[(size_t)g * ctx->oriF + ires]);
+            }
+        ls[g] = acc;
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx->sigma2.upload(ctx->sigma2_h.data(), ctx->sigma2_h.size(), s));
+    HIPCHK(ctx->pdf_dir.upload(ctx->pdf_dir_h.data(), ctx->pdf_dir_h.size(), s));
+    HIPCHK(ctx->lpdf_dir.upload(lp.data(), lp.size(), s));
+    HIPCHK(ctx->prior_x.upload(ctx->prior_x_h.data(), K, s));
+    HIPCHK(ctx->prior_y.upload(ctx->prior_y_h.data(), K, s));
+    HIPCHK(ctx->valid.upload(valid.data(), valid.size(), s));
+    HIPCHK(ctx->logsigma2.upload(ls.data(), G, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->have_model = true;
+    return ROME_OK;
+}
+
+int rome_set_reference(rome_ctx* ctx, int iclass, const double* grid) {
+    if (!ctx || !grid) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<float2> h(ctx->grid_n);
+    for (size_t q = 0; q < ctx->grid_n; ++q) h[q] = make_float2((float)grid[2 * q], (float)grid[2 * q + 1]);
+    HIPCHK(hipMemcpyAsync(ctx->grids.p + ctx->grid_n * iclass, h.data(), ctx->grid_n * sizeof(float2),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_ref[iclass] = 1;
+    ctx->fct_dirty = true;
+    return ROME_OK;
+}
+
+int rome_reset_sums(rome_ctx* ctx) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemsetAsync(ctx->bp_data.p, 0, 2 * ctx->grid_n * ctx->K * sizeof(float), s));
+    HIPCHK(hipMemsetAsync(ctx->bp_weight.p, 0, ctx->grid_n * ctx->K * sizeof(float), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_sigma2.p, 0, (size_t)ctx->cfg.nr_groups * ctx->oriF * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_pdf_class.p, 0, ctx->K * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_pdf_dir.p, 0, (size_t)ctx->K * ctx->D * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_sumw_group.p, 0, ctx->cfg.nr_groups * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_prior_x.p, 0, ctx->K * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_prior_y.p, 0, ctx->K * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->wsum_scalars.p, 0, 4 * sizeof(double), s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ROME_OK;
+}
+
+int rome_expectation(rome_ctx* ctx, const rome_images* im, rome_results* res) {
+    if (!ctx || !im) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    const int n = im->n;
+    if (n <= 0) return fail(ctx, ROME_ERR_ARG, "empty image pool");
+    if (!im->fmask_re || !im->fmask_im || !im->fnomask_re || !im->fnomask_im || !im->fctf || !im->highres_Xi2 ||
+        !im->power_img || !im->old_offx || !im->old_offy || !im->norm || !im->group)
+        return fail(ctx, ROME_ERR_ARG, "null image array");
+    HIPCHK(hipSetDevice(ctx->device));
+    for (int i = 0; i < n; ++i)
+        if (im->group[i] < 0 || im->group[i] >= ctx->cfg.nr_groups) return fail(ctx, ROME_ERR_ARG, "group out of range");
+    const size_t F2 = ctx->F2f;
+    std::vector<float2> fm((size_t)n * F2), fn((size_t)n * F2);
+    std::vector<float> fc((size_t)n * F2);
+    for (size_t q = 0; q < (size_t)n * F2; ++q) {
+        if (!std::isfinite(im->fmask_re[q]) || !std::isfinite(im->fmask_im[q]))
+            return fail(ctx, ROME_ERR_NUMERIC, "non-finite value in image %zu", q / F2);
+        fm[q] = make_float2((float)im->fmask_re[q], (float)im->fmask_im[q]);
+        fn[q] = make_float2((float)im->fnomask_re[q], (float)im->fnomask_im[q]);
+        fc[q] = (float)im->fctf[q];
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx->in_fmask.upload(fm.data(), fm.size(), s));
+    HIPCHK(ctx->in_fnomask.upload(fn.data(), fn.size(), s));
+    HIPCHK(ctx->in_fctf.upload(fc.data(), fc.size(), s));
+    HIPCHK(ctx->in_hxi2.upload(im->highres_Xi2, n, s));
+    HIPCHK(ctx->in_power.upload(im->power_img, (size_t)n * ctx->oriF, s));
+    HIPCHK(ctx->in_offx.upload(im->old_offx, n, s));
+    HIPCHK(ctx->in_offy.upload(im->old_offy, n, s));
+    HIPCHK(ctx->in_norm.upload(im->norm, n, s));
+    HIPCHK(ctx->in_group.upload(im->group, n, s));
+    return run_estep(ctx, n, res);
+}
+
+int rome_expectation_device(rome_ctx* ctx, const rome_device_images* im, rome_results* res) {
+    if (!ctx || !im) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    const int n = im->n;
+    if (n <= 0) return fail(ctx, ROME_ERR_ARG, "empty image pool");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t F2 = ctx->F2f;
+    auto d2d = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    HIPCHK(ctx->in_fmask.reserve((size_t)n * F2));
+    HIPCHK(ctx->in_fnomask.reserve((size_t)n * F2));
+    HIPCHK(ctx->in_fctf.reserve((size_t)n * F2));
+    for (auto* b : {&ctx->in_hxi2, &ctx->in_offx, &ctx->in_offy, &ctx->in_norm}) HIPCHK(b->reserve(n));
+    HIPCHK(ctx->in_power.reserve((size_t)n * ctx->oriF));
+    HIPCHK(ctx->in_group.reserve(n));
+    HIPCHK(d2d(ctx->in_fmask.p, im->fmask, (size_t)n * F2 * sizeof(float2)));
+    HIPCHK(d2d(ctx->in_fnomask.p, im->fnomask, (size_t)n * F2 * sizeof(float2)));
+    HIPCHK(d2d(ctx->in_fctf.p, im->fctf, (size_t)n * F2 * sizeof(float)));
+    HIPCHK(d2d(ctx->in_hxi2.p, im->highres_Xi2, n * sizeof(double)));
+    HIPCHK(d2d(ctx->in_power.p, im->power_img, (size_t)n * ctx->oriF * sizeof(double)));
+    HIPCHK(d2d(ctx->in_offx.p, im->old_offx, n * sizeof(double)));
+    HIPCHK(d2d(ctx->in_offy.p, im->old_offy, n * sizeof(double)));
+    HIPCHK(d2d(ctx->in_norm.p, im->norm, n * sizeof(double)));
+    HIPCHK(d2d(ctx->in_group.p, im->group, n * sizeof(int)));
+    return run_estep(ctx, n, res);
+}
+
+int rome_get_sums(rome_ctx* ctx, rome_sums* o) {
+    if (!ctx || !o) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    auto cp = [&](double* h, const double* d, size_t n) -> hipError_t {
+        return h ? hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    const int G = ctx->cfg.nr_groups;
+    HIPCHK(cp(o->wsum_sigma2_noise, ctx->wsum_sigma2.p, (size_t)G * ctx->oriF));
+    HIPCHK(cp(o->wsum_sumw_group, ctx->wsum_sumw_group.p, G));
+    HIPCHK(cp(o->wsum_pdf_class, ctx->wsum_pdf_class.p, ctx->K));
+    HIPCHK(cp(o->wsum_pdf_direction, ctx->wsum_pdf_dir.p, (size_t)ctx->K * ctx->D));
+    HIPCHK(cp(o->wsum_prior_offx, ctx->wsum_prior_x.p, ctx->K));
+    HIPCHK(cp(o->wsum_prior_offy, ctx->wsum_prior_y.p, ctx->K));
+    HIPCHK(cp(o->wsum_scalars, ctx->wsum_scalars.p, 4));
+    HIPCHK(hipStreamSynchronize(s));
+    return ROME_OK;
+}
+
+int rome_get_backprojector(rome_ctx* ctx, int iclass, double* data, double* weight) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<float> d(2 * ctx->grid_n), w(ctx->grid_n);
+    HIPCHK(hipMemcpyAsync(d.data(), ctx->bp_data.p + 2 * ctx->grid_n * iclass, d.size() * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(w.data(), ctx->bp_weight.p + ctx->grid_n * iclass, w.size() * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (data)
+        for (size_t q = 0; q < d.size(); ++q) data[q] = d[q];
+    if (weight)
+        for (size_t q = 0; q < w.size(); ++q) weight[q] = w[q];
+    return ROME_OK;
+}
+
+int rome_get_coarse_diff2(rome_ctx* ctx, double* out) {
+    if (!ctx || !out) return ROME_ERR_ARG;
+    const int n = ctx->last_n;
+    if (n == 0) return fail(ctx, ROME_ERR_ARG, "no E-step has run");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t per = (size_t)ctx->Mtot * ctx->T;
+    std::vector<float> e((size_t)n * per);
+    std::vector<double> c2(n);
+    HIPCHK(hipMemcpy(e.data(), ctx->E.p, e.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c2.data(), ctx->C2c.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> valid(ctx->Mtot);
+    HIPCHK(hipMemcpy(valid.data(), ctx->valid.p, ctx->Mtot, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+        for (size_t q = 0; q < per; ++q)
+            out[(size_t)i * per + q] =
+                valid[q / ctx->T] ? (double)e[(size_t)i * per + q] + c2[i] : std::numeric_limits<double>::quiet_NaN();
+    return ROME_OK;
+}
+
+long long rome_get_fine_count(rome_ctx* ctx) {
+    if (!ctx) return -1;
+    return ctx->last_blocks * (long long)ctx->NORf * ctx->NOTf;
+}
+
+int rome_get_fine_list(rome_ctx* ctx, long long* keys, double* diff2) {
+    if (!ctx) return ROME_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    const long long nb = ctx->last_blocks, nt = ctx->last_tasks;
+    const int NE = ctx->NORf * ctx->NOTf;
+    std::vector<int> bt(nb), btask(nb), tm(nt), ti(nt);
+    std::vector<double> d((size_t)nb * NE);
+    if (nb > 0) {
+        HIPCHK(hipMemcpy(bt.data(), ctx->blk_t.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(btask.data(), ctx->blk_task.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tm.data(), ctx->task_m.p, nt * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ti.data(), ctx->task_img.p, nt * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(d.data(), ctx->D2.p, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (long long b = 0; b < nb; ++b) {
+        const int q = btask[b];
+        const long long base = (((long long)ti[q] * ctx->Mtot + tm[q]) * ctx->T + bt[b]) * NE;
+        for (int e = 0; e < NE; ++e) {
+            if (keys) keys[b * NE + e] = base + e;
+            if (diff2) diff2[b * NE + e] = d[(size_t)b * NE + e];
+        }
+    }
+    return ROME_OK;
+}
+
+int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, double* out_re, double* out_im) {
+    if (!ctx || !A9 || nrot < 0) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K || !ctx->have_ref[iclass]) return fail(ctx, ROME_ERR_ARG, "class not set");
+    if (size <= 0 || (size & 1) || size / 2 > ctx->r_max) return fail(ctx, ROME_ERR_ARG, "bad projection size");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<double> ai(6 * (size_t)std::max(nrot, 1));
+    for (int r = 0; r < nrot; ++r) rot6(A9 + 9 * (size_t)r, &ai[6 * r]);
+    DevBuf<double> dai;
+    DevBuf<float2> dout;
+    const size_t F2 = (size_t)size * (size / 2 + 1);
+    HIPCHK(dai.upload(ai.data(), ai.size(), ctx->stream));
+    HIPCHK(dout.reserve(F2 * std::max(nrot, 1)));
+    HIPCHK(launch_project_plane(ctx->stream, ctx->grids.p + ctx->grid_n * iclass, ctx->pad, ctx->padF, ctx->cgrid,
+                                ctx->is3d, dai.p, nrot, size, std::min(ctx->r_max, size / 2), dout.p));
+    std::vector<float2> h(F2 * nrot);
+    HIPCHK(hipMemcpyAsync(h.data(), dout.p, h.size() * sizeof(float2), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q < h.size(); ++q) {
+        if (out_re) out_re[q] = h[q].x;
+        if (out_im) out_im[q] = h[q].y;
+    }
+    return ROME_OK;
+}
+
+int rome_sampling_pose3d(int order, int oversampling, double perturbation, double* out) {
+    return out ? pose_table3d(order, oversampling, perturbation, out) : ROME_ERR_ARG;
+}
+int rome_sampling_pose2d(double psi_step, int oversampling, double perturbation, double* out) {
+    return out ? pose_table2d(psi_step, oversampling, perturbation, out) : ROME_ERR_ARG;
+}
+int rome_sampling_translations(double range, double step, double* tx, double* ty) {
+    return translations(range, step, tx, ty);
+}
+int rome_sampling_shifts(double range, double step, int oversampling, double perturbation, double* ox, double* oy) {
+    return (ox && oy) ? shift_table(range, step, oversampling, perturbation, ox, oy) : ROME_ERR_ARG;
+}
+int rome_healpix_pix2ang(int order, int pix, double* z, double* phi) {
+    if (!z || !phi || order < 0 || order > 13 || pix < 0 || pix >= (12 << (2 * order))) return ROME_ERR_ARG;
+    healpix_pix2ang(order, pix, *z, *phi);
+    return ROME_OK;
+}
+
+}  // extern "C"

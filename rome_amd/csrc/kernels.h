@@ -1,0 +1,188 @@
+// Kernel argument blocks and launch wrappers of the E-step engine (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rome_common.h"
+
+namespace rome {
+
+constexpr int kMaxKT = 1024;    // classes x translations held in LDS by the weight kernels
+constexpr int kMaxT = 128;      // translations per task held in LDS by the post kernel
+constexpr int kMaxShell = 512;  // ori_size/2+1 upper bound
+constexpr int kMaxK = 64;       // classes
+
+struct PrepArgs {
+    // inputs
+    const float2* fmask;    // [n][F2f]
+    const float2* fnomask;  // [n][F2f]
+    const float* fctf;      // [n][F2f]
+    const int* group;       // [n]
+    const double* hxi2;     // [n]
+    const double* sigma2;   // [G][ori_Fsize]
+    int ori_Fsize, F2f;
+    double sigma2_fudge;
+    int ctf_in_diff, do_ctf, do_map, have_fine;
+    // coarse pixel set
+    int Pc, Pc_pad, T, Npad;
+    const int* c_nfine;
+    const int* c_ires;
+    const float2* Sc;       // [T][Pc]
+    // fine pixel set
+    int Pf;
+    const int* f_nfine;
+    const int* f_ires;
+    // outputs
+    float* WcT;             // [2*Pc_pad][Npad]
+    float* MC2c;            // [n][Pc]
+    double* C2c;            // [n]  1/2 (sum Minv|X|^2 + hxi2), coarse
+    float2* Xf;             // [n][Pf]
+    float* Minvf;           // [n][Pf]
+    float* Ctff;            // [n][Pf]
+    double* C2f;            // [n]
+    float2* NMf;            // [n][Pf]
+    float* MC2bp;           // [n][Pf]
+};
+
+struct GemmArgs {
+    const float* A;   // [K][Mpad]
+    const float* B;   // [K][Npad]
+    const float* At;  // [n][Mtot]
+    float* E;         // [n][Mtot][T]
+    int K, Mpad, Npad, Mtot, Ncols, T;
+};
+
+struct WeightParams {
+    int K, D, PSI, R, Mtot, T;
+    const double* pdf_dir;   // [K][D]
+    const double* lpdf_dir;  // [K][D] log (or -inf)
+    const double* trans_x;   // [T] unperturbed translations
+    const double* trans_y;
+    const double* old_offx;  // [n]
+    const double* old_offy;
+    const double* prior_x;   // [K]
+    const double* prior_y;
+    double sigma2_offset, adaptive_fraction;
+};
+
+struct WeightOutArgs {
+    double* dmin;
+    double* sumw;
+    double* maxw;
+    long long* amax;
+    double* thr;
+    int* nsig;
+    int* status;
+};
+
+struct FineArgs {
+    const int* task_img;
+    const int* task_m;
+    const long long* task_blk0;
+    const int* task_nt;
+    const int* blk_t;
+    const int* m2u;
+    const float2* Pfine;  // [U][NOR][Pf]
+    const float2* Xf;
+    const float* Minvf;
+    const float* Ctff;
+    const float2* Sf;     // [T*NOT][Pf]
+    const double* C2f;
+    double* D2;           // [blocks][NOR*NOT]
+    int Pf;
+};
+
+struct PostArgs {
+    int K, D, PSI, R, is3d, Pf, oriF, pad, padF, c, r_max;
+    size_t grid_stride;
+    const int* group;
+    const double* sumw;
+    const double* thr;
+    const double* dmin;
+    const long long* task_off;  // [n+1]
+    const int* task_m;
+    const int* task_nt;
+    const long long* task_blk0;
+    const int* blk_t;
+    const int* m2u;
+    const double* D2;
+    const double* pdf_dir;
+    const double* trans_x;
+    const double* trans_y;
+    const double* shift_x;      // [T*NOT] last-pass shifts
+    const double* shift_y;
+    const double* old_offx;
+    const double* old_offy;
+    const double* prior_x;
+    const double* prior_y;
+    double sigma2_offset;
+    const double* Ai_fine;      // [R][NOR][6]
+    const float2* Pfine;
+    const float2* Xf;
+    const float* Ctff;
+    const float2* NMf;
+    const float* MC2bp;
+    const float2* Sf;
+    const int* f_xy;
+    const int* f_ires;
+    // outputs
+    float* bp_data;             // [K][grid][2]
+    float* bp_weight;           // [K][grid]
+    double* wsum_sigma2;        // [G][oriF]
+    double* wsum_pdf_class;     // [K]
+    double* wsum_pdf_dir;       // [K][D]
+    double* wsum_sumw_group;    // [G]
+    double* wsum_prior_x;       // [K]
+    double* wsum_prior_y;
+    double* wsum_scalars;       // [4]
+    double* wnorm;              // [n]
+};
+
+struct StoreArgs {
+    int n, curF, oriF, do_norm;
+    double avg_norm;
+    const int* group;
+    const double* power;      // [n][oriF]
+    const double* norm;       // [n]
+    const double* sumw;
+    const double* dmin;
+    const double* logsigma2;  // [G]
+    double* wnorm;            // [n] in/out
+    double* meta;             // [n][9]
+    double* wsum_sigma2;
+    double* wsum_scalars;
+};
+
+hipError_t launch_phase_table(hipStream_t s, const double* sx, const double* sy, int nshift, const int* xy, int P, int ori,
+                              const double* sintab, const double* costab, float2* out);
+hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
+                                 const double* Ai, int R, int Mtot, const int* xy, int Pc, int Pc_pad, int my_r_max,
+                                 int Mpad, float* FcT);
+hipError_t launch_project_fine(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
+                               const double* Ai_fine, int R, int NOR, const int* ulist, int U, const int* xy, int Pf,
+                               int r_max, float2* Pfine);
+hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
+                                int nrot, int size, int my_r_max, float2* out);
+hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a);
+hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
+                               const float* MC2c, float* At);
+hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a);
+hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const unsigned char* valid,
+                                 const double* C2c, const WeightOutArgs& wo, uint32_t* sigmask, int TW);
+hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
+                               const int* blk_t, const int* task_m, const long long* img_blk, int NE,
+                               const WeightOutArgs& wo);
+hipError_t launch_task_build(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, int* ntask, int* nblk,
+                             long long* task_off, long long* blk_off);
+hipError_t launch_task_emit(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, const long long* task_off,
+                            const long long* blk_off, int* task_img, int* task_m, long long* task_blk0, int* task_nt,
+                            int* blk_task, int* blk_t, unsigned char* used, int* ulist, int* m2u, int* ucount);
+hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a);
+hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a);
+hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
+                          long long nblocks, const float* E, const double* C2c, int M, int T, double* D2);
+hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
+                                const int* blk_t, const int* task_m, int n, int NE, int2* out);
+hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
+
+}  // namespace rome

@@ -1,0 +1,1334 @@
+// CDNA4 (gfx950) kernels of the MAP E-step (ROME src/map3d_optimizer.cpp:883-2504,
+// src/map2d_optimizer.cpp:802-1873) and the Fourier projector/backprojector
+// (src/reconstruct.cpp:412-1164, 823-958, 1353-1499).
+//
+// Data layout in HBM (all per E-step batch unless noted):
+//   grid    [K][pad][pad][pad/2+1] float2   projector reference (per iteration)
+//   FcT     [2*Pc_pad][M_pad] float          coarse projections, K-major GEMM operand
+//   WcT     [2*Pc_pad][N_pad] float          CTF*Minv weighted shifted images, N = img*T + t
+//   E       [n][M][T] float                  coarse pose term  1/2*sum Minv|CF|^2 - sum Minv Re(CF conj(XS))
+//   Pfine   [U][NOR][Pf] float2              fine projections of the rotations any image needs
+//   D2      [blocks][NOR*NOT] double         fine squared differences, block = (task, significant t)
+// diff2 = E + 1/2 (sum Minv |X|^2 + highres_Xi2): the per-image constant is added in FP64.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace rome {
+
+#define DEV __device__ __forceinline__
+
+// ------------------------------------------------------------------ helpers
+// Separately rounded FP64 multiply / add: the reference's host arithmetic has no FMA,
+// and table indices / floor() decisions depend on the last bit.
+DEV double dmul(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+DEV double dadd(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+DEV float linf(float a, float l, float h) { return l + (h - l) * a; }
+
+struct Coords {
+    int x0, y0, z0;
+    float fx, fy, fz;
+    bool neg;
+};
+
+// Logical -> grid coordinates exactly as MyProjector::projectOneTile computes them
+// (src/reconstruct.cpp:517-560), FP64 without contraction so that floor() agrees.
+DEV Coords map_coords(const double* Ai, int x, int y, int c, bool is3d) {
+#pragma clang fp contract(off)
+    double xp = dadd(dmul(Ai[0], (double)x), dmul(Ai[1], (double)y));
+    double yp = dadd(dmul(Ai[2], (double)x), dmul(Ai[3], (double)y));
+    double zp = is3d ? dadd(dmul(Ai[4], (double)x), dmul(Ai[5], (double)y)) : 0.0;
+    Coords r;
+    r.neg = xp < 0;
+    if (r.neg) {
+        xp = -xp;
+        yp = -yp;
+        zp = -zp;
+    }
+    double fx0 = floor(xp), fy0 = floor(yp), fz0 = floor(zp);
+    r.x0 = (int)fx0;
+    r.y0 = (int)fy0 + c;
+    r.z0 = (int)fz0 + c;
+    r.fx = (float)(xp - fx0);
+    r.fy = (float)(yp - fy0);
+    r.fz = (float)(zp - fz0);
+    return r;
+}
+
+// Trilinear (3D) / bilinear (2D) gather of the half-transform with Friedel mirror.
+DEV float2 gather(const float2* __restrict__ g, int pad, int padF, const Coords& q, bool is3d) {
+    float2 out;
+    if (is3d) {
+        const float2* p00 = g + ((size_t)q.z0 * pad + q.y0) * padF + q.x0;
+        const float2* p01 = p00 + padF;                  // (z0, y1)
+        const float2* p10 = p00 + (size_t)pad * padF;    // (z1, y0)
+        const float2* p11 = p10 + padF;                  // (z1, y1)
+        float2 d000 = p00[0], d001 = p00[1], d010 = p01[0], d011 = p01[1];
+        float2 d100 = p10[0], d101 = p10[1], d110 = p11[0], d111 = p11[1];
+        float dx00r = linf(q.fx, d000.x, d001.x), dx00i = linf(q.fx, d000.y, d001.y);
+        float dx01r = linf(q.fx, d100.x, d101.x), dx01i = linf(q.fx, d100.y, d101.y);
+        float dx10r = linf(q.fx, d010.x, d011.x), dx10i = linf(q.fx, d010.y, d011.y);
+        float dx11r = linf(q.fx, d110.x, d111.x), dx11i = linf(q.fx, d110.y, d111.y);
+        float dxy0r = linf(q.fy, dx00r, dx10r), dxy0i = linf(q.fy, dx00i, dx10i);
+        float dxy1r = linf(q.fy, dx01r, dx11r), dxy1i = linf(q.fy, dx01i, dx11i);
+        out.x = linf(q.fz, dxy0r, dxy1r);
+        out.y = linf(q.fz, dxy0i, dxy1i);
+    } else {
+        const float2* p0 = g + (size_t)q.y0 * padF + q.x0;
+        const float2* p1 = p0 + padF;
+        float2 d00 = p0[0], d01 = p0[1], d10 = p1[0], d11 = p1[1];
+        float dx0r = linf(q.fx, d00.x, d01.x), dx0i = linf(q.fx, d00.y, d01.y);
+        float dx1r = linf(q.fx, d10.x, d11.x), dx1i = linf(q.fx, d10.y, d11.y);
+        out.x = linf(q.fy, dx0r, dx1r);
+        out.y = linf(q.fy, dx0i, dx1i);
+    }
+    if (q.neg) out.y = -out.y;
+    return out;
+}
+
+DEV int px_x(int xy) { return xy & 0xffff; }
+DEV int px_y(int xy) { return (int)(short)(xy >> 16); }
+
+DEV float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+template <typename T>
+DEV T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+DEV double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// block-wide reduction over blockDim.x threads (multiple of 64), result broadcast
+template <typename T, typename Op>
+DEV T block_reduce(T v, Op op, T* scratch) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    T r = scratch[0];
+    for (int w = 1; w < nw; ++w) r = op(r, scratch[w]);
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------ phase tables
+// Tabulated shift phase of shiftImageInFourierTransform (src/image.h:1478-1574):
+// index = (int)(|2 pi (x xs + y ys)| / (2 pi / 5000)) % 5000, xs = shift_x / (-ori).
+__global__ void k_phase_table(const double* __restrict__ sx, const double* __restrict__ sy, int nshift,
+                              const int* __restrict__ xy, int P, int ori, const double* __restrict__ sintab,
+                              const double* __restrict__ costab, float2* __restrict__ out) {
+#pragma clang fp contract(off)
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nshift * P) return;
+    const int s = (int)(idx / P), p = (int)(idx % P);
+    const double xs = __ddiv_rn(sx[s], (double)(-ori));
+    const double ys = __ddiv_rn(sy[s], (double)(-ori));
+    float2 r;
+    if (fabs(xs) < 1e-6 && fabs(ys) < 1e-6) {
+        r = make_float2(1.f, 0.f);
+    } else {
+        const double twopi = 2.0 * 3.14159265358979323846;
+        const double sampling = twopi / 5000.0;
+        const double x = (double)px_x(xy[p]), y = (double)px_y(xy[p]);
+        const double dotp = dmul(twopi, dadd(dmul(x, xs), dmul(y, ys)));
+        const int it = ((int)__ddiv_rn(fabs(dotp), sampling)) % 5000;
+        double a = costab[it], b = sintab[it];
+        if (dotp < 0) b = -b;
+        r = make_float2((float)a, (float)b);
+    }
+    out[idx] = r;
+}
+
+// ------------------------------------------------------------------ projections
+// Coarse projection table (image independent): FcT[p][m] = Re, FcT[Pc_pad+p][m] = Im
+// for rotation m = k*R + r.  Pixels outside r <= size/2 stay zero (projectOneTile).
+__global__ void k_project_coarse(const float2* __restrict__ grids, size_t grid_stride, int pad, int padF, int c,
+                                 int is3d, const double* __restrict__ Ai, int R, int Mtot, const int* __restrict__ xy,
+                                 int Pc, int Pc_pad, int my_r_max, int Mpad, float* __restrict__ FcT) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (m >= Mtot) return;
+    const int k = m / R, r = m % R;
+    const int x = px_x(xy[p]), y = px_y(xy[p]);
+    float2 v = make_float2(0.f, 0.f);
+    if (x * x + y * y <= my_r_max * my_r_max) {
+        Coords q = map_coords(Ai + 6 * (size_t)r, x, y, c, is3d != 0);
+        v = gather(grids + grid_stride * k, pad, padF, q, is3d != 0);
+    }
+    FcT[(size_t)p * Mpad + m] = v.x;
+    FcT[(size_t)(Pc_pad + p) * Mpad + m] = v.y;
+}
+
+// Fine projections for the unique rotations the batch needs: Pf[u][o][p].
+__global__ void k_project_fine(const float2* __restrict__ grids, size_t grid_stride, int pad, int padF, int c, int is3d,
+                               const double* __restrict__ Ai_fine, int R, int NOR, const int* __restrict__ ulist,
+                               const int* __restrict__ xy, int Pf, int r_max, float2* __restrict__ Pfine) {
+    const int p = blockIdx.y * blockDim.x + threadIdx.x;
+    const int uo = blockIdx.x;  // U*NOR can exceed the 65535 limit of gridDim.y
+    if (p >= Pf) return;
+    const int u = uo / NOR, o = uo % NOR;
+    const int m = ulist[u];
+    const int k = m / R, r = m % R;
+    const int x = px_x(xy[p]), y = px_y(xy[p]);
+    float2 v = make_float2(0.f, 0.f);
+    if (x * x + y * y <= r_max * r_max) {
+        Coords q = map_coords(Ai_fine + 6 * ((size_t)r * NOR + o), x, y, c, is3d != 0);
+        v = gather(grids + grid_stride * k, pad, padF, q, is3d != 0);
+    }
+    Pfine[((size_t)u * NOR + o) * Pf + p] = v;
+}
+
+// Standalone projection of arbitrary rotations into full half-planes (parity tests and
+// synthetic data): out[i][n] for rotation i, n over size*(size/2+1).
+__global__ void k_project_plane(const float2* __restrict__ grid, int pad, int padF, int c, int is3d,
+                                const double* __restrict__ Ai, int size, int my_r_max, float2* __restrict__ out) {
+    const int Fsize = size / 2 + 1;
+    const int n = blockIdx.y * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
+    if (n >= size * Fsize) return;
+    const int row = n / Fsize, x = n % Fsize;
+    int y;
+    float2 v = make_float2(0.f, 0.f);
+    bool ok = true;
+    if (row <= my_r_max) y = row;
+    else if (row >= size - my_r_max) y = row - size;
+    else ok = false;
+    if (ok && x <= my_r_max && x * x + y * y <= my_r_max * my_r_max) {
+        Coords q = map_coords(Ai + 6 * (size_t)i, x, y, c, is3d != 0);
+        v = gather(grid, pad, padF, q, is3d != 0);
+    }
+    out[(size_t)i * size * Fsize + n] = v;
+}
+
+// ------------------------------------------------------------------ image preparation (K6)
+// One block per image.  Windows the masked image to the coarse size, builds the
+// CTF-Minv weighted shifted images (GEMM operand), the coarse norm weights and the
+// FP64 image norm; gathers the fine pixel set (masked, unmasked*Minv*CTF for the
+// backprojector).  src/exp_model.cpp:935-1103, src/map3d_optimizer.cpp:2246-2308.
+__global__ void k_prep_images(PrepArgs a) {
+    const int img = blockIdx.x;
+    const int g = a.group[img];
+    const double* s2 = a.sigma2 + (size_t)g * a.ori_Fsize;
+    const float2* fm = a.fmask + (size_t)img * a.F2f;
+    const float2* fn = a.fnomask + (size_t)img * a.F2f;
+    const float* fc = a.fctf + (size_t)img * a.F2f;
+    __shared__ double red[16];
+    // coarse
+    double x2 = 0.0;
+    for (int p = threadIdx.x; p < a.Pc; p += blockDim.x) {
+        const int nf = a.c_nfine[p];
+        const float2 X = fm[nf];
+        const double minv = 1.0 / (a.sigma2_fudge * s2[a.c_ires[p]]);
+        const float ctf = a.ctf_in_diff ? fc[nf] : 1.f;
+        const float mc = (float)minv * ctf;
+        a.MC2c[(size_t)img * a.Pc + p] = mc * ctf;
+        x2 += minv * ((double)X.x * X.x + (double)X.y * X.y);
+        for (int t = 0; t < a.T; ++t) {
+            const float2 S = a.Sc[(size_t)t * a.Pc + p];
+            const float2 XS = cmul(X, S);
+            const size_t col = (size_t)img * a.T + t;
+            a.WcT[(size_t)p * a.Npad + col] = mc * XS.x;
+            a.WcT[(size_t)(a.Pc_pad + p) * a.Npad + col] = mc * XS.y;
+        }
+    }
+    double s = block_reduce(x2, [](double u, double v) { return u + v; }, red);
+    if (threadIdx.x == 0) a.C2c[img] = 0.5 * (s + a.hxi2[img]);
+    if (!a.have_fine) return;
+    // fine
+    double x2f = 0.0;
+    for (int p = threadIdx.x; p < a.Pf; p += blockDim.x) {
+        const int nf = a.f_nfine[p];
+        const int ires = a.f_ires[p];
+        const float2 X = fm[nf];
+        const double minv = ires > 0 ? 1.0 / (a.sigma2_fudge * s2[ires]) : 0.0;
+        const float ctf_d = a.ctf_in_diff ? fc[nf] : 1.f;
+        const size_t o = (size_t)img * a.Pf + p;
+        a.Xf[o] = X;
+        a.Minvf[o] = (float)minv;
+        a.Ctff[o] = ctf_d;
+        x2f += minv * ((double)X.x * X.x + (double)X.y * X.y);
+        // backprojection inputs: Minv with the origin restored, do_map / do_ctf rules
+        double mbp = (ires > 0) ? minv : (ires == 0 ? 1.0 / (a.sigma2_fudge * s2[0]) : 0.0);
+        if (!a.do_map) mbp = 1.0;
+        const float ctf_b = a.do_ctf ? fc[nf] : 1.f;
+        const float mc = (float)mbp * ctf_b;
+        const float2 N = fn[nf];
+        a.NMf[o] = make_float2(N.x * mc, N.y * mc);
+        a.MC2bp[o] = mc * ctf_b;
+    }
+    s = block_reduce(x2f, [](double u, double v) { return u + v; }, red);
+    if (threadIdx.x == 0) a.C2f[img] = 0.5 * (s + a.hxi2[img]);
+}
+
+// A-term: At[img][m] = 0.5 * sum_p MC2c[img][p] |F_m(p)|^2
+__global__ void k_coarse_aterm(const float* __restrict__ FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
+                               const float* __restrict__ MC2c, float* __restrict__ At) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int img = blockIdx.y;
+    if (m >= Mtot) return;
+    const float* w = MC2c + (size_t)img * Pc;
+    float acc = 0.f;
+    for (int p = 0; p < Pc; ++p) {
+        const float fr = FcT[(size_t)p * Mpad + m], fi = FcT[(size_t)(Pc_pad + p) * Mpad + m];
+        acc += w[p] * (fr * fr + fi * fi);
+    }
+    At[(size_t)img * Mtot + m] = 0.5f * acc;
+}
+
+// ------------------------------------------------------------------ coarse cross term (K2), FP32 MFMA
+// C[m][j] = sum_kk FcT[kk][m] * WcT[kk][j];  E[img][m][t] = At[img][m] - C[m][img*T+t].
+// 256 threads, 128x128 tile, BK = 16, v_mfma_f32_32x32x2f32, LDS double buffer.
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
+    constexpr int BM = 128, BN = 128, BK = 16;
+    __shared__ float As[2][BK][BM];
+    __shared__ float Bs[2][BK][BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // XCD-aware tile order: consecutive workgroups go round-robin over the 8 XCDs, so
+    // remap such that each XCD owns a contiguous band of tiles (shared operand panels).
+    const int nbm = a.Mpad / BM, nbn = a.Npad / BN, ntiles = nbm * nbn;
+    int bid = blockIdx.x;
+    if ((ntiles & 7) == 0) {  // bijection: bid = 8*loc + xcd  ->  xcd*(ntiles/8) + loc
+        const int per = ntiles >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    const int bm = bid % nbm, bn = bid / nbm;
+    const int m0 = bm * BM, n0 = bn * BN;
+    // global load mapping: 16 rows x 128 cols = 2048 floats = 512 float4 -> 2 per thread per operand
+    const int lr = tid >> 5, lc = (tid & 31) * 4;  // rows lr and lr+8
+    f16v acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    const int nk = a.K / BK;
+    auto load = [&](int kc, float4 (&ra)[2], float4 (&rb)[2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = kc * BK + lr + 8 * h;
+            ra[h] = *reinterpret_cast<const float4*>(a.A + (size_t)row * a.Mpad + m0 + lc);
+            rb[h] = *reinterpret_cast<const float4*>(a.B + (size_t)row * a.Npad + n0 + lc);
+        }
+    };
+    auto store = [&](int buf, const float4 (&ra)[2], const float4 (&rb)[2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            *reinterpret_cast<float4*>(&As[buf][lr + 8 * h][lc]) = ra[h];
+            *reinterpret_cast<float4*>(&Bs[buf][lr + 8 * h][lc]) = rb[h];
+        }
+    };
+    float4 ra[2], rb[2];
+    load(0, ra, rb);
+    store(0, ra, rb);
+    __syncthreads();
+    for (int kc = 0; kc < nk; ++kc) {
+        const int cur = kc & 1;
+        if (kc + 1 < nk) load(kc + 1, ra, rb);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 2) {
+            float av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = As[cur][kk + (lane >> 5)][wm * 64 + i * 32 + (lane & 31)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[cur][kk + (lane >> 5)][wn * 64 + j * 32 + (lane & 31)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kc + 1 < nk) store(cur ^ 1, ra, rb);
+        __syncthreads();
+    }
+    // epilogue: lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+            if (col >= a.Ncols) continue;
+            const int img = col / a.T, t = col - img * a.T;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row < a.Mtot) {
+                    const float at = a.At[(size_t)img * a.Mtot + row];
+                    a.E[((size_t)img * a.Mtot + row) * a.T + t] = at - acc[i][j][r];
+                }
+            }
+        }
+}
+
+// ------------------------------------------------------------------ weights + significance (K3)
+// One block (1024 threads) per image; see src/map3d_optimizer.cpp:1610-1931.
+// Entries are visited through an accessor; both passes share the code.
+struct CoarseAcc {
+    const float* E;    // [n][M][T]
+    const unsigned char* valid;
+    int M, T;
+    DEV size_t count() const { return (size_t)M * T; }
+    DEV bool ok(int, size_t e) const { return valid[e / T] != 0; }
+    DEV double d(int img, size_t e) const { return (double)E[(size_t)img * M * T + e]; }
+    DEV int m_of(int, size_t e) const { return (int)(e / T); }
+    DEV int t_of(int, size_t e) const { return (int)(e % T); }
+};
+struct FineAcc {
+    const double* D2;           // [blocks][NE]
+    const int* blk_task;        // block -> task
+    const int* blk_t;           // block -> t
+    const int* task_m;          // task -> m
+    const long long* img_blk;   // [n+1] block offsets
+    int NE;
+    DEV size_t count_img(int img) const { return (size_t)(img_blk[img + 1] - img_blk[img]) * NE; }
+    DEV size_t base(int img) const { return (size_t)img_blk[img] * NE; }
+};
+
+struct WeightOut {
+    double* dmin;    // [n] min diff2
+    double* sumw;    // [n]
+    double* maxw;    // [n]
+    long long* amax; // [n] argmax entry (pass-local index)
+    double* thr;     // [n] significance threshold
+    int* nsig;       // [n] number of significant samples (coarse pass)
+    int* status;     // [n] 0 ok, 1 zero significant, 2 unresolved tie bucket
+};
+
+DEV double pdf_offset(double tx, double ty, double ox, double oy, double px, double py, double s2off) {
+    const double offx = tx + ox, offy = ty + oy;
+    const double td = (offx - px) * (offx - px) + (offy - py) * (offy - py);
+    if (s2off < 0.0001) return td > 0. ? 0. : 1.;
+    return exp(td / (-2. * s2off)) / (2. * 3.14159265358979323846 * s2off);
+}
+
+template <int NT>
+struct WShared {
+    double poff[kMaxKT];
+    double lpoff[kMaxKT];
+    double hsum[kBinCount];
+    unsigned int hcnt[kBinCount];
+    double cw[kSortCap];
+    long long ce[kSortCap];
+    double red_d[NT / 64];
+    long long red_l[NT / 64];
+    int ncand;
+    double lo, width;
+    double mbefore;
+    long long cbefore;
+    int done;
+};
+
+// Generic per-image weight pass.  get(e, &valid, &d, &k, &t, &dir) describes entry e.
+template <int NT, typename Get>
+DEV void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, WeightOut wo,
+                     WShared<NT>& sh, bool coarse_pass, uint32_t* sigmask, int T, int TW) {
+    const int tid = threadIdx.x;
+    // pdf_offset table per (k, t) for this image (src/map3d_optimizer.cpp:1700-1716)
+    for (int q = tid; q < wp.K * wp.T; q += NT) {
+        const int k = q / wp.T, t = q % wp.T;
+        const double po = pdf_offset(wp.trans_x[t], wp.trans_y[t], wp.old_offx[img], wp.old_offy[img],
+                                     wp.prior_x[k], wp.prior_y[k], wp.sigma2_offset);
+        sh.poff[q] = po;
+        sh.lpoff[q] = po > 0 ? log(po) : -INFINITY;
+    }
+    for (int b = tid; b < kBinCount; b += NT) {
+        sh.hsum[b] = 0.;
+        sh.hcnt[b] = 0;
+    }
+    __syncthreads();
+    // pass 1: minimum of diff2 and of u = d - ln(pdf) (histogram origin)
+    double mn = INFINITY, umin = INFINITY;
+    for (size_t e = tid; e < count; e += NT) {
+        bool v;
+        double d;
+        int k, t, dir;
+        get(e, v, d, k, t, dir);
+        if (!v) continue;
+        mn = fmin(mn, d);
+        const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
+        umin = fmin(umin, d - lp);
+    }
+    mn = block_reduce(mn, [](double a, double b) { return fmin(a, b); }, sh.red_d);
+    umin = block_reduce(umin, [](double a, double b) { return fmin(a, b); }, sh.red_d);
+    const double dmin = mn + cimg;  // min over (d + c) is (min d) + c: rounding is monotone
+    // pass 2: weights, sum, max / argmax (first in visiting order), level-1 histogram
+    double sum = 0., mx = -1.;
+    long long am = -1;
+    for (size_t e = tid; e < count; e += NT) {
+        bool v;
+        double d;
+        int k, t, dir;
+        get(e, v, d, k, t, dir);
+        if (!v) continue;
+        const double dd = (d + cimg) - dmin;
+        double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
+        w = dd > 700. ? 0. : w * exp(-dd);
+        sum += w;
+        if (w > mx) {  // entries are visited in increasing e per thread: keeps the first maximum
+            mx = w;
+            am = (long long)e;
+        }
+        if (w > 0.) {
+            const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
+            const double u = (d - lp) - umin;
+            const double bf = u * kBinScale;
+            if (bf < (double)kBinCount) {
+                const int b = (int)bf;
+                atomicAdd(&sh.hsum[b], w);
+                atomicAdd(&sh.hcnt[b], 1u);
+            }
+        }
+    }
+    sum = block_reduce(sum, [](double a, double b) { return a + b; }, sh.red_d);
+    // argmax: max weight, ties -> smallest entry index (first in the reference's loop order)
+    {
+        double m2 = block_reduce(mx, [](double a, double b) { return fmax(a, b); }, sh.red_d);
+        long long cand = (mx == m2 && am >= 0) ? am : (long long)0x7fffffffffffffffLL;
+        long long e2 = block_reduce(cand, [](long long a, long long b) { return a < b ? a : b; }, sh.red_l);
+        mx = m2;
+        am = e2;
+    }
+    const double target = wp.adaptive_fraction * sum;
+    // significance threshold: find crossing bucket, refine until it fits the sort buffer
+    if (tid == 0) {
+        sh.lo = 0.;
+        sh.width = 1.0 / kBinScale;
+        sh.mbefore = 0.;
+        sh.cbefore = 0;
+        sh.done = 0;
+    }
+    __syncthreads();
+    int level = 0;
+    double thr = 0.;
+    long long nsig = 0;
+    int status = 0;
+    for (;;) {
+        // scan buckets (thread 0)
+        if (tid == 0) {
+            double cum = sh.mbefore;
+            long long cnt = sh.cbefore;
+            int bstar = -1;
+            int last_nonempty = -1;
+            for (int b = 0; b < kBinCount; ++b) {
+                if (sh.hcnt[b] == 0) continue;
+                last_nonempty = b;
+                if (cum + sh.hsum[b] > target) {
+                    bstar = b;
+                    break;
+                }
+                cum += sh.hsum[b];
+                cnt += sh.hcnt[b];
+            }
+            if (bstar < 0) {  // rounding: crossing not reached -> use the last bucket
+                bstar = last_nonempty;
+                if (bstar >= 0) {
+                    cum -= sh.hsum[bstar];
+                    cnt -= sh.hcnt[bstar];
+                }
+            }
+            if (bstar < 0) {
+                sh.done = 2;  // no positive weight
+            } else {
+                sh.mbefore = cum;
+                sh.cbefore = cnt;
+                sh.lo = sh.lo + bstar * sh.width;
+                sh.done = (sh.hcnt[bstar] <= (unsigned)kSortCap || level >= 2) ? 1 : 0;
+                sh.width = sh.width / (double)kBinCount;
+            }
+            sh.ncand = 0;
+        }
+        __syncthreads();
+        if (sh.done == 2) {
+            status = 1;
+            break;
+        }
+        const double lo = sh.lo;
+        const double wbin = sh.width * kBinCount;  // width of the selected bucket
+        const double hi = lo + wbin;
+        if (sh.done == 1) {
+            // collect candidates of the selected bucket
+            for (size_t e = tid; e < count; e += NT) {
+                bool v;
+                double d;
+                int k, t, dir;
+                get(e, v, d, k, t, dir);
+                if (!v) continue;
+                const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
+                const double u = (d - lp) - umin;
+                if (!(u >= lo && u < hi)) continue;
+                const double dd = (d + cimg) - dmin;
+                double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
+                w = dd > 700. ? 0. : w * exp(-dd);
+                if (!(w > 0.)) continue;
+                const int slot = atomicAdd(&sh.ncand, 1);
+                if (slot < kSortCap) {
+                    sh.cw[slot] = w;
+                    sh.ce[slot] = (long long)e;
+                }
+            }
+            __syncthreads();
+            int nc = sh.ncand < kSortCap ? sh.ncand : kSortCap;
+            if (sh.ncand > kSortCap) status = 2;
+            // bitonic sort descending by weight over the next power of two
+            int np2 = 1;
+            while (np2 < nc) np2 <<= 1;
+            for (int q = nc + tid; q < np2; q += NT) {
+                sh.cw[q] = -1.0;
+                sh.ce[q] = -1;
+            }
+            __syncthreads();
+            for (int ksz = 2; ksz <= np2; ksz <<= 1)
+                for (int j = ksz >> 1; j > 0; j >>= 1) {
+                    for (int q = tid; q < np2; q += NT) {
+                        const int ixj = q ^ j;
+                        if (ixj > q) {
+                            const bool desc = ((q & ksz) == 0);
+                            const double a1 = sh.cw[q], a2 = sh.cw[ixj];
+                            if (desc ? (a1 < a2) : (a1 > a2)) {
+                                sh.cw[q] = a2;
+                                sh.cw[ixj] = a1;
+                                const long long t1 = sh.ce[q];
+                                sh.ce[q] = sh.ce[ixj];
+                                sh.ce[ixj] = t1;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            if (tid == 0) {
+                double frac = sh.mbefore;
+                long long cnt = sh.cbefore;
+                double my = 0.;
+                for (int q = 0; q < nc; ++q) {
+                    ++cnt;
+                    my = sh.cw[q];
+                    frac += my;
+                    if (frac > target) break;
+                }
+                sh.red_d[0] = my;
+                sh.red_l[0] = cnt;
+            }
+            __syncthreads();
+            thr = sh.red_d[0];
+            nsig = sh.red_l[0];
+            __syncthreads();
+            break;
+        }
+        // refine: histogram inside the selected bucket
+        for (int b = tid; b < kBinCount; b += NT) {
+            sh.hsum[b] = 0.;
+            sh.hcnt[b] = 0;
+        }
+        __syncthreads();
+        const double sw = sh.width;
+        for (size_t e = tid; e < count; e += NT) {
+            bool v;
+            double d;
+            int k, t, dir;
+            get(e, v, d, k, t, dir);
+            if (!v) continue;
+            const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
+            const double u = (d - lp) - umin;
+            if (!(u >= lo && u < hi)) continue;
+            const double dd = (d + cimg) - dmin;
+            double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
+            w = dd > 700. ? 0. : w * exp(-dd);
+            if (!(w > 0.)) continue;
+            int b = (int)((u - lo) / sw);
+            b = b < 0 ? 0 : (b >= kBinCount ? kBinCount - 1 : b);
+            atomicAdd(&sh.hsum[b], w);
+            atomicAdd(&sh.hcnt[b], 1u);
+        }
+        __syncthreads();
+        ++level;
+    }
+    if (tid == 0) {
+        wo.dmin[img] = dmin;
+        wo.sumw[img] = sum;
+        wo.maxw[img] = mx;
+        wo.amax[img] = am;
+        wo.thr[img] = thr;
+        if (wo.nsig) wo.nsig[img] = (int)nsig;
+        wo.status[img] = status;
+    }
+    // significance bits (coarse): w >= threshold, exactly as the reference compares
+    if (coarse_pass && sigmask) {
+        uint32_t* msk = sigmask + (size_t)img * (count / T) * TW;
+        for (size_t q = tid; q < (count / T) * TW; q += NT) msk[q] = 0u;
+        __syncthreads();
+        for (size_t e = tid; e < count; e += NT) {
+            bool v;
+            double d;
+            int k, t, dir;
+            get(e, v, d, k, t, dir);
+            if (!v) continue;
+            const double dd = (d + cimg) - dmin;
+            double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
+            w = dd > 700. ? 0. : w * exp(-dd);
+            if (w >= thr) {
+                const size_t m = e / T;
+                const int tt = (int)(e % T);
+                atomicOr(&msk[m * TW + (tt >> 5)], 1u << (tt & 31));
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const float* __restrict__ E,
+                                                         const unsigned char* __restrict__ valid,
+                                                         const double* __restrict__ C2c, WeightOut wo,
+                                                         uint32_t* __restrict__ sigmask, int TW) {
+    __shared__ WShared<1024> sh;
+    const int img = blockIdx.x;
+    const int M = wp.Mtot, T = wp.T, R = wp.R, PSI = wp.PSI;
+    const float* Ei = E + (size_t)img * M * T;
+    auto get = [&](size_t e, bool& v, double& d, int& k, int& t, int& dir) {
+        const int m = (int)(e / T);
+        t = (int)(e - (size_t)m * T);
+        v = valid[m] != 0;
+        k = m / R;
+        dir = (m - k * R) / PSI;
+        d = (double)Ei[e];
+    };
+    weight_pass<1024>(img, (size_t)M * T, C2c[img], wp, get, wo, sh, true, sigmask, T, TW);
+}
+
+__global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
+                                                       const int* __restrict__ blk_task, const int* __restrict__ blk_t,
+                                                       const int* __restrict__ task_m,
+                                                       const long long* __restrict__ img_blk, int NE, WeightOut wo) {
+    __shared__ WShared<1024> sh;
+    const int img = blockIdx.x;
+    const long long b0 = img_blk[img];
+    const size_t count = (size_t)(img_blk[img + 1] - b0) * NE;
+    const int R = wp.R, PSI = wp.PSI;
+    auto get = [&](size_t e, bool& v, double& d, int& k, int& t, int& dir) {
+        const long long b = b0 + (long long)(e / NE);
+        const int m = task_m[blk_task[b]];
+        t = blk_t[b];
+        k = m / R;
+        dir = (m - k * R) / PSI;
+        d = D2[(size_t)b0 * NE + e];
+        v = true;
+    };
+    weight_pass<1024>(img, count, 0.0, wp, get, wo, sh, false, nullptr, wp.T, 1);
+}
+
+// ------------------------------------------------------------------ task building
+// Per image: count rotations (m) with any significant translation and the number of
+// significant (m, t) pairs.  One block per image.
+__global__ void k_task_count(const uint32_t* __restrict__ sigmask, int M, int TW, int* __restrict__ ntask,
+                             int* __restrict__ nblk) {
+    __shared__ long long red[16];
+    const int img = blockIdx.x;
+    const uint32_t* msk = sigmask + (size_t)img * M * TW;
+    long long nt = 0, nb = 0;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        int c = 0;
+        for (int w = 0; w < TW; ++w) c += __popc(msk[(size_t)m * TW + w]);
+        nt += c > 0;
+        nb += c;
+    }
+    nt = block_reduce(nt, [](long long a, long long b) { return a + b; }, red);
+    nb = block_reduce(nb, [](long long a, long long b) { return a + b; }, red);
+    if (threadIdx.x == 0) {
+        ntask[img] = (int)nt;
+        nblk[img] = (int)nb;
+    }
+}
+
+// Exclusive scans of the per-image counts (single block).
+__global__ void k_scan_images(const int* __restrict__ ntask, const int* __restrict__ nblk, int n,
+                              long long* __restrict__ task_off, long long* __restrict__ blk_off) {
+    if (threadIdx.x != 0) return;
+    long long a = 0, b = 0;
+    for (int i = 0; i < n; ++i) {
+        task_off[i] = a;
+        blk_off[i] = b;
+        a += ntask[i];
+        b += nblk[i];
+    }
+    task_off[n] = a;
+    blk_off[n] = b;
+}
+
+// Ordered emission of tasks (m ascending) and blocks (t ascending) for one image per block.
+__global__ __launch_bounds__(1024) void k_task_emit(const uint32_t* __restrict__ sigmask, int M, int TW, int T,
+                                                    const long long* __restrict__ task_off,
+                                                    const long long* __restrict__ blk_off, int* __restrict__ task_img,
+                                                    int* __restrict__ task_m, long long* __restrict__ task_blk0,
+                                                    int* __restrict__ task_nt, int* __restrict__ blk_task,
+                                                    int* __restrict__ blk_t, unsigned char* __restrict__ used) {
+    __shared__ int scan_t[1024], scan_b[1024];
+    __shared__ long long base_t, base_b;
+    const int img = blockIdx.x, tid = threadIdx.x;
+    const uint32_t* msk = sigmask + (size_t)img * M * TW;
+    if (tid == 0) {
+        base_t = task_off[img];
+        base_b = blk_off[img];
+    }
+    __syncthreads();
+    for (int m0 = 0; m0 < M; m0 += 1024) {
+        const int m = m0 + tid;
+        int c = 0;
+        if (m < M)
+            for (int w = 0; w < TW; ++w) c += __popc(msk[(size_t)m * TW + w]);
+        scan_t[tid] = c > 0;
+        scan_b[tid] = c;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+            int vt = tid >= o ? scan_t[tid - o] : 0;
+            int vb = tid >= o ? scan_b[tid - o] : 0;
+            __syncthreads();
+            scan_t[tid] += vt;
+            scan_b[tid] += vb;
+            __syncthreads();
+        }
+        if (c > 0) {
+            const long long q = base_t + scan_t[tid] - 1;
+            const long long b = base_b + scan_b[tid] - c;
+            task_img[q] = img;
+            task_m[q] = m;
+            task_blk0[q] = b;
+            task_nt[q] = c;
+            used[m] = 1;
+            int j = 0;
+            for (int w = 0; w < TW; ++w) {
+                uint32_t bits = msk[(size_t)m * TW + w];
+                while (bits) {
+                    const int bit = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    blk_task[b + j] = (int)q;
+                    blk_t[b + j] = w * 32 + bit;
+                    ++j;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 1023) {
+            base_t += scan_t[1023];
+            base_b += scan_b[1023];
+        }
+        __syncthreads();
+    }
+}
+
+// Compaction of the rotations used by any image: ulist[u] = m, m2u[m] = u (single block).
+__global__ __launch_bounds__(1024) void k_unique_rot(const unsigned char* __restrict__ used, int M,
+                                                     int* __restrict__ ulist, int* __restrict__ m2u,
+                                                     int* __restrict__ ucount) {
+    __shared__ int sc[1024];
+    __shared__ int base;
+    const int tid = threadIdx.x;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int m0 = 0; m0 < M; m0 += 1024) {
+        const int m = m0 + tid;
+        const int f = (m < M) ? used[m] : 0;
+        sc[tid] = f;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int v = tid >= o ? sc[tid - o] : 0;
+            __syncthreads();
+            sc[tid] += v;
+            __syncthreads();
+        }
+        if (f) {
+            const int u = base + sc[tid] - 1;
+            ulist[u] = m;
+            m2u[m] = u;
+        }
+        __syncthreads();
+        if (tid == 1023) base += sc[1023];
+        __syncthreads();
+    }
+    if (tid == 0) *ucount = base;
+}
+
+// ------------------------------------------------------------------ fine squared differences
+// One block per task (image, rotation m); for each significant translation t of the
+// task and each (over_rot, over_trans): diff2 = 1/2 (sum Minv|CF|^2 - 2 sum Minv Re(CF conj(XS)) + C)
+// with FP32 products, per-thread FP32 partials and an FP64 block reduction.
+// src/map3d_optimizer.cpp:1261-1372 (fine), 2D src/map2d_optimizer.cpp:964-1051.
+template <int NOR, int NOT>
+__global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
+    constexpr int NT = 256;
+    constexpr int TG = (NOR * NOT >= 32) ? 2 : ((NOR * NOT >= 8) ? 8 : 16);  // translations per group
+    constexpr int NJ = TG * NOT;
+    __shared__ double red[NT / 64][NOR * NJ + NOR];
+    __shared__ int ts[kMaxT];
+    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int img = a.task_img[q], m = a.task_m[q], nt = a.task_nt[q];
+    const long long b0 = a.task_blk0[q];
+    const int u = a.m2u[m];
+    for (int j = tid; j < nt; j += NT) ts[j] = a.blk_t[b0 + j];
+    __syncthreads();
+    const float2* Pf = a.Pfine + (size_t)u * NOR * a.Pf;
+    const float2* X = a.Xf + (size_t)img * a.Pf;
+    const float* Mi = a.Minvf + (size_t)img * a.Pf;
+    const float* Ct = a.Ctff + (size_t)img * a.Pf;
+    const double cimg = a.C2f[img];
+    float Aacc[NOR];
+#pragma unroll
+    for (int o = 0; o < NOR; ++o) Aacc[o] = 0.f;
+    for (int g0 = 0; g0 < nt; g0 += TG) {
+        const int ng = min(TG, nt - g0);
+        float B[NOR][NJ];
+#pragma unroll
+        for (int o = 0; o < NOR; ++o)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) B[o][j] = 0.f;
+        for (int p = tid; p < a.Pf; p += NT) {
+            const float minv = Mi[p];
+            if (minv == 0.f) continue;
+            const float ctf = Ct[p];
+            const float2 x = X[p];
+            float2 CF[NOR];
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) {
+                const float2 f = Pf[(size_t)o * a.Pf + p];
+                CF[o] = make_float2(ctf * f.x, ctf * f.y);
+                if (g0 == 0) Aacc[o] += minv * (CF[o].x * CF[o].x + CF[o].y * CF[o].y);
+            }
+#pragma unroll
+            for (int tt = 0; tt < TG; ++tt) {
+                if (tt < ng) {
+                    const int t = ts[g0 + tt];
+#pragma unroll
+                    for (int ot = 0; ot < NOT; ++ot) {
+                        const float2 S = a.Sf[((size_t)t * NOT + ot) * a.Pf + p];
+                        const float2 xs = cmul(x, S);
+                        const float yr = minv * xs.x, yi = minv * xs.y;
+#pragma unroll
+                        for (int o = 0; o < NOR; ++o) B[o][tt * NOT + ot] += CF[o].x * yr + CF[o].y * yi;
+                    }
+                }
+            }
+        }
+        // reduce: wave shuffles then across waves in FP64
+#pragma unroll
+        for (int o = 0; o < NOR; ++o)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                double v = wave_sum((double)B[o][j]);
+                if (lane == 0) red[wid][o * NJ + j] = v;
+            }
+        if (g0 == 0) {
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) {
+                double v = wave_sum((double)Aacc[o]);
+                if (lane == 0) red[wid][NOR * NJ + o] = v;
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < NOR * NJ; idx += NT) {
+            const int o = idx / NJ, j = idx % NJ;
+            const int tt = j / NOT, ot = j % NOT;
+            if (tt < ng) {
+                double bsum = 0., asum = 0.;
+                for (int w = 0; w < NT / 64; ++w) {
+                    bsum += red[w][idx];
+                    asum += red[w][NOR * NJ + o];
+                }
+                const double diff2 = 0.5 * (asum - 2.0 * bsum) + cimg;
+                a.D2[(size_t)(b0 + g0 + tt) * (NOR * NOT) + o * NOT + ot] = diff2;
+            }
+        }
+        __syncthreads();
+        // keep A partial sums for later groups
+        if (g0 == 0) {
+            // nothing: Aacc is reused (it was fully accumulated in the first group)
+        }
+    }
+}
+
+// ------------------------------------------------------------------ sigma2 / pdf sums / backprojection
+// One block per image.  For every significant fine sample (w >= threshold):
+//   sigma2 : wsum_sigma2_noise[g][ires] += w |CF - XS|^2 (masked image)   src/map3d_optimizer.cpp:1426-1602
+//   pdfs   : sumw_group, pdf_class, pdf_direction, sigma2_offset           src/map3d_optimizer.cpp:2141-2177
+//   bp     : slice(rot) += w * shift(Nomask*Minv*CTF), weight += w Minv CTF^2, then trilinear
+//            scatter into the class grid                                      src/map3d_optimizer.cpp:2360-2479
+template <int NOR, int NOT, int PPT, int NT>
+__global__ __launch_bounds__(NT) void k_post(PostArgs a) {
+    constexpr int NE = NOR * NOT;
+    __shared__ double wts[kMaxT * NE];
+    __shared__ double red[NT / 64];
+    __shared__ double shell[kMaxShell];
+    __shared__ double s_pdfk[kMaxK], s_px[kMaxK], s_py[kMaxK];
+    const int img = blockIdx.x, tid = threadIdx.x;
+    const int g = a.group[img];
+    const double sumw = a.sumw[img], thr = a.thr[img], dmin = a.dmin[img];
+    const long long q0 = a.task_off[img], q1 = a.task_off[img + 1];
+    double wsig[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) wsig[j] = 0.;
+    for (int s = tid; s < a.oriF; s += NT) shell[s] = 0.;
+    for (int k = tid; k < a.K; k += NT) s_pdfk[k] = s_px[k] = s_py[k] = 0.;
+    double t_sumw = 0., t_off = 0.;  // thread 0 accumulators
+    __syncthreads();
+    const float2* X = a.Xf + (size_t)img * a.Pf;
+    const float* Ct = a.Ctff + (size_t)img * a.Pf;
+    const float2* NM = a.NMf + (size_t)img * a.Pf;
+    const float* MC2 = a.MC2bp + (size_t)img * a.Pf;
+    const double ox = a.old_offx[img], oy = a.old_offy[img];
+    for (long long q = q0; q < q1; ++q) {
+        const int m = a.task_m[q], nt = a.task_nt[q];
+        const long long b0 = a.task_blk0[q];
+        const int k = m / a.R, r = m % a.R, dir = r / a.PSI;
+        const int u = a.m2u[m];
+        // weights of all entries of the task
+        for (int e = tid; e < nt * NE; e += NT) {
+            const int tb = e / NE;
+            const int t = a.blk_t[b0 + tb];
+            const double d2 = a.D2[(size_t)(b0 + tb) * NE + (e % NE)];
+            const double dd = d2 - dmin;
+            double w = a.pdf_dir[(size_t)k * a.D + dir] *
+                       pdf_offset(a.trans_x[t], a.trans_y[t], ox, oy, a.prior_x[k], a.prior_y[k], a.sigma2_offset);
+            w = dd > 700. ? 0. : w * exp(-dd);
+            wts[e] = w;
+        }
+        __syncthreads();
+        const double* Ai = a.Ai_fine + 6 * ((size_t)r * NOR);
+        for (int o = 0; o < NOR; ++o) {
+            float2 acc[PPT];
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) acc[j] = make_float2(0.f, 0.f);
+            double wslice = 0.;
+            const float2* Pf = a.Pfine + ((size_t)u * NOR + o) * a.Pf;
+            float2 CF[PPT];
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                const int p = tid + j * NT;
+                if (p < a.Pf) {
+                    const float2 f = Pf[p];
+                    const float c = Ct[p];
+                    CF[j] = make_float2(c * f.x, c * f.y);
+                }
+            }
+            for (int tb = 0; tb < nt; ++tb) {
+                const int t = a.blk_t[b0 + tb];
+                for (int ot = 0; ot < NOT; ++ot) {
+                    const double w = wts[tb * NE + o * NOT + ot];
+                    if (!(w >= thr)) continue;
+                    const double wn = w / sumw;
+                    wslice += wn;
+                    if (tid == 0) {
+                        t_sumw += wn;
+                        s_pdfk[k] += wn;
+                        const double sx = a.shift_x[t * NOT + ot], sy = a.shift_y[t * NOT + ot];
+                        const double offx = a.prior_x[k] - ox - sx, offy = a.prior_y[k] - oy - sy;
+                        t_off += wn * (offx * offx + offy * offy);
+                        if (!a.is3d) {
+                            s_px[k] += wn * (ox + sx);
+                            s_py[k] += wn * (oy + sy);
+                        }
+                        atomicAdd(&a.wsum_pdf_dir[(size_t)k * a.D + dir], wn);
+                    }
+                    const float wf = (float)wn;
+                    const float2* S = a.Sf + ((size_t)t * NOT + ot) * a.Pf;
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        const int p = tid + j * NT;
+                        if (p < a.Pf) {
+                            const float2 s = S[p];
+                            const float2 xs = cmul(X[p], s);
+                            const float dr = CF[j].x - xs.x, di = CF[j].y - xs.y;
+                            wsig[j] += wn * (double)(dr * dr + di * di);
+                            const float2 ns = cmul(NM[p], s);
+                            acc[j].x += wf * ns.x;
+                            acc[j].y += wf * ns.y;
+                        }
+                    }
+                }
+            }
+            if (wslice > 0.) {
+                // trilinear scatter of the slice (MyBackProjector::backproject, src/reconstruct.cpp:977-1164)
+                float* gd = a.bp_data + a.grid_stride * 2 * k;
+                float* gw = a.bp_weight + a.grid_stride * k;
+                const double* A6 = Ai + 6 * o;
+#pragma unroll
+                for (int j = 0; j < PPT; ++j) {
+                    const int p = tid + j * NT;
+                    if (p >= a.Pf) continue;
+                    const int xy = a.f_xy[p];
+                    const int x = px_x(xy), y = px_y(xy);
+                    if (x * x + y * y > a.r_max * a.r_max) continue;
+                    const float wgt = (float)((double)MC2[p] * wslice);
+                    if (!(wgt > 0.f)) continue;
+                    const Coords c = map_coords(A6, x, y, a.c, a.is3d != 0);
+                    float vr = acc[j].x, vi = c.neg ? -acc[j].y : acc[j].y;
+                    const float mfx = 1.f - c.fx, mfy = 1.f - c.fy, mfz = 1.f - c.fz;
+                    if (a.is3d) {
+                        const float dd[8] = {mfz * mfy * mfx, mfz * mfy * c.fx, mfz * c.fy * mfx, mfz * c.fy * c.fx,
+                                             c.fz * mfy * mfx, c.fz * mfy * c.fx, c.fz * c.fy * mfx, c.fz * c.fy * c.fx};
+#pragma unroll
+                        for (int cc = 0; cc < 8; ++cc) {
+                            const int dz = cc >> 2, dy = (cc >> 1) & 1, dx = cc & 1;
+                            const size_t idx = ((size_t)(c.z0 + dz) * a.pad + (c.y0 + dy)) * a.padF + (c.x0 + dx);
+                            atomicAdd(&gd[2 * idx], dd[cc] * vr);
+                            atomicAdd(&gd[2 * idx + 1], dd[cc] * vi);
+                            atomicAdd(&gw[idx], dd[cc] * wgt);
+                        }
+                    } else {
+                        const float dd[4] = {mfy * mfx, mfy * c.fx, c.fy * mfx, c.fy * c.fx};
+#pragma unroll
+                        for (int cc = 0; cc < 4; ++cc) {
+                            const int dy = cc >> 1, dx = cc & 1;
+                            const size_t idx = (size_t)(c.y0 + dy) * a.padF + (c.x0 + dx);
+                            atomicAdd(&gd[2 * idx], dd[cc] * vr);
+                            atomicAdd(&gd[2 * idx + 1], dd[cc] * vi);
+                            atomicAdd(&gw[idx], dd[cc] * wgt);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // sigma2 shells and norm-correction sum
+    double tot = 0.;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int p = tid + j * NT;
+        if (p < a.Pf) {
+            atomicAdd(&shell[a.f_ires[p]], wsig[j]);
+            tot += wsig[j];
+        }
+    }
+    tot = block_reduce(tot, [](double x, double y) { return x + y; }, red);
+    __syncthreads();
+    for (int s = tid; s < a.oriF; s += NT)
+        if (shell[s] != 0.) atomicAdd(&a.wsum_sigma2[(size_t)g * a.oriF + s], shell[s]);
+    for (int kk = tid; kk < a.K; kk += NT) {
+        if (s_pdfk[kk] != 0.) atomicAdd(&a.wsum_pdf_class[kk], s_pdfk[kk]);
+        if (s_px[kk] != 0.) atomicAdd(&a.wsum_prior_x[kk], s_px[kk]);
+        if (s_py[kk] != 0.) atomicAdd(&a.wsum_prior_y[kk], s_py[kk]);
+    }
+    if (tid == 0) {
+        a.wnorm[img] = tot;
+        atomicAdd(&a.wsum_sumw_group[g], t_sumw);
+        atomicAdd(&a.wsum_scalars[0], t_off);
+    }
+}
+
+// ------------------------------------------------------------------ storeWeightedSums (src/map3d_optimizer.cpp:1933-2057)
+__global__ void k_store_sums(StoreArgs a) {
+    const int img = blockIdx.x * blockDim.x + threadIdx.x;
+    if (img >= a.n) return;
+    const int g = a.group[img];
+    const double* pw = a.power + (size_t)img * a.oriF;
+    double wn = a.wnorm[img];
+    for (int ires = a.curF; ires < a.oriF; ++ires) {
+        atomicAdd(&a.wsum_sigma2[(size_t)g * a.oriF + ires], pw[ires]);
+        wn += pw[ires];
+    }
+    a.wnorm[img] = wn;
+    double* meta = a.meta + (size_t)img * 9;
+    meta[7] = a.norm[img];
+    if (a.do_norm) {
+        const double old = a.norm[img] / a.avg_norm;
+        meta[7] = old * sqrt(wn * 2.);
+        atomicAdd(&a.wsum_scalars[3], meta[7]);
+    }
+    const double dll = log(a.sumw[img]) - a.dmin[img] - a.logsigma2[g];
+    meta[8] = dll;
+    atomicAdd(&a.wsum_scalars[1], dll);
+    atomicAdd(&a.wsum_scalars[2], meta[0]);
+}
+
+// ------------------------------------------------------------------ small helpers
+// Single-pass mode: the significant entries take their diff2 from the pass's own
+// coarse values, so weights, threshold and sums all come from one set of numbers.
+__global__ void k_fill_d2(const int* __restrict__ blk_task, const int* __restrict__ blk_t, const int* __restrict__ task_img,
+                          const int* __restrict__ task_m, long long nblocks, const float* __restrict__ E,
+                          const double* __restrict__ C2c, int M, int T, double* __restrict__ D2) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int q = blk_task[b];
+    const int img = task_img[q];
+    D2[b] = (double)E[((size_t)img * M + task_m[q]) * T + blk_t[b]] + C2c[img];
+}
+
+// argmax entry of the fine list -> (rotation m, translation t)
+__global__ void k_decode_argmax(const long long* __restrict__ amax, const long long* __restrict__ img_blk,
+                                const int* __restrict__ blk_task, const int* __restrict__ blk_t,
+                                const int* __restrict__ task_m, int n, int NE, int2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long b = img_blk[i] + amax[i] / NE;
+    out[i] = make_int2(task_m[blk_task[b]], blk_t[b]);
+}
+
+// ------------------------------------------------------------------ launch wrappers
+#define LAUNCH_CHECK()                                   \
+    do {                                                 \
+        hipError_t _e = hipGetLastError();               \
+        if (_e != hipSuccess) return _e;                 \
+    } while (0)
+
+hipError_t launch_phase_table(hipStream_t s, const double* sx, const double* sy, int nshift, const int* xy, int P, int ori,
+                              const double* sintab, const double* costab, float2* out) {
+    const size_t tot = (size_t)nshift * P;
+    if (tot == 0) return hipSuccess;
+    k_phase_table<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(sx, sy, nshift, xy, P, ori, sintab, costab, out);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
+                                 const double* Ai, int R, int Mtot, const int* xy, int Pc, int Pc_pad, int my_r_max,
+                                 int Mpad, float* FcT) {
+    dim3 grid((Mtot + 255) / 256, Pc);
+    k_project_coarse<<<grid, 256, 0, s>>>(grids, grid_stride, pad, padF, c, is3d, Ai, R, Mtot, xy, Pc, Pc_pad, my_r_max,
+                                          Mpad, FcT);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_project_fine(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
+                               const double* Ai_fine, int R, int NOR, const int* ulist, int U, const int* xy, int Pf,
+                               int r_max, float2* Pfine) {
+    if (U == 0) return hipSuccess;
+    dim3 grid(U * NOR, (Pf + 255) / 256);
+    k_project_fine<<<grid, 256, 0, s>>>(grids, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist, xy, Pf, r_max,
+                                        Pfine);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
+                                int nrot, int size, int my_r_max, float2* out) {
+    if (nrot == 0) return hipSuccess;
+    const int n = size * (size / 2 + 1);
+    dim3 grid_(nrot, (n + 255) / 256);
+    k_project_plane<<<grid_, 256, 0, s>>>(grid, pad, padF, c, is3d, Ai, size, my_r_max, out);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a) {
+    k_prep_images<<<n, 256, 0, s>>>(a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
+                               const float* MC2c, float* At) {
+    dim3 grid((Mtot + 255) / 256, n);
+    k_coarse_aterm<<<grid, 256, 0, s>>>(FcT, Mtot, Mpad, Pc, Pc_pad, MC2c, At);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a) {
+    const int tiles = (a.Mpad / 128) * (a.Npad / 128);
+    k_coarse_gemm<<<tiles, 256, 0, s>>>(a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const unsigned char* valid,
+                                 const double* C2c, const WeightOutArgs& wo, uint32_t* sigmask, int TW) {
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, valid, C2c, o, sigmask, TW);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
+                               const int* blk_t, const int* task_m, const long long* img_blk, int NE,
+                               const WeightOutArgs& wo) {
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    k_weights_fine<<<n, 1024, 0, s>>>(wp, D2, blk_task, blk_t, task_m, img_blk, NE, o);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_task_build(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, int* ntask, int* nblk,
+                             long long* task_off, long long* blk_off) {
+    k_task_count<<<n, 256, 0, s>>>(sigmask, M, TW, ntask, nblk);
+    LAUNCH_CHECK();
+    k_scan_images<<<1, 64, 0, s>>>(ntask, nblk, n, task_off, blk_off);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_task_emit(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, const long long* task_off,
+                            const long long* blk_off, int* task_img, int* task_m, long long* task_blk0, int* task_nt,
+                            int* blk_task, int* blk_t, unsigned char* used, int* ulist, int* m2u, int* ucount) {
+    k_task_emit<<<n, 1024, 0, s>>>(sigmask, M, TW, T, task_off, blk_off, task_img, task_m, task_blk0, task_nt, blk_task,
+                                    blk_t, used);
+    LAUNCH_CHECK();
+    k_unique_rot<<<1, 1024, 0, s>>>(used, M, ulist, m2u, ucount);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a) {
+    if (ntask == 0) return hipSuccess;
+    if (NOR == 8 && NOT == 4) k_fine_diff2<8, 4><<<ntask, 256, 0, s>>>(a);
+    else if (NOR == 2 && NOT == 4) k_fine_diff2<2, 4><<<ntask, 256, 0, s>>>(a);
+    else if (NOR == 1 && NOT == 1) k_fine_diff2<1, 1><<<ntask, 256, 0, s>>>(a);
+    else return hipErrorInvalidValue;
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NOR, int NOT>
+static hipError_t post_ppt(hipStream_t s, int n, int Pf, const PostArgs& a) {
+    if (Pf <= 4 * 256) k_post<NOR, NOT, 4, 256><<<n, 256, 0, s>>>(a);
+    else if (Pf <= 4 * 1024) k_post<NOR, NOT, 4, 1024><<<n, 1024, 0, s>>>(a);
+    else if (Pf <= 8 * 1024) k_post<NOR, NOT, 8, 1024><<<n, 1024, 0, s>>>(a);
+    else if (Pf <= 16 * 1024) k_post<NOR, NOT, 16, 1024><<<n, 1024, 0, s>>>(a);
+    else return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a) {
+    hipError_t e;
+    if (NOR == 8 && NOT == 4) e = post_ppt<8, 4>(s, n, a.Pf, a);
+    else if (NOR == 2 && NOT == 4) e = post_ppt<2, 4>(s, n, a.Pf, a);
+    else if (NOR == 1 && NOT == 1) e = post_ppt<1, 1>(s, n, a.Pf, a);
+    else return hipErrorInvalidValue;
+    if (e != hipSuccess) return e;
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
+                          long long nblocks, const float* E, const double* C2c, int M, int T, double* D2) {
+    if (nblocks == 0) return hipSuccess;
+    k_fill_d2<<<(unsigned)((nblocks + 255) / 256), 256, 0, s>>>(blk_task, blk_t, task_img, task_m, nblocks, E, C2c, M, T, D2);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
+                                const int* blk_t, const int* task_m, int n, int NE, int2* out) {
+    k_decode_argmax<<<(n + 127) / 128, 128, 0, s>>>(amax, img_blk, blk_task, blk_t, task_m, n, NE, out);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a) {
+    k_store_sums<<<(a.n + 127) / 128, 128, 0, s>>>(a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+}  // namespace rome

@@ -1,0 +1,303 @@
+"""Seeded synthetic MAP2D / MAP3D E-step problems (numpy).
+
+ROME ships no data generator (src/simulatedata.h is an empty stub), so the inputs of
+the E-step are synthesised here in the shapes the reference hands to
+expectationSomeParticles: reference volumes (sums of Gaussian blobs) turned into
+2x-padded Fourier grids the way MyProjector::computeFourierTransformMap does
+(src/reconstruct.cpp:106-223), particles = CTF * projection * shift phase + white
+noise at a given SNR, written directly as the prepared Fourier half-planes
+(no real-space soft mask), plus the model state (sigma2_noise, pdfs, priors) and the
+perturbed sampling tables of the product's own HealpixSampler port.
+"""
+import numpy as np
+
+from . import pose_table2d, pose_table3d, shift_table, translations
+
+PI = 3.14159265358979323846
+
+
+# ---------------- volumes ----------------
+def blob_volume(ori, nblobs, sigma_range, radius, rng, ndim=3):
+    """Sum of isotropic Gaussian blobs inside a sphere / disc of `radius` px (centred)."""
+    ax = np.arange(ori) - ori // 2
+    grids = np.meshgrid(*([ax] * ndim), indexing="ij")
+    vol = np.zeros([ori] * ndim)
+    for _ in range(nblobs):
+        while True:
+            c = rng.uniform(-radius, radius, ndim)
+            if np.sum(c * c) <= radius * radius:
+                break
+        s = rng.uniform(*sigma_range)
+        r2 = sum((g - ci) ** 2 for g, ci in zip(grids, c))
+        vol += rng.uniform(0.5, 1.5) * np.exp(-r2 / (2 * s * s))
+    return vol
+
+
+def fourier_grid(vol, current_size, padding_factor=2, gridding=True):
+    """2x-padded, r <= 2*r_max windowed Fourier grid [z][y][x] (x half), complex interleaved.
+
+    Follows computeFourierTransformMap: gridding correction (divide by sinc^2),
+    zero padding to 2*ori, centre, forward FFT normalised by 1/N, scale by normfft.
+    """
+    ndim = vol.ndim
+    ori = vol.shape[0]
+    r_max = min(current_size // 2, ori // 2)
+    pad = 2 * (padding_factor * r_max + 1) + 1
+    padF = pad // 2 + 1
+    v = vol.astype(np.float64).copy()
+    if gridding:
+        ax = np.arange(ori) - ori // 2
+        g = np.meshgrid(*([ax] * ndim), indexing="ij")
+        r = np.sqrt(sum(x * x for x in g))
+        rval = r / (ori * padding_factor)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            sinc = np.sin(PI * rval) / (PI * rval)
+        corr = np.where(r > 0, sinc * sinc, 1.0)
+        v = v / corr
+    N = padding_factor * ori
+    P = np.zeros([N] * ndim)
+    o = N // 2 - ori // 2
+    sl = tuple(slice(o, o + ori) for _ in range(ndim))
+    P[sl] = v
+    P = np.fft.ifftshift(P)   # origin of the map at index 0
+    F = np.fft.rfftn(P) / P.size
+    normfft = padding_factor ** 3 * ori if ndim == 3 else padding_factor ** 2
+    F = F * normfft
+    max_r2 = (r_max * padding_factor) ** 2
+    c = pad // 2
+    if ndim == 3:
+        grid = np.zeros((pad, pad, padF), np.complex128)
+        for kp in range(-2 * r_max, 2 * r_max + 1):
+            for ip in range(-2 * r_max, 2 * r_max + 1):
+                jmax2 = max_r2 - kp * kp - ip * ip
+                if jmax2 < 0:
+                    continue
+                jm = int(np.floor(np.sqrt(jmax2)))
+                grid[kp + c, ip + c, :jm + 1] = F[kp % N, ip % N, :jm + 1]
+    else:
+        grid = np.zeros((pad, padF), np.complex128)
+        for ip in range(-2 * r_max, 2 * r_max + 1):
+            jmax2 = max_r2 - ip * ip
+            if jmax2 < 0:
+                continue
+            jm = int(np.floor(np.sqrt(jmax2)))
+            grid[ip + c, :jm + 1] = F[ip % N, :jm + 1]
+    out = np.zeros(grid.shape + (2,))
+    out[..., 0] = grid.real
+    out[..., 1] = grid.imag
+    return out.reshape(-1), pad
+
+
+# ---------------- geometry ----------------
+def euler_matrix(rot, tilt, psi):
+    a, b, g = np.deg2rad(rot), np.deg2rad(tilt), np.deg2rad(psi)
+    ca, cb, cg, sa, sb, sg = np.cos(a), np.cos(b), np.cos(g), np.sin(a), np.sin(b), np.sin(g)
+    cc, cs, sc, ss = cb * ca, cb * sa, sb * ca, sb * sa
+    return np.array([[cg * cc - sg * sa, cg * cs + sg * ca, -cg * sb],
+                     [-sg * cc - cg * sa, -sg * cs + cg * ca, sg * sb],
+                     [sc, ss, cb]])
+
+
+def halfplane_xy(size):
+    F = size // 2 + 1
+    i = np.arange(size)[:, None]
+    j = np.arange(F)[None, :]
+    y = np.where(i < F, i, i - size) + 0 * j
+    x = j + 0 * i
+    return x.reshape(-1), y.reshape(-1)
+
+
+def project_numpy(grid, pad, r_max, A, size, is3d=True):
+    """Vectorised trilinear (3D) / bilinear (2D) slice of the padded grid (float64)."""
+    padF = pad // 2 + 1
+    c = pad // 2
+    g = grid.reshape((pad, pad, padF, 2) if is3d else (pad, padF, 2))
+    gc = g[..., 0] + 1j * g[..., 1]
+    x, y = halfplane_xy(size)
+    myr = min(r_max, size // 2)
+    F = size // 2 + 1
+    i = np.arange(size * F) // F
+    okrow = (i <= myr) | (i >= size - myr)
+    ok = okrow & (x <= myr) & (x * x + y * y <= myr * myr)
+    Ai = 2.0 * A.T
+    xp = Ai[0, 0] * x + Ai[0, 1] * y
+    yp = Ai[1, 0] * x + Ai[1, 1] * y
+    zp = Ai[2, 0] * x + Ai[2, 1] * y
+    neg = xp < 0
+    xp, yp, zp = np.where(neg, -xp, xp), np.where(neg, -yp, yp), np.where(neg, -zp, zp)
+    x0, y0, z0 = np.floor(xp).astype(int), np.floor(yp).astype(int), np.floor(zp).astype(int)
+    fx, fy, fz = xp - x0, yp - y0, zp - z0
+    x0 = np.where(ok, x0, 0)
+    y0 = np.where(ok, y0 + c, c)
+    z0 = np.where(ok, z0 + c, c)
+    if is3d:
+        def at(dz, dy, dx):
+            return gc[z0 + dz, y0 + dy, x0 + dx]
+        v = ((1 - fz) * ((1 - fy) * ((1 - fx) * at(0, 0, 0) + fx * at(0, 0, 1)) +
+                         fy * ((1 - fx) * at(0, 1, 0) + fx * at(0, 1, 1))) +
+             fz * ((1 - fy) * ((1 - fx) * at(1, 0, 0) + fx * at(1, 0, 1)) +
+                   fy * ((1 - fx) * at(1, 1, 0) + fx * at(1, 1, 1))))
+    else:
+        def at2(dy, dx):
+            return gc[y0 + dy, x0 + dx]
+        v = (1 - fy) * ((1 - fx) * at2(0, 0) + fx * at2(0, 1)) + fy * ((1 - fx) * at2(1, 0) + fx * at2(1, 1))
+    v = np.where(neg, np.conj(v), v)
+    return np.where(ok, v, 0)
+
+
+def ctf_image(size, ori, angpix, defU, defV, ang_deg, kv=300.0, cs=2.7, q0=0.1, bfac=0.0):
+    """CTF::getFftwImage (src/ctf.cpp:173-191) with CTF::getCTF (src/ctf.h:151-205)."""
+    local_cs = cs * 1e7
+    local_kv = kv * 1e3
+    lam = 12.2643247 / np.sqrt(local_kv * (1. + local_kv * 0.978466e-6))
+    K1 = PI / 2 * 2 * lam
+    K2 = PI / 2 * local_cs * lam ** 3
+    K3 = np.sqrt(1 - q0 * q0)
+    K4 = -bfac / 4.
+    defavg = -(defU + defV) * 0.5
+    defdev = -(defU - defV) * 0.5
+    rad_az = np.deg2rad(ang_deg)
+    xs = ori * angpix
+    x, y = halfplane_xy(size)
+    X = x / xs
+    Y = y / xs
+    u2 = X * X + Y * Y
+    ell = np.arctan2(Y, X) - rad_az
+    deltaf = np.where((np.abs(X) < 1e-6) & (np.abs(Y) < 1e-6), 0.0, defavg + defdev * np.cos(2 * ell))
+    arg = K1 * deltaf * u2 + K2 * u2 * u2
+    return -(K3 * np.sin(arg) - q0 * np.cos(arg)) * np.exp(K4 * u2)
+
+
+def mresol(size):
+    x, y = halfplane_xy(size)
+    F = size // 2 + 1
+    ires = np.round(np.sqrt(x * x + y * y)).astype(int)
+    return np.where((ires < F) & ~((x == 0) & (y < 0)), ires, -1)
+
+
+def window(arr, in_size, out_size):
+    """windowFourierTransform down-size branch on a [.., in F2] array."""
+    Fi, Fo = in_size // 2 + 1, out_size // 2 + 1
+    a = arr.reshape(arr.shape[:-1] + (in_size, Fi))
+    rows = [ip if ip >= 0 else ip + in_size for ip in [i if i < Fo else i - out_size for i in range(out_size)]]
+    return a[..., rows, :Fo].reshape(arr.shape[:-1] + (out_size * Fo,))
+
+
+# ---------------- problem ----------------
+def make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, psi_step=10.0, offset_range=2.0,
+                 offset_step=1.0, oversampling=1, perturbation=0.3, n=4, snr=0.5, nr_groups=1, seed=0,
+                 do_ctf=True, angpix=2.0, nblobs=20, sigma_offset=None, true_class=None):
+    rng = np.random.default_rng(seed)
+    ndim = 3 if is3d else 2
+    radius = 0.35 * ori
+    vols = [blob_volume(ori, nblobs, (1.0, max(1.5, ori / 16)), radius, rng, ndim) for _ in range(K)]
+    grids = []
+    pad = None
+    for v in vols:
+        g, pad = fourier_grid(v, current)
+        grids.append(g)
+    grids = np.stack(grids)
+    r_max = min(current // 2, ori // 2)
+    # sampling (product port of HealpixSampler)
+    if is3d:
+        pose_c = pose_table3d(order, 0, perturbation)
+        pose_f = pose_table3d(order, 1, perturbation) if oversampling else None
+        D, PSI = pose_c.shape[0], pose_c.shape[1]
+    else:
+        pose_c = pose_table2d(psi_step, 0, perturbation)
+        pose_f = pose_table2d(psi_step, 1, perturbation) if oversampling else None
+        D, PSI = 1, pose_c.shape[0]
+    tx, ty = translations(offset_range, offset_step)
+    T = len(tx)
+    scx, scy = shift_table(offset_range, offset_step, 0, perturbation)
+    sfx, sfy = shift_table(offset_range, offset_step, 1, perturbation) if oversampling else (None, None)
+    oriF = ori // 2 + 1
+    F2o = ori * oriF
+    # particles at ori size, then windowed
+    imgs = np.zeros((n, F2o), np.complex128)
+    ctfs = np.zeros((n, F2o))
+    truth = []
+    sig_power = []
+    for i in range(n):
+        k = int(rng.integers(K)) if true_class is None else true_class
+        if is3d:
+            rot, tilt, psi = rng.uniform(-180, 180), np.rad2deg(np.arccos(rng.uniform(-1, 1))), rng.uniform(0, 360)
+        else:
+            rot, tilt, psi = 0.0, 0.0, rng.uniform(0, 360)
+        sh = rng.integers(-int(offset_range), int(offset_range) + 1, 2).astype(float)
+        A = euler_matrix(rot, tilt, psi)
+        proj = project_numpy(grids[k], pad, r_max, A, ori if current == ori else current, is3d)
+        if current != ori:
+            full = np.zeros(F2o, np.complex128)
+            xo, yo = halfplane_xy(ori)
+            xc, yc = halfplane_xy(current)
+            idx = {(a, b): q for q, (a, b) in enumerate(zip(xo, yo))}
+            for q, (a, b) in enumerate(zip(xc, yc)):
+                full[idx[(a, b)]] = proj[q]
+            proj = full
+        if do_ctf:
+            ctf = ctf_image(ori, ori, angpix, rng.uniform(1e4, 3e4), rng.uniform(1e4, 3e4), rng.uniform(0, 180))
+        else:
+            ctf = np.ones(F2o)
+        xo, yo = halfplane_xy(ori)
+        # particle translated so that the reference's estimate XOFF/YOFF equals `sh`
+        phase = np.exp(2j * PI * (xo * sh[0] + yo * sh[1]) / ori)
+        sig = ctf * proj * phase
+        imgs[i] = sig
+        ctfs[i] = ctf
+        sig_power.append(np.mean(np.abs(sig) ** 2))
+        truth.append((k, rot, tilt, psi, sh[0], sh[1]))
+    sigma2 = 0.5 * float(np.mean(sig_power)) / snr if snr > 0 else 1.0
+    noise = rng.normal(0, np.sqrt(sigma2), (n, F2o)) + 1j * rng.normal(0, np.sqrt(sigma2), (n, F2o))
+    imgs = imgs + noise
+    # x = 0 column: keep Hermitian-consistent values (y<0 mirrors y>0)
+    xo, yo = halfplane_xy(ori)
+    mres_o = mresol(ori)
+    power = np.zeros((n, oriF))
+    hxi2 = np.zeros(n)
+    curF = current // 2 + 1
+    for i in range(n):
+        a2 = np.abs(imgs[i]) ** 2
+        sel = mres_o > 0
+        np.add.at(power[i], mres_o[sel], a2[sel])
+        if current < ori:
+            hxi2[i] = a2[sel & (mres_o >= curF)].sum()
+    fimg = window(imgs, ori, current)
+    fctf = window(ctfs, ori, current)
+    groups = rng.integers(0, nr_groups, n).astype(np.int32)
+    sigma2_noise = np.full((nr_groups, oriF), sigma2) * rng.uniform(0.9, 1.1, (nr_groups, oriF))
+    prob = dict(
+        is_3d=int(is3d), ori_size=ori, current_size=current, coarse_size=coarse if oversampling else current,
+        nr_classes=K, nr_groups=nr_groups, adaptive_oversampling=oversampling, nr_dir=D, nr_psi=PSI, nr_trans=T,
+        adaptive_fraction=0.999, sigma2_fudge=1.0,
+        sigma2_offset=(offset_range / 2.0) ** 2 if sigma_offset is None else sigma_offset,
+        avg_norm_correction=1.0, do_ctf_correction=int(do_ctf), refs_are_ctf_corrected=1, do_norm_correction=1,
+        do_map=1, pad=pad, sigma2_noise=sigma2_noise, pdf_class=np.full(K, 1.0 / K),
+        pdf_direction=np.full((K, D), 1.0 / D), prior_offx=np.zeros(K), prior_offy=np.zeros(K), grids=grids,
+        pose_coarse=pose_c, pose_fine=pose_f, shift_coarse_x=scx[:, 0], shift_coarse_y=scy[:, 0],
+        shift_fine_x=sfx, shift_fine_y=sfy, trans_x=tx, trans_y=ty)
+    batch = dict(n=n, fmask_re=fimg.real.copy(), fmask_im=fimg.imag.copy(), fnomask_re=fimg.real.copy(),
+                 fnomask_im=fimg.imag.copy(), fctf=fctf, highres_Xi2=hxi2, power_img=power,
+                 old_offx=np.zeros(n), old_offy=np.zeros(n), group=groups, norm=np.ones(n))
+    return prob, batch, truth
+
+
+def engine_config(prob):
+    keys = ("ori_size", "current_size", "coarse_size", "nr_classes", "nr_groups", "adaptive_oversampling",
+            "nr_dir", "nr_psi", "nr_trans", "adaptive_fraction", "sigma2_fudge", "sigma2_offset",
+            "avg_norm_correction", "do_ctf_correction", "refs_are_ctf_corrected", "do_norm_correction", "do_map")
+    cfg = {k: prob[k] for k in keys}
+    cfg["ref_dim"] = 3 if prob["is_3d"] else 2
+    return cfg
+
+
+def load_engine(engine, prob):
+    """Configure an Engine with a problem dict (same tables the oracle receives)."""
+    engine.configure(**engine_config(prob))
+    engine.set_sampling(prob["pose_coarse"], prob["pose_fine"], prob["shift_coarse_x"], prob["shift_coarse_y"],
+                        prob["shift_fine_x"], prob["shift_fine_y"], prob["trans_x"], prob["trans_y"])
+    engine.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
+                     prob["prior_offy"])
+    for k in range(prob["nr_classes"]):
+        engine.set_reference(k, prob["grids"][k])
+    engine.reset_sums()
