@@ -149,6 +149,20 @@ int rome_get_fine_list(rome_ctx* ctx, long long* keys, double* diff2);
 int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9,
                  double* out_re, double* out_im);
 
+/* Projector into a device buffer (float2 [nrot][size*(size/2+1)]): synthetic particles. */
+int rome_project_device(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, void* out_device);
+
+/* Measurement: stage times from HIP events recorded on the engine stream around each kernel
+ * group (no counterpart in the reference, whose timers are TIMEPOINT macros, src/time.h:61-75).
+ * Stages: 0 prep, 1 coarse norms, 2 coarse cross term, 3 coarse weights, 4 task build,
+ * 5 fine projections, 6 fine diff2, 7 fine weights, 8 sigma2/pdf/backprojection, 9 store sums.
+ * rome_set_profiling(1) resets the accumulators; rome_get_profile returns the stage count. */
+int rome_set_profiling(rome_ctx* ctx, int on);
+int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, int nstages);
+/* Last pool: n, fine tasks, fine blocks, unique rotations, coarse pixels, fine pixels,
+ * rotations x classes, translations. */
+int rome_get_batch_stats(rome_ctx* ctx, long long* out8);
+
 /* ---- host sampling helpers (HealpixSampler, C1) ---- */
 int rome_sampling_pose3d(int healpix_order, int oversampling, double perturbation, double* out);
 int rome_sampling_pose2d(double psi_step, int oversampling, double perturbation, double* out);

@@ -24,8 +24,12 @@ EXPORTED = (
     "rome_set_reference", "rome_reset_sums", "rome_expectation", "rome_expectation_device", "rome_get_sums",
     "rome_get_backprojector", "rome_get_coarse_diff2", "rome_get_fine_count", "rome_get_fine_list", "rome_project",
     "rome_sampling_pose3d", "rome_sampling_pose2d", "rome_sampling_translations", "rome_sampling_shifts",
-    "rome_healpix_pix2ang",
+    "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
+    "rome_get_batch_stats",
 )
+
+STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
+          "fine_weights", "post", "store")
 
 
 class RomeError(RuntimeError):
@@ -253,6 +257,25 @@ class Engine:
         d2 = np.zeros(max(cnt, 1))
         self._check(lib().rome_get_fine_list(self._h, _p(keys, C.c_longlong), _p(d2)), "rome_get_fine_list")
         return keys[:cnt], d2[:cnt]
+
+    def project_device(self, iclass, size, A, out_ptr):
+        A = _f64(A).reshape(-1, 9)
+        self._check(lib().rome_project_device(self._h, iclass, size, A.shape[0], _p(A), C.c_void_p(out_ptr)),
+                    "rome_project_device")
+
+    def set_profiling(self, on=True):
+        self._check(lib().rome_set_profiling(self._h, int(on)), "rome_set_profiling")
+
+    def profile(self):
+        ms = np.zeros(len(STAGES))
+        calls = np.zeros(len(STAGES), np.int64)
+        lib().rome_get_profile(self._h, _p(ms), _p(calls, C.c_longlong), len(STAGES))
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(STAGES)}
+
+    def batch_stats(self):
+        out = np.zeros(8, np.int64)
+        self._check(lib().rome_get_batch_stats(self._h, _p(out, C.c_longlong)), "rome_get_batch_stats")
+        return dict(zip(("n", "tasks", "blocks", "unique_rot", "Pc", "Pf", "M", "T"), out.tolist()))
 
     def project(self, iclass, size, A):
         A = _f64(A).reshape(-1, 9)

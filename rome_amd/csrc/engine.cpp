@@ -110,6 +110,14 @@ struct rome_ctx {
     DevBuf<float2> Pfine;
     DevBuf<double> D2;
     DevBuf<int2> amax_mt;
+    // stage timing with HIP events on the engine stream (rome_set_profiling)
+    static constexpr int kStages = 10;
+    bool profiling = false;
+    hipEvent_t ev[kStages][2] = {};
+    bool ev_used[kStages] = {};
+    double stage_ms[kStages] = {};
+    long long stage_calls[kStages] = {};
+    long long last_U = 0;
     // last batch bookkeeping (parity hooks)
     int last_n = 0;
     long long last_blocks = 0, last_tasks = 0;
@@ -134,6 +142,27 @@ int fail(rome_ctx* c, int code, const char* fmt, ...) {
             return fail(ctx, ROME_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),       \
                         __FILE__, __LINE__);                                                           \
     } while (0)
+
+void stage_begin(rome_ctx* ctx, int i) {
+    if (!ctx->profiling) return;
+    (void)hipEventRecord(ctx->ev[i][0], ctx->stream);
+    ctx->ev_used[i] = true;
+}
+void stage_end(rome_ctx* ctx, int i) {
+    if (ctx->profiling) (void)hipEventRecord(ctx->ev[i][1], ctx->stream);
+}
+void stage_collect(rome_ctx* ctx) {
+    if (!ctx->profiling) return;
+    for (int i = 0; i < rome_ctx::kStages; ++i) {
+        if (!ctx->ev_used[i]) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[i][0], ctx->ev[i][1]) == hipSuccess) {
+            ctx->stage_ms[i] += ms;
+            ctx->stage_calls[i] += 1;
+        }
+        ctx->ev_used[i] = false;
+    }
+}
 
 inline int pack_xy(int x, int y) { return (int)(((uint32_t)(y & 0xffff) << 16) | (uint32_t)(x & 0xffff)); }
 
@@ -286,13 +315,19 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     pa.C2f = ctx->C2f.p;
     pa.NMf = ctx->NMf.p;
     pa.MC2bp = ctx->MC2bp.p;
+    stage_begin(ctx, 0);
     HIPCHK(launch_prep_images(s, n, pa));
+    stage_end(ctx, 0);
     // ---- coarse cross term + norms
     HIPCHK(ctx->At.reserve((size_t)n * M));
+    stage_begin(ctx, 1);
     HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
+    stage_end(ctx, 1);
     HIPCHK(ctx->E.reserve((size_t)n * M * T));
     GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, ctx->E.p, Kdim, ctx->Mpad, Npad, M, Ncols, T};
+    stage_begin(ctx, 2);
     HIPCHK(launch_coarse_gemm(s, ga));
+    stage_end(ctx, 2);
     // ---- coarse weights and significance
     for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_maxw, &ctx->c_thr, &ctx->f_dmin, &ctx->f_sumw, &ctx->f_maxw,
                     &ctx->f_thr, &ctx->wnorm})
@@ -307,14 +342,18 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     WeightParams wp = weight_params(ctx);
     WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p, ctx->c_nsig.p,
                      ctx->c_status.p};
+    stage_begin(ctx, 3);
     HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->valid.p, ctx->C2c.p, wc, ctx->sigmask.p, ctx->TW));
+    stage_end(ctx, 3);
     // ---- tasks (image, rotation) of the fine pass
     HIPCHK(ctx->ntask.reserve(n));
     HIPCHK(ctx->nblk.reserve(n));
     HIPCHK(ctx->task_off.reserve(n + 1));
     HIPCHK(ctx->blk_off.reserve(n + 1));
+    stage_begin(ctx, 4);
     HIPCHK(launch_task_build(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p,
                              ctx->blk_off.p));
+    stage_end(ctx, 4);
     long long tot[2];
     HIPCHK(hipMemcpyAsync(&tot[0], ctx->task_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&tot[1], ctx->blk_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -347,8 +386,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(ctx->D2.reserve((size_t)nblocks * NE));
     HIPCHK(ctx->Pfine.reserve((size_t)U * NOR * ctx->Pf));
+    stage_begin(ctx, 5);
     HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d, ctx->Ai_f.p,
                                ctx->R, NOR, ctx->ulist.p, U, ctx->f_xy.p, ctx->Pf, ctx->r_max, ctx->Pfine.p));
+    stage_end(ctx, 5);
     FineArgs fa;
     fa.task_img = ctx->task_img.p;
     fa.task_m = ctx->task_m.p;
@@ -370,9 +411,13 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     WeightOutArgs wf{ctx->f_dmin.p, ctx->f_sumw.p, ctx->f_maxw.p, ctx->f_amax.p, ctx->f_thr.p, nullptr,
                      ctx->f_status.p};
     if (ctx->two_pass) {
+        stage_begin(ctx, 6);
         HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa));
+        stage_end(ctx, 6);
+        stage_begin(ctx, 7);
         HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE,
                                    wf));
+        stage_end(ctx, 7);
         thr_p = ctx->f_thr.p;
         sum_p = ctx->f_sumw.p;
         dmin_p = ctx->f_dmin.p;
@@ -439,7 +484,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.wsum_prior_y = ctx->wsum_prior_y.p;
         pp.wsum_scalars = ctx->wsum_scalars.p;
         pp.wnorm = ctx->wnorm.p;
+        stage_begin(ctx, 8);
         HIPCHK(launch_post(s, n, NOR, NOT, pp));
+        stage_end(ctx, 8);
     } else {
         HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
     }
@@ -518,7 +565,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     sa.meta = ctx->meta.p;
     sa.wsum_sigma2 = ctx->wsum_sigma2.p;
     sa.wsum_scalars = ctx->wsum_scalars.p;
+    stage_begin(ctx, 9);
     HIPCHK(launch_store_sums(s, sa));
+    stage_end(ctx, 9);
     if (res) {
         auto d2h = [&](void* h, const void* d, size_t bytes) -> hipError_t {
             return h ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
@@ -535,6 +584,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         if (res->argmax) std::memcpy(res->argmax, argmax.data(), argmax.size() * sizeof(int));
     }
     HIPCHK(hipStreamSynchronize(s));
+    stage_collect(ctx);
+    ctx->last_U = U;
     (void)fstatus;
     return ROME_OK;
 }
@@ -576,6 +627,9 @@ void rome_destroy(rome_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < rome_ctx::kStages; ++i)
+        for (int j = 0; j < 2; ++j)
+            if (ctx->ev[i][j]) (void)hipEventDestroy(ctx->ev[i][j]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -973,6 +1027,60 @@ int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9
         if (out_re) out_re[q] = h[q].x;
         if (out_im) out_im[q] = h[q].y;
     }
+    return ROME_OK;
+}
+
+int rome_set_profiling(rome_ctx* ctx, int on) {
+    if (!ctx) return ROME_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (on && !ctx->ev[0][0])
+        for (int i = 0; i < rome_ctx::kStages; ++i)
+            for (int j = 0; j < 2; ++j) HIPCHK(hipEventCreate(&ctx->ev[i][j]));
+    ctx->profiling = on != 0;
+    for (int i = 0; i < rome_ctx::kStages; ++i) {
+        ctx->stage_ms[i] = 0.;
+        ctx->stage_calls[i] = 0;
+        ctx->ev_used[i] = false;
+    }
+    return ROME_OK;
+}
+
+int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, int nstages) {
+    if (!ctx || nstages < 0) return ROME_ERR_ARG;
+    for (int i = 0; i < nstages && i < rome_ctx::kStages; ++i) {
+        if (stage_ms) stage_ms[i] = ctx->stage_ms[i];
+        if (stage_calls) stage_calls[i] = ctx->stage_calls[i];
+    }
+    return rome_ctx::kStages;
+}
+
+int rome_get_batch_stats(rome_ctx* ctx, long long* out) {
+    if (!ctx || !out) return ROME_ERR_ARG;
+    out[0] = ctx->last_n;
+    out[1] = ctx->last_tasks;
+    out[2] = ctx->last_blocks;
+    out[3] = ctx->last_U;
+    out[4] = ctx->Pc;
+    out[5] = ctx->Pf;
+    out[6] = ctx->Mtot;
+    out[7] = ctx->T;
+    return ROME_OK;
+}
+
+int rome_project_device(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, void* out) {
+    if (!ctx || !A9 || !out || nrot < 0) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K || !ctx->have_ref[iclass]) return fail(ctx, ROME_ERR_ARG, "class not set");
+    if (size <= 0 || (size & 1) || size / 2 > ctx->r_max) return fail(ctx, ROME_ERR_ARG, "bad projection size");
+    if (nrot == 0) return ROME_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<double> ai(6 * (size_t)nrot);
+    for (int r = 0; r < nrot; ++r) rot6(A9 + 9 * (size_t)r, &ai[6 * r]);
+    DevBuf<double> dai;
+    HIPCHK(dai.upload(ai.data(), ai.size(), ctx->stream));
+    HIPCHK(launch_project_plane(ctx->stream, ctx->grids.p + ctx->grid_n * iclass, ctx->pad, ctx->padF, ctx->cgrid,
+                                ctx->is3d, dai.p, nrot, size, std::min(ctx->r_max, size / 2), (float2*)out));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return ROME_OK;
 }
 

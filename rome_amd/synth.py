@@ -88,6 +88,26 @@ def fourier_grid(vol, current_size, padding_factor=2, gridding=True):
     return out.reshape(-1), pad
 
 
+def blob_volume_fast(ori, nblobs, sigma_range, radius, rng):
+    """3D blob volume evaluated on a +-4 sigma sub-box per blob (same model as blob_volume)."""
+    vol = np.zeros((ori, ori, ori))
+    c0 = ori // 2
+    for _ in range(nblobs):
+        while True:
+            c = rng.uniform(-radius, radius, 3)
+            if np.sum(c * c) <= radius * radius:
+                break
+        s = rng.uniform(*sigma_range)
+        amp = rng.uniform(0.5, 1.5)
+        h = int(np.ceil(4 * s))
+        lo = [max(0, int(np.floor(ci)) + c0 - h) for ci in c]
+        hi = [min(ori, int(np.floor(ci)) + c0 + h + 2) for ci in c]
+        ax = [np.arange(l, u) - c0 - ci for l, u, ci in zip(lo, hi, c)]
+        r2 = ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2
+        vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += amp * np.exp(-r2 / (2 * s * s))
+    return vol
+
+
 # ---------------- geometry ----------------
 def euler_matrix(rot, tilt, psi):
     a, b, g = np.deg2rad(rot), np.deg2rad(tilt), np.deg2rad(psi)
@@ -282,6 +302,81 @@ def make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, psi_ste
     return prob, batch, truth
 
 
+def make_particles_device(engine, nr_classes, n, size, ori, angpix, offset_range, snr, seed, chunk=2000,
+                          is3d=True):
+    """Particles generated on the GPU: projector of the engine (HIP) + CTF / shift / noise in torch.
+
+    Returns device tensors: F (complex64 [n][F2]) used as both masked and unmasked image, CTF (float32
+    [n][F2]), and the host-side truth (class, rot, tilt, psi, sx, sy) plus the noise sigma2 per component.
+    """
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    F2 = size * (size // 2 + 1)
+    cls = rng.integers(0, nr_classes, n)
+    if is3d:
+        rot = rng.uniform(-180, 180, n)
+        tilt = np.rad2deg(np.arccos(rng.uniform(-1, 1, n)))
+    else:
+        rot = np.zeros(n)
+        tilt = np.zeros(n)
+    psi = rng.uniform(0, 360, n)
+    sh = rng.uniform(-offset_range, offset_range, (n, 2))
+    defU = rng.uniform(1e4, 3e4, n)
+    defV = rng.uniform(1e4, 3e4, n)
+    ang = rng.uniform(0, 180, n)
+    F = torch.empty((n, F2), dtype=torch.complex64, device=dev)
+    x, y = halfplane_xy(size)
+    xt = torch.tensor(x, dtype=torch.float64, device=dev)
+    yt = torch.tensor(y, dtype=torch.float64, device=dev)
+    # CTF (src/ctf.h:151-205) evaluated in FP64 on the device
+    lam = 12.2643247 / np.sqrt(300e3 * (1. + 300e3 * 0.978466e-6))
+    K1 = PI / 2 * 2 * lam
+    K2 = PI / 2 * 2.7e7 * lam ** 3
+    q0 = 0.1
+    K3 = np.sqrt(1 - q0 * q0)
+    xs = ori * angpix
+    X = xt / xs
+    Y = yt / xs
+    u2 = X * X + Y * Y
+    atan = torch.atan2(Y, X)
+    origin = (X.abs() < 1e-6) & (Y.abs() < 1e-6)
+    ctf = torch.empty((n, F2), dtype=torch.float32, device=dev)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        idx = np.arange(c0, c1)
+        for k in range(nr_classes):
+            sel = idx[cls[idx] == k]
+            if len(sel) == 0:
+                continue
+            A = np.stack([euler_matrix(rot[i], tilt[i], psi[i]) for i in sel])
+            buf = torch.empty((len(sel), F2), dtype=torch.complex64, device=dev)
+            engine.project_device(k, size, A, buf.data_ptr())
+            F[torch.tensor(sel, device=dev)] = buf
+        dU = torch.tensor(defU[c0:c1], device=dev)[:, None]
+        dV = torch.tensor(defV[c0:c1], device=dev)[:, None]
+        az = torch.tensor(np.deg2rad(ang[c0:c1]), device=dev)[:, None]
+        davg = -(dU + dV) * 0.5
+        ddev = -(dU - dV) * 0.5
+        deltaf = torch.where(origin[None, :], torch.zeros_like(davg), davg + ddev * torch.cos(2 * (atan[None, :] - az)))
+        arg = K1 * deltaf * u2[None, :] + K2 * u2[None, :] ** 2
+        c = -(K3 * torch.sin(arg) - q0 * torch.cos(arg))
+        ctf[c0:c1] = c.float()
+        sx = torch.tensor(sh[c0:c1, 0], device=dev)[:, None]
+        sy = torch.tensor(sh[c0:c1, 1], device=dev)[:, None]
+        ph = 2 * PI * (xt[None, :] * sx + yt[None, :] * sy) / ori
+        F[c0:c1] = F[c0:c1] * c.to(torch.complex64) * torch.polar(torch.ones_like(ph), ph).to(torch.complex64)
+    inside = torch.tensor((x * x + y * y <= (size // 2) ** 2).astype(np.float32), device=dev)
+    sig_pow = float(((F.abs() ** 2) * inside).sum() / (inside.sum() * n))
+    sigma2 = 0.5 * sig_pow / snr
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed) + 2)
+    noise = torch.randn((n, F2, 2), generator=g, device=dev, dtype=torch.float32) * float(np.sqrt(sigma2))
+    F += torch.view_as_complex(noise)
+    truth = np.column_stack([cls, rot, tilt, psi, sh])
+    return F, ctf, truth, sigma2
+
+
 def engine_config(prob):
     keys = ("ori_size", "current_size", "coarse_size", "nr_classes", "nr_groups", "adaptive_oversampling",
             "nr_dir", "nr_psi", "nr_trans", "adaptive_fraction", "sigma2_fudge", "sigma2_offset",
@@ -291,13 +386,18 @@ def engine_config(prob):
     return cfg
 
 
-def load_engine(engine, prob):
-    """Configure an Engine with a problem dict (same tables the oracle receives)."""
+def load_engine_tables(engine, prob):
+    """configure + sampling tables + reference grids (everything but the model)."""
     engine.configure(**engine_config(prob))
     engine.set_sampling(prob["pose_coarse"], prob["pose_fine"], prob["shift_coarse_x"], prob["shift_coarse_y"],
                         prob["shift_fine_x"], prob["shift_fine_y"], prob["trans_x"], prob["trans_y"])
-    engine.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
-                     prob["prior_offy"])
     for k in range(prob["nr_classes"]):
         engine.set_reference(k, prob["grids"][k])
+
+
+def load_engine(engine, prob):
+    """Configure an Engine with a problem dict (same tables the oracle receives)."""
+    load_engine_tables(engine, prob)
+    engine.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
+                     prob["prior_offy"])
     engine.reset_sums()
