@@ -84,7 +84,13 @@ struct rome_ctx {
     DevBuf<double> Ai_c, Ai_f, sintab, costab, d_sxc, d_syc, d_sxf, d_syf, d_tx, d_ty;
     DevBuf<int> c_nfine, c_ires, c_xy, f_nfine, f_ires, f_xy;
     DevBuf<float2> Sc, Sf;
-    DevBuf<double> sigma2, pdf_dir, lpdf_dir, prior_x, prior_y, logsigma2;
+    DevBuf<double> sigma2, pdf_dir, lpdf_dir, prior_x, prior_y, logsigma2, pdf_m, lpdf_m;
+    DevBuf<int> k_m;
+    double lpdf_max = 0.;
+    DevBuf<int> emin_key;
+    DevBuf<double> cand_w, cand_du;
+    DevBuf<long long> cand_e;
+    static constexpr int kCandCap = 8192;
     DevBuf<unsigned char> valid;
     DevBuf<float> FcT;
     // sums
@@ -247,8 +253,11 @@ WeightParams weight_params(rome_ctx* ctx) {
     wp.R = ctx->R;
     wp.Mtot = ctx->Mtot;
     wp.T = ctx->T;
-    wp.pdf_dir = ctx->pdf_dir.p;
-    wp.lpdf_dir = ctx->lpdf_dir.p;
+    wp.magicT = ((1ULL << 40) + (unsigned long long)ctx->T - 1) / (unsigned long long)ctx->T;
+    wp.pdf_m = ctx->pdf_m.p;
+    wp.lpdf_m = ctx->lpdf_m.p;
+    wp.k_m = ctx->k_m.p;
+    wp.lpdf_max = ctx->lpdf_max;
     wp.trans_x = ctx->d_tx.p;
     wp.trans_y = ctx->d_ty.p;
     wp.old_offx = ctx->in_offx.p;
@@ -324,7 +333,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
     stage_end(ctx, 1);
     HIPCHK(ctx->E.reserve((size_t)n * M * T));
-    GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, ctx->E.p, Kdim, ctx->Mpad, Npad, M, Ncols, T};
+    HIPCHK(ctx->emin_key.reserve(n));
+    HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
+    GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, ctx->E.p, ctx->valid.p, ctx->emin_key.p, Kdim, ctx->Mpad, Npad, M,
+                Ncols, T};
     stage_begin(ctx, 2);
     HIPCHK(launch_coarse_gemm(s, ga));
     stage_end(ctx, 2);
@@ -343,7 +355,11 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p, ctx->c_nsig.p,
                      ctx->c_status.p};
     stage_begin(ctx, 3);
-    HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->valid.p, ctx->C2c.p, wc, ctx->sigmask.p, ctx->TW));
+    HIPCHK(ctx->cand_w.reserve((size_t)n * rome_ctx::kCandCap));
+    HIPCHK(ctx->cand_du.reserve((size_t)n * rome_ctx::kCandCap));
+    HIPCHK(ctx->cand_e.reserve((size_t)n * rome_ctx::kCandCap));
+    const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
+    HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->C2c.p, ctx->emin_key.p, wc, ctx->sigmask.p, ctx->TW, cand));
     stage_end(ctx, 3);
     // ---- tasks (image, rotation) of the fine pass
     HIPCHK(ctx->ntask.reserve(n));
@@ -361,7 +377,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(hipMemcpyAsync(cstatus.data(), ctx->c_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int i = 0; i < n; ++i)
-        if (cstatus[i] == 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
+        if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
     const long long ntasks = tot[0], nblocks = tot[1];
     ctx->last_n = n;
     ctx->last_tasks = ntasks;
@@ -416,7 +432,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         stage_end(ctx, 6);
         stage_begin(ctx, 7);
         HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE,
-                                   wf));
+                                   wf, cand));
         stage_end(ctx, 7);
         thr_p = ctx->f_thr.p;
         sum_p = ctx->f_sumw.p;
@@ -790,13 +806,28 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
     ctx->prior_y_h.assign(prior_offy, prior_offy + K);
     std::vector<double> lp((size_t)K * D);
     std::vector<unsigned char> valid((size_t)ctx->Mtot);
+    std::vector<double> pm((size_t)ctx->Mtot), lpm((size_t)ctx->Mtot);
+    std::vector<int> km((size_t)ctx->Mtot);
     for (int k = 0; k < K; ++k)
         for (int d = 0; d < D; ++d) {
             const double p = pdf_direction[(size_t)k * D + d];
             lp[(size_t)k * D + d] = p > 0 ? log(p) : -INFINITY;
+            // the reference skips (class, direction) pairs with a non-positive prior
+            // (src/map3d_optimizer.cpp:1204-1206): no diff2, no weight, no significance entry
             const bool ok = pdf_class[k] > 0. && p > 0.;
-            for (int s = 0; s < ctx->PSI; ++s) valid[(size_t)k * ctx->R + d * ctx->PSI + s] = ok;
+            for (int s = 0; s < ctx->PSI; ++s) {
+                const size_t m = (size_t)k * ctx->R + d * ctx->PSI + s;
+                valid[m] = ok;
+                pm[m] = ok ? p : 0.;
+                lpm[m] = ok ? log(p) : -INFINITY;
+                km[m] = k;
+            }
         }
+    HIPCHK(ctx->pdf_m.upload(pm.data(), pm.size(), ctx->stream));
+    HIPCHK(ctx->lpdf_m.upload(lpm.data(), lpm.size(), ctx->stream));
+    HIPCHK(ctx->k_m.upload(km.data(), km.size(), ctx->stream));
+    ctx->lpdf_max = -INFINITY;
+    for (double v : lpm) ctx->lpdf_max = std::max(ctx->lpdf_max, v);
     // logsigma2 per group (storeWeightedSums, src/map3d_optimizer.cpp:1998-2008): same loop order as the reference
     std::vector<double> ls(G, 0.);
     const int sz = ctx->fs, F = sz / 2 + 1;

@@ -49,13 +49,18 @@ struct GemmArgs {
     const float* B;   // [K][Npad]
     const float* At;  // [n][Mtot]
     float* E;         // [n][Mtot][T]
+    const unsigned char* valid;  // [Mtot] rotation has a positive prior
+    int* emin_key;    // [n] ordered-int key of the per-image minimum of E (init INT_MAX)
     int K, Mpad, Npad, Mtot, Ncols, T;
 };
 
 struct WeightParams {
     int K, D, PSI, R, Mtot, T;
-    const double* pdf_dir;   // [K][D]
-    const double* lpdf_dir;  // [K][D] log (or -inf)
+    unsigned long long magicT;  // ceil(2^40 / T): e / T == (e * magicT) >> 40 for e < 2^40 / T^2
+    const double* pdf_m;     // [Mtot] pdf_direction of the rotation's (class, direction); 0 if skipped
+    const double* lpdf_m;    // [Mtot] log(pdf_m) or -inf
+    double lpdf_max;         // max over rotations of lpdf_m
+    const int* k_m;          // [Mtot] class of the rotation
     const double* trans_x;   // [T] unperturbed translations
     const double* trans_y;
     const double* old_offx;  // [n]
@@ -167,11 +172,19 @@ hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a);
 hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
                                const float* MC2c, float* At);
 hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a);
-hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const unsigned char* valid,
-                                 const double* C2c, const WeightOutArgs& wo, uint32_t* sigmask, int TW);
+struct CandArgs {  // per-image candidate list of the weight pass: [n][cap]
+    double* w;
+    double* du;
+    long long* e;
+    int cap;
+};
+
+hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const double* C2c,
+                                 const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                                 const CandArgs& cand);
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
-                               const WeightOutArgs& wo);
+                               const WeightOutArgs& wo, const CandArgs& cand);
 hipError_t launch_task_build(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, int* ntask, int* nblk,
                              long long* task_off, long long* blk_off);
 hipError_t launch_task_emit(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, const long long* task_off,

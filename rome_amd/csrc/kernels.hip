@@ -13,115 +13,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_util.h"
 #include "kernels.h"
 
 namespace rome {
-
-#define DEV __device__ __forceinline__
-
-// ------------------------------------------------------------------ helpers
-// Separately rounded FP64 multiply / add: the reference's host arithmetic has no FMA,
-// and table indices / floor() decisions depend on the last bit.
-DEV double dmul(double a, double b) {
-#pragma clang fp contract(off)
-    return a * b;
-}
-DEV double dadd(double a, double b) {
-#pragma clang fp contract(off)
-    return a + b;
-}
-
-DEV float linf(float a, float l, float h) { return l + (h - l) * a; }
-
-struct Coords {
-    int x0, y0, z0;
-    float fx, fy, fz;
-    bool neg;
-};
-
-// Logical -> grid coordinates exactly as MyProjector::projectOneTile computes them
-// (src/reconstruct.cpp:517-560), FP64 without contraction so that floor() agrees.
-DEV Coords map_coords(const double* Ai, int x, int y, int c, bool is3d) {
-#pragma clang fp contract(off)
-    double xp = dadd(dmul(Ai[0], (double)x), dmul(Ai[1], (double)y));
-    double yp = dadd(dmul(Ai[2], (double)x), dmul(Ai[3], (double)y));
-    double zp = is3d ? dadd(dmul(Ai[4], (double)x), dmul(Ai[5], (double)y)) : 0.0;
-    Coords r;
-    r.neg = xp < 0;
-    if (r.neg) {
-        xp = -xp;
-        yp = -yp;
-        zp = -zp;
-    }
-    double fx0 = floor(xp), fy0 = floor(yp), fz0 = floor(zp);
-    r.x0 = (int)fx0;
-    r.y0 = (int)fy0 + c;
-    r.z0 = (int)fz0 + c;
-    r.fx = (float)(xp - fx0);
-    r.fy = (float)(yp - fy0);
-    r.fz = (float)(zp - fz0);
-    return r;
-}
-
-// Trilinear (3D) / bilinear (2D) gather of the half-transform with Friedel mirror.
-DEV float2 gather(const float2* __restrict__ g, int pad, int padF, const Coords& q, bool is3d) {
-    float2 out;
-    if (is3d) {
-        const float2* p00 = g + ((size_t)q.z0 * pad + q.y0) * padF + q.x0;
-        const float2* p01 = p00 + padF;                  // (z0, y1)
-        const float2* p10 = p00 + (size_t)pad * padF;    // (z1, y0)
-        const float2* p11 = p10 + padF;                  // (z1, y1)
-        float2 d000 = p00[0], d001 = p00[1], d010 = p01[0], d011 = p01[1];
-        float2 d100 = p10[0], d101 = p10[1], d110 = p11[0], d111 = p11[1];
-        float dx00r = linf(q.fx, d000.x, d001.x), dx00i = linf(q.fx, d000.y, d001.y);
-        float dx01r = linf(q.fx, d100.x, d101.x), dx01i = linf(q.fx, d100.y, d101.y);
-        float dx10r = linf(q.fx, d010.x, d011.x), dx10i = linf(q.fx, d010.y, d011.y);
-        float dx11r = linf(q.fx, d110.x, d111.x), dx11i = linf(q.fx, d110.y, d111.y);
-        float dxy0r = linf(q.fy, dx00r, dx10r), dxy0i = linf(q.fy, dx00i, dx10i);
-        float dxy1r = linf(q.fy, dx01r, dx11r), dxy1i = linf(q.fy, dx01i, dx11i);
-        out.x = linf(q.fz, dxy0r, dxy1r);
-        out.y = linf(q.fz, dxy0i, dxy1i);
-    } else {
-        const float2* p0 = g + (size_t)q.y0 * padF + q.x0;
-        const float2* p1 = p0 + padF;
-        float2 d00 = p0[0], d01 = p0[1], d10 = p1[0], d11 = p1[1];
-        float dx0r = linf(q.fx, d00.x, d01.x), dx0i = linf(q.fx, d00.y, d01.y);
-        float dx1r = linf(q.fx, d10.x, d11.x), dx1i = linf(q.fx, d10.y, d11.y);
-        out.x = linf(q.fy, dx0r, dx1r);
-        out.y = linf(q.fy, dx0i, dx1i);
-    }
-    if (q.neg) out.y = -out.y;
-    return out;
-}
-
-DEV int px_x(int xy) { return xy & 0xffff; }
-DEV int px_y(int xy) { return (int)(short)(xy >> 16); }
-
-DEV float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-
-template <typename T>
-DEV T wave_sum(T v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-DEV double wave_min(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// block-wide reduction over blockDim.x threads (multiple of 64), result broadcast
-template <typename T, typename Op>
-DEV T block_reduce(T v, Op op, T* scratch) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
-    __syncthreads();
-    if (lane == 0) scratch[wid] = v;
-    __syncthreads();
-    T r = scratch[0];
-    for (int w = 1; w < nw; ++w) r = op(r, scratch[w]);
-    __syncthreads();
-    return r;
-}
 
 // ------------------------------------------------------------------ phase tables
 // Tabulated shift phase of shiftImageInFourierTransform (src/image.h:1478-1574):
@@ -296,6 +191,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
     constexpr int BM = 128, BN = 128, BK = 16;
     __shared__ float As[2][BK][BM];
     __shared__ float Bs[2][BK][BN];
+    __shared__ int colmin[BN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     // XCD-aware tile order: consecutive workgroups go round-robin over the 8 XCDs, so
@@ -357,6 +253,8 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
         __syncthreads();
     }
     // epilogue: lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
+    if (tid < BN) colmin[tid] = 0x7fffffff;
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -364,369 +262,36 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
             const int col = n0 + wn * 64 + j * 32 + (lane & 31);
             if (col >= a.Ncols) continue;
             const int img = col / a.T, t = col - img * a.T;
+            float emin = INFINITY;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (row < a.Mtot) {
                     const float at = a.At[(size_t)img * a.Mtot + row];
-                    a.E[((size_t)img * a.Mtot + row) * a.T + t] = at - acc[i][j][r];
+                    const float e = at - acc[i][j][r];
+                    a.E[((size_t)img * a.Mtot + row) * a.T + t] = e;
+                    if (a.valid[row]) emin = fminf(emin, e);
                 }
             }
+            // per-column minimum in LDS, per-image minimum below (weight pass input)
+            if (emin < INFINITY) atomicMin(&colmin[wn * 64 + j * 32 + (lane & 31)], float_order_key(emin));
         }
-}
-
-// ------------------------------------------------------------------ weights + significance (K3)
-// One block (1024 threads) per image; see src/map3d_optimizer.cpp:1610-1931.
-// Entries are visited through an accessor; both passes share the code.
-struct CoarseAcc {
-    const float* E;    // [n][M][T]
-    const unsigned char* valid;
-    int M, T;
-    DEV size_t count() const { return (size_t)M * T; }
-    DEV bool ok(int, size_t e) const { return valid[e / T] != 0; }
-    DEV double d(int img, size_t e) const { return (double)E[(size_t)img * M * T + e]; }
-    DEV int m_of(int, size_t e) const { return (int)(e / T); }
-    DEV int t_of(int, size_t e) const { return (int)(e % T); }
-};
-struct FineAcc {
-    const double* D2;           // [blocks][NE]
-    const int* blk_task;        // block -> task
-    const int* blk_t;           // block -> t
-    const int* task_m;          // task -> m
-    const long long* img_blk;   // [n+1] block offsets
-    int NE;
-    DEV size_t count_img(int img) const { return (size_t)(img_blk[img + 1] - img_blk[img]) * NE; }
-    DEV size_t base(int img) const { return (size_t)img_blk[img] * NE; }
-};
-
-struct WeightOut {
-    double* dmin;    // [n] min diff2
-    double* sumw;    // [n]
-    double* maxw;    // [n]
-    long long* amax; // [n] argmax entry (pass-local index)
-    double* thr;     // [n] significance threshold
-    int* nsig;       // [n] number of significant samples (coarse pass)
-    int* status;     // [n] 0 ok, 1 zero significant, 2 unresolved tie bucket
-};
-
-DEV double pdf_offset(double tx, double ty, double ox, double oy, double px, double py, double s2off) {
-    const double offx = tx + ox, offy = ty + oy;
-    const double td = (offx - px) * (offx - px) + (offy - py) * (offy - py);
-    if (s2off < 0.0001) return td > 0. ? 0. : 1.;
-    return exp(td / (-2. * s2off)) / (2. * 3.14159265358979323846 * s2off);
-}
-
-template <int NT>
-struct WShared {
-    double poff[kMaxKT];
-    double lpoff[kMaxKT];
-    double hsum[kBinCount];
-    unsigned int hcnt[kBinCount];
-    double cw[kSortCap];
-    long long ce[kSortCap];
-    double red_d[NT / 64];
-    long long red_l[NT / 64];
-    int ncand;
-    double lo, width;
-    double mbefore;
-    long long cbefore;
-    int done;
-};
-
-// Generic per-image weight pass.  get(e, &valid, &d, &k, &t, &dir) describes entry e.
-template <int NT, typename Get>
-DEV void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, WeightOut wo,
-                     WShared<NT>& sh, bool coarse_pass, uint32_t* sigmask, int T, int TW) {
-    const int tid = threadIdx.x;
-    // pdf_offset table per (k, t) for this image (src/map3d_optimizer.cpp:1700-1716)
-    for (int q = tid; q < wp.K * wp.T; q += NT) {
-        const int k = q / wp.T, t = q % wp.T;
-        const double po = pdf_offset(wp.trans_x[t], wp.trans_y[t], wp.old_offx[img], wp.old_offy[img],
-                                     wp.prior_x[k], wp.prior_y[k], wp.sigma2_offset);
-        sh.poff[q] = po;
-        sh.lpoff[q] = po > 0 ? log(po) : -INFINITY;
-    }
-    for (int b = tid; b < kBinCount; b += NT) {
-        sh.hsum[b] = 0.;
-        sh.hcnt[b] = 0;
-    }
     __syncthreads();
-    // pass 1: minimum of diff2 and of u = d - ln(pdf) (histogram origin)
-    double mn = INFINITY, umin = INFINITY;
-    for (size_t e = tid; e < count; e += NT) {
-        bool v;
-        double d;
-        int k, t, dir;
-        get(e, v, d, k, t, dir);
-        if (!v) continue;
-        mn = fmin(mn, d);
-        const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
-        umin = fmin(umin, d - lp);
+    if (tid == 0) {  // at most a few images per tile: one global atomic each
+        int cur_img = -1, key = 0x7fffffff;
+        for (int c = 0; c < BN; ++c) {
+            const int col = n0 + c;
+            if (col >= a.Ncols) break;
+            const int img = col / a.T;
+            if (img != cur_img) {
+                if (cur_img >= 0 && key != 0x7fffffff) atomicMin(&a.emin_key[cur_img], key);
+                cur_img = img;
+                key = 0x7fffffff;
+            }
+            key = min(key, colmin[c]);
+        }
+        if (cur_img >= 0 && key != 0x7fffffff) atomicMin(&a.emin_key[cur_img], key);
     }
-    mn = block_reduce(mn, [](double a, double b) { return fmin(a, b); }, sh.red_d);
-    umin = block_reduce(umin, [](double a, double b) { return fmin(a, b); }, sh.red_d);
-    const double dmin = mn + cimg;  // min over (d + c) is (min d) + c: rounding is monotone
-    // pass 2: weights, sum, max / argmax (first in visiting order), level-1 histogram
-    double sum = 0., mx = -1.;
-    long long am = -1;
-    for (size_t e = tid; e < count; e += NT) {
-        bool v;
-        double d;
-        int k, t, dir;
-        get(e, v, d, k, t, dir);
-        if (!v) continue;
-        const double dd = (d + cimg) - dmin;
-        double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
-        w = dd > 700. ? 0. : w * exp(-dd);
-        sum += w;
-        if (w > mx) {  // entries are visited in increasing e per thread: keeps the first maximum
-            mx = w;
-            am = (long long)e;
-        }
-        if (w > 0.) {
-            const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
-            const double u = (d - lp) - umin;
-            const double bf = u * kBinScale;
-            if (bf < (double)kBinCount) {
-                const int b = (int)bf;
-                atomicAdd(&sh.hsum[b], w);
-                atomicAdd(&sh.hcnt[b], 1u);
-            }
-        }
-    }
-    sum = block_reduce(sum, [](double a, double b) { return a + b; }, sh.red_d);
-    // argmax: max weight, ties -> smallest entry index (first in the reference's loop order)
-    {
-        double m2 = block_reduce(mx, [](double a, double b) { return fmax(a, b); }, sh.red_d);
-        long long cand = (mx == m2 && am >= 0) ? am : (long long)0x7fffffffffffffffLL;
-        long long e2 = block_reduce(cand, [](long long a, long long b) { return a < b ? a : b; }, sh.red_l);
-        mx = m2;
-        am = e2;
-    }
-    const double target = wp.adaptive_fraction * sum;
-    // significance threshold: find crossing bucket, refine until it fits the sort buffer
-    if (tid == 0) {
-        sh.lo = 0.;
-        sh.width = 1.0 / kBinScale;
-        sh.mbefore = 0.;
-        sh.cbefore = 0;
-        sh.done = 0;
-    }
-    __syncthreads();
-    int level = 0;
-    double thr = 0.;
-    long long nsig = 0;
-    int status = 0;
-    for (;;) {
-        // scan buckets (thread 0)
-        if (tid == 0) {
-            double cum = sh.mbefore;
-            long long cnt = sh.cbefore;
-            int bstar = -1;
-            int last_nonempty = -1;
-            for (int b = 0; b < kBinCount; ++b) {
-                if (sh.hcnt[b] == 0) continue;
-                last_nonempty = b;
-                if (cum + sh.hsum[b] > target) {
-                    bstar = b;
-                    break;
-                }
-                cum += sh.hsum[b];
-                cnt += sh.hcnt[b];
-            }
-            if (bstar < 0) {  // rounding: crossing not reached -> use the last bucket
-                bstar = last_nonempty;
-                if (bstar >= 0) {
-                    cum -= sh.hsum[bstar];
-                    cnt -= sh.hcnt[bstar];
-                }
-            }
-            if (bstar < 0) {
-                sh.done = 2;  // no positive weight
-            } else {
-                sh.mbefore = cum;
-                sh.cbefore = cnt;
-                sh.lo = sh.lo + bstar * sh.width;
-                sh.done = (sh.hcnt[bstar] <= (unsigned)kSortCap || level >= 2) ? 1 : 0;
-                sh.width = sh.width / (double)kBinCount;
-            }
-            sh.ncand = 0;
-        }
-        __syncthreads();
-        if (sh.done == 2) {
-            status = 1;
-            break;
-        }
-        const double lo = sh.lo;
-        const double wbin = sh.width * kBinCount;  // width of the selected bucket
-        const double hi = lo + wbin;
-        if (sh.done == 1) {
-            // collect candidates of the selected bucket
-            for (size_t e = tid; e < count; e += NT) {
-                bool v;
-                double d;
-                int k, t, dir;
-                get(e, v, d, k, t, dir);
-                if (!v) continue;
-                const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
-                const double u = (d - lp) - umin;
-                if (!(u >= lo && u < hi)) continue;
-                const double dd = (d + cimg) - dmin;
-                double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
-                w = dd > 700. ? 0. : w * exp(-dd);
-                if (!(w > 0.)) continue;
-                const int slot = atomicAdd(&sh.ncand, 1);
-                if (slot < kSortCap) {
-                    sh.cw[slot] = w;
-                    sh.ce[slot] = (long long)e;
-                }
-            }
-            __syncthreads();
-            int nc = sh.ncand < kSortCap ? sh.ncand : kSortCap;
-            if (sh.ncand > kSortCap) status = 2;
-            // bitonic sort descending by weight over the next power of two
-            int np2 = 1;
-            while (np2 < nc) np2 <<= 1;
-            for (int q = nc + tid; q < np2; q += NT) {
-                sh.cw[q] = -1.0;
-                sh.ce[q] = -1;
-            }
-            __syncthreads();
-            for (int ksz = 2; ksz <= np2; ksz <<= 1)
-                for (int j = ksz >> 1; j > 0; j >>= 1) {
-                    for (int q = tid; q < np2; q += NT) {
-                        const int ixj = q ^ j;
-                        if (ixj > q) {
-                            const bool desc = ((q & ksz) == 0);
-                            const double a1 = sh.cw[q], a2 = sh.cw[ixj];
-                            if (desc ? (a1 < a2) : (a1 > a2)) {
-                                sh.cw[q] = a2;
-                                sh.cw[ixj] = a1;
-                                const long long t1 = sh.ce[q];
-                                sh.ce[q] = sh.ce[ixj];
-                                sh.ce[ixj] = t1;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                }
-            if (tid == 0) {
-                double frac = sh.mbefore;
-                long long cnt = sh.cbefore;
-                double my = 0.;
-                for (int q = 0; q < nc; ++q) {
-                    ++cnt;
-                    my = sh.cw[q];
-                    frac += my;
-                    if (frac > target) break;
-                }
-                sh.red_d[0] = my;
-                sh.red_l[0] = cnt;
-            }
-            __syncthreads();
-            thr = sh.red_d[0];
-            nsig = sh.red_l[0];
-            __syncthreads();
-            break;
-        }
-        // refine: histogram inside the selected bucket
-        for (int b = tid; b < kBinCount; b += NT) {
-            sh.hsum[b] = 0.;
-            sh.hcnt[b] = 0;
-        }
-        __syncthreads();
-        const double sw = sh.width;
-        for (size_t e = tid; e < count; e += NT) {
-            bool v;
-            double d;
-            int k, t, dir;
-            get(e, v, d, k, t, dir);
-            if (!v) continue;
-            const double lp = wp.lpdf_dir[(size_t)k * wp.D + dir] + sh.lpoff[k * wp.T + t];
-            const double u = (d - lp) - umin;
-            if (!(u >= lo && u < hi)) continue;
-            const double dd = (d + cimg) - dmin;
-            double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
-            w = dd > 700. ? 0. : w * exp(-dd);
-            if (!(w > 0.)) continue;
-            int b = (int)((u - lo) / sw);
-            b = b < 0 ? 0 : (b >= kBinCount ? kBinCount - 1 : b);
-            atomicAdd(&sh.hsum[b], w);
-            atomicAdd(&sh.hcnt[b], 1u);
-        }
-        __syncthreads();
-        ++level;
-    }
-    if (tid == 0) {
-        wo.dmin[img] = dmin;
-        wo.sumw[img] = sum;
-        wo.maxw[img] = mx;
-        wo.amax[img] = am;
-        wo.thr[img] = thr;
-        if (wo.nsig) wo.nsig[img] = (int)nsig;
-        wo.status[img] = status;
-    }
-    // significance bits (coarse): w >= threshold, exactly as the reference compares
-    if (coarse_pass && sigmask) {
-        uint32_t* msk = sigmask + (size_t)img * (count / T) * TW;
-        for (size_t q = tid; q < (count / T) * TW; q += NT) msk[q] = 0u;
-        __syncthreads();
-        for (size_t e = tid; e < count; e += NT) {
-            bool v;
-            double d;
-            int k, t, dir;
-            get(e, v, d, k, t, dir);
-            if (!v) continue;
-            const double dd = (d + cimg) - dmin;
-            double w = wp.pdf_dir[(size_t)k * wp.D + dir] * sh.poff[k * wp.T + t];
-            w = dd > 700. ? 0. : w * exp(-dd);
-            if (w >= thr) {
-                const size_t m = e / T;
-                const int tt = (int)(e % T);
-                atomicOr(&msk[m * TW + (tt >> 5)], 1u << (tt & 31));
-            }
-        }
-    }
-}
-
-__global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const float* __restrict__ E,
-                                                         const unsigned char* __restrict__ valid,
-                                                         const double* __restrict__ C2c, WeightOut wo,
-                                                         uint32_t* __restrict__ sigmask, int TW) {
-    __shared__ WShared<1024> sh;
-    const int img = blockIdx.x;
-    const int M = wp.Mtot, T = wp.T, R = wp.R, PSI = wp.PSI;
-    const float* Ei = E + (size_t)img * M * T;
-    auto get = [&](size_t e, bool& v, double& d, int& k, int& t, int& dir) {
-        const int m = (int)(e / T);
-        t = (int)(e - (size_t)m * T);
-        v = valid[m] != 0;
-        k = m / R;
-        dir = (m - k * R) / PSI;
-        d = (double)Ei[e];
-    };
-    weight_pass<1024>(img, (size_t)M * T, C2c[img], wp, get, wo, sh, true, sigmask, T, TW);
-}
-
-__global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
-                                                       const int* __restrict__ blk_task, const int* __restrict__ blk_t,
-                                                       const int* __restrict__ task_m,
-                                                       const long long* __restrict__ img_blk, int NE, WeightOut wo) {
-    __shared__ WShared<1024> sh;
-    const int img = blockIdx.x;
-    const long long b0 = img_blk[img];
-    const size_t count = (size_t)(img_blk[img + 1] - b0) * NE;
-    const int R = wp.R, PSI = wp.PSI;
-    auto get = [&](size_t e, bool& v, double& d, int& k, int& t, int& dir) {
-        const long long b = b0 + (long long)(e / NE);
-        const int m = task_m[blk_task[b]];
-        t = blk_t[b];
-        k = m / R;
-        dir = (m - k * R) / PSI;
-        d = D2[(size_t)b0 * NE + e];
-        v = true;
-    };
-    weight_pass<1024>(img, count, 0.0, wp, get, wo, sh, false, nullptr, wp.T, 1);
 }
 
 // ------------------------------------------------------------------ task building
@@ -1238,23 +803,6 @@ hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot,
 hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a) {
     const int tiles = (a.Mpad / 128) * (a.Npad / 128);
     k_coarse_gemm<<<tiles, 256, 0, s>>>(a);
-    LAUNCH_CHECK();
-    return hipSuccess;
-}
-
-hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const unsigned char* valid,
-                                 const double* C2c, const WeightOutArgs& wo, uint32_t* sigmask, int TW) {
-    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
-    k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, valid, C2c, o, sigmask, TW);
-    LAUNCH_CHECK();
-    return hipSuccess;
-}
-
-hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
-                               const int* blk_t, const int* task_m, const long long* img_blk, int NE,
-                               const WeightOutArgs& wo) {
-    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
-    k_weights_fine<<<n, 1024, 0, s>>>(wp, D2, blk_task, blk_t, task_m, img_blk, NE, o);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -1,0 +1,379 @@
+// diff2 -> weight conversion and significance (K3):
+//   convertSquaredDifferencesToWeights  src/map3d_optimizer.cpp:1610-1810 (2D :1077-1224)
+//   findAllSignificantPoints            src/map3d_optimizer.cpp:1812-1931 (2D :1226-1311)
+//
+// One 1024-thread block per image.  With u = d - ln(pdf_direction * pdf_offset) the weight of
+// an entry is w = exp(dmin - u); wherever a weight is used this kernel evaluates the reference's
+// own expression pdf_direction * pdf_offset * exp(-(diff2 - min_diff2)) (0 beyond 700).
+//
+//   scan   : one pass over the image's entries.  Entries within kWin nats of the lower bound
+//            u0 <= min u get their exact weight: sum, max / argmax, a 1/32-nat log histogram and
+//            a per-image candidate list in HBM.  Entries further away weigh < exp(-48) of the
+//            maximum (the window is re-anchored when the priors spread u by more than 16 nats),
+//            below the FP64 resolution of the sum for any grid of up to 1e6 entries.
+//   select : the bucket where the descending cumulative mass crosses adaptive_fraction * sum,
+//            refined while it holds more candidates than the LDS sort; exact weights of that
+//            bucket sorted and accumulated from the top as the reference's sorted loop does
+//            -> threshold and NR_SIGN.
+//   mark   : (coarse) significance bits for candidates with w >= threshold.
+// When an image has more candidates than the list holds, select / mark rescan the entries.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_util.h"
+#include "kernels.h"
+
+namespace rome {
+
+namespace {
+
+constexpr double kWin = 64.0;     // histogram window (2048 bins x 1/32 nat)
+constexpr double kAnchorSlack = 16.0;
+
+struct WeightOut {
+    double* dmin;
+    double* sumw;
+    double* maxw;
+    long long* amax;
+    double* thr;
+    int* nsig;
+    int* status;
+};
+
+struct Cand {
+    double* w;
+    double* du;
+    long long* e;
+    int cap;
+};
+
+struct WShared {
+    double poff[kMaxKT];
+    double lpoff[kMaxKT];
+    double hsum[kBinCount];
+    unsigned int hcnt[kBinCount];
+    double cw[kSortCap];
+    long long ce[kSortCap];
+    double red_d[16];
+    long long red_l[16];
+    int ncand_sort;
+    int ncand;
+    int done;
+    double lo, width, mbefore;
+    long long cbefore;
+    double thr;
+    long long nsig;
+};
+
+template <typename Get>
+__device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, WeightOut wo,
+                            WShared& sh, uint32_t* sigmask, int TW, Cand cand, const int* emin_key) {
+    constexpr int NT = 1024;
+    const int tid = threadIdx.x;
+    const int T = wp.T;
+    double lpo_max = -INFINITY;
+    for (int q = tid; q < wp.K * T; q += NT) {
+        const int k = q / T, t = q % T;
+        const double po = pdf_offset(wp.trans_x[t], wp.trans_y[t], wp.old_offx[img], wp.old_offy[img], wp.prior_x[k],
+                                     wp.prior_y[k], wp.sigma2_offset);
+        sh.poff[q] = po;
+        sh.lpoff[q] = po > 0 ? log(po) : -INFINITY;
+        lpo_max = fmax(lpo_max, sh.lpoff[q]);
+    }
+    lpo_max = block_reduce(lpo_max, [](double a, double b) { return fmax(a, b); }, sh.red_d);
+    auto u_of = [&](double d, int m, int t) { return d - (wp.lpdf_m[m] + sh.lpoff[wp.k_m[m] * T + t]); };
+    auto w_of = [&](double d, int m, int t, double dmin) {
+        const double dd = (d + cimg) - dmin;
+        const double w = wp.pdf_m[m] * sh.poff[wp.k_m[m] * T + t];
+        return dd > 700. ? 0. : w * exp(-dd);
+    };
+    // ---- minimum of d (the coarse pass gets it from the cross-term epilogue)
+    double mn;
+    if (emin_key) {
+        mn = (double)float_from_order_key(emin_key[img]);
+    } else {
+        mn = INFINITY;
+        for (size_t e = tid; e < count; e += NT) {
+            double d;
+            int m, t;
+            if (get(e, d, m, t)) mn = fmin(mn, d);
+        }
+        mn = block_reduce(mn, [](double a, double b) { return fmin(a, b); }, sh.red_d);
+    }
+    const double dmin = mn + cimg;  // (d + c) is monotone in d
+    double u0 = mn - (wp.lpdf_max + lpo_max);
+    // ---- scan
+    double sum = 0., mx = -1.;
+    long long am = -1;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        for (int b = tid; b < kBinCount; b += NT) {
+            sh.hsum[b] = 0.;
+            sh.hcnt[b] = 0;
+        }
+        if (tid == 0) sh.ncand = 0;
+        __syncthreads();
+        sum = 0.;
+        mx = -1.;
+        am = -1;
+        double umin = INFINITY;
+        for (size_t e = tid; e < count; e += NT) {
+            double d;
+            int m, t;
+            if (!get(e, d, m, t)) continue;
+            const double u = u_of(d, m, t);
+            umin = fmin(umin, u);
+            const double du = u - u0;
+            if (!(du < kWin)) continue;
+            const double w = w_of(d, m, t, dmin);
+            sum += w;
+            if (w > mx) {  // strided visit in increasing e: keeps this thread's first maximum
+                mx = w;
+                am = (long long)e;
+            }
+            if (w > 0.) {
+                const int b = du > 0. ? (int)(du * kBinScale) : 0;
+                atomicAdd(&sh.hsum[b], w);
+                atomicAdd(&sh.hcnt[b], 1u);
+                const int slot = atomicAdd(&sh.ncand, 1);
+                if (slot < cand.cap) {
+                    const size_t o = (size_t)img * cand.cap + slot;
+                    cand.w[o] = w;
+                    cand.du[o] = du;
+                    cand.e[o] = (long long)e;
+                }
+            }
+        }
+        umin = block_reduce(umin, [](double a, double b) { return fmin(a, b); }, sh.red_d);
+        if (!(umin - u0 > kAnchorSlack)) break;
+        u0 = umin;  // priors spread u widely: anchor the window at the true minimum and rescan
+    }
+    sum = block_reduce(sum, [](double a, double b) { return a + b; }, sh.red_d);
+    {
+        const double m2 = block_reduce(mx, [](double a, double b) { return fmax(a, b); }, sh.red_d);
+        const long long c = (mx == m2 && am >= 0) ? am : (long long)0x7fffffffffffffffLL;
+        am = block_reduce(c, [](long long a, long long b) { return a < b ? a : b; }, sh.red_l);
+        mx = m2;
+    }
+    const double target = wp.adaptive_fraction * sum;
+    const int ncand = sh.ncand;
+    const bool listed = ncand <= cand.cap;
+    // visit(lo, hi, fn): every entry with weight > 0 and lo <= du < hi, as (w, du, e)
+    auto visit = [&](double lo, double hi, auto&& fn) {
+        if (listed) {
+            for (int q = tid; q < ncand; q += NT) {
+                const size_t o = (size_t)img * cand.cap + q;
+                const double du = cand.du[o];
+                if (du >= lo && du < hi) fn(cand.w[o], du, cand.e[o]);
+            }
+        } else {
+            for (size_t e = tid; e < count; e += NT) {
+                double d;
+                int m, t;
+                if (!get(e, d, m, t)) continue;
+                const double du = u_of(d, m, t) - u0;
+                if (!(du >= lo && du < hi && du < kWin)) continue;
+                const double w = w_of(d, m, t, dmin);
+                if (w > 0.) fn(w, du, (long long)e);
+            }
+        }
+    };
+    // ---- select
+    if (tid == 0) {
+        sh.lo = 0.;
+        sh.width = 1.0 / kBinScale;
+        sh.mbefore = 0.;
+        sh.cbefore = 0;
+        sh.done = 0;
+    }
+    __syncthreads();
+    int status = 0;
+    for (int level = 0;; ++level) {
+        if (tid == 0) {
+            double cum = sh.mbefore;
+            long long cnt = sh.cbefore;
+            int bstar = -1, last = -1;
+            for (int b = 0; b < kBinCount; ++b) {
+                if (sh.hcnt[b] == 0) continue;
+                last = b;
+                if (cum + sh.hsum[b] > target) {
+                    bstar = b;
+                    break;
+                }
+                cum += sh.hsum[b];
+                cnt += sh.hcnt[b];
+            }
+            if (bstar < 0 && last >= 0) {  // rounding left the crossing unreached: last bucket
+                bstar = last;
+                cum -= sh.hsum[last];
+                cnt -= sh.hcnt[last];
+            }
+            if (bstar < 0) {
+                sh.done = 2;
+            } else {
+                sh.mbefore = cum;
+                sh.cbefore = cnt;
+                sh.lo = sh.lo + bstar * sh.width;
+                sh.done = (sh.hcnt[bstar] <= (unsigned)kSortCap || level >= 2) ? 1 : 0;
+                sh.width = sh.width / (double)kBinCount;
+            }
+            sh.ncand_sort = 0;
+        }
+        __syncthreads();
+        if (sh.done == 2) {
+            status = 1;
+            break;
+        }
+        const double lo = sh.lo, hi = lo + sh.width * kBinCount;
+        if (sh.done == 1) {
+            visit(lo, hi, [&](double w, double, long long e) {
+                const int slot = atomicAdd(&sh.ncand_sort, 1);
+                if (slot < kSortCap) {
+                    sh.cw[slot] = w;
+                    sh.ce[slot] = e;
+                }
+            });
+            __syncthreads();
+            const int nc = sh.ncand_sort < kSortCap ? sh.ncand_sort : kSortCap;
+            if (sh.ncand_sort > kSortCap) status = 2;
+            int np2 = 1;
+            while (np2 < nc) np2 <<= 1;
+            for (int q = nc + tid; q < np2; q += NT) {
+                sh.cw[q] = -1.0;
+                sh.ce[q] = -1;
+            }
+            __syncthreads();
+            for (int ksz = 2; ksz <= np2; ksz <<= 1)
+                for (int j = ksz >> 1; j > 0; j >>= 1) {
+                    for (int q = tid; q < np2; q += NT) {
+                        const int ixj = q ^ j;
+                        if (ixj > q) {
+                            const bool desc = ((q & ksz) == 0);
+                            const double a1 = sh.cw[q], a2 = sh.cw[ixj];
+                            if (desc ? (a1 < a2) : (a1 > a2)) {
+                                sh.cw[q] = a2;
+                                sh.cw[ixj] = a1;
+                                const long long t1 = sh.ce[q];
+                                sh.ce[q] = sh.ce[ixj];
+                                sh.ce[ixj] = t1;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            if (tid == 0) {
+                double frac = sh.mbefore;
+                long long cnt = sh.cbefore;
+                double my = 0.;
+                for (int q = 0; q < nc; ++q) {
+                    ++cnt;
+                    my = sh.cw[q];
+                    frac += my;
+                    if (frac > target) break;
+                }
+                sh.thr = my;
+                sh.nsig = cnt;
+            }
+            __syncthreads();
+            break;
+        }
+        // refine the crossing bucket
+        for (int b = tid; b < kBinCount; b += NT) {
+            sh.hsum[b] = 0.;
+            sh.hcnt[b] = 0;
+        }
+        __syncthreads();
+        const double sw = sh.width;
+        visit(lo, hi, [&](double w, double du, long long) {
+            int b = (int)((du - lo) / sw);
+            b = b < 0 ? 0 : (b >= kBinCount ? kBinCount - 1 : b);
+            atomicAdd(&sh.hsum[b], w);
+            atomicAdd(&sh.hcnt[b], 1u);
+        });
+        __syncthreads();
+    }
+    const double thr = status == 1 ? 0. : sh.thr;
+    if (tid == 0) {
+        wo.dmin[img] = dmin;
+        wo.sumw[img] = sum;
+        wo.maxw[img] = mx;
+        wo.amax[img] = am;
+        wo.thr[img] = thr;
+        if (wo.nsig) wo.nsig[img] = status == 1 ? 0 : (int)sh.nsig;
+        wo.status[img] = status | (listed ? 0 : 4);
+    }
+    if (!sigmask) return;
+    // ---- mark: w >= threshold, with the weight the reference compares
+    uint32_t* msk = sigmask + (size_t)img * wp.Mtot * TW;
+    for (size_t q = tid; q < (size_t)wp.Mtot * TW; q += NT) msk[q] = 0u;
+    __syncthreads();
+    if (status == 1) return;
+    visit(0.0, kWin, [&](double w, double, long long e) {
+        if (!(w >= thr)) return;
+        double d;
+        int m, t;
+        get((size_t)e, d, m, t);
+        atomicOr(&msk[(size_t)m * TW + (t >> 5)], 1u << (t & 31));
+    });
+}
+
+__global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const float* __restrict__ E,
+                                                         const double* __restrict__ C2c,
+                                                         const int* __restrict__ emin_key, WeightOut wo,
+                                                         uint32_t* __restrict__ sigmask, int TW, Cand cand) {
+    __shared__ WShared sh;
+    const int img = blockIdx.x;
+    const int T = wp.T;
+    const unsigned long long magic = wp.magicT;
+    const float* Ei = E + (size_t)img * wp.Mtot * T;
+    auto get = [&](size_t e, double& d, int& m, int& t) -> bool {
+        m = (int)(((unsigned long long)e * magic) >> 40);
+        t = (int)e - m * T;
+        if (!(wp.pdf_m[m] > 0.)) return false;
+        d = (double)Ei[e];
+        return true;
+    };
+    weight_pass(img, (size_t)wp.Mtot * T, C2c[img], wp, get, wo, sh, sigmask, TW, cand, emin_key);
+}
+
+__global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
+                                                       const int* __restrict__ blk_task, const int* __restrict__ blk_t,
+                                                       const int* __restrict__ task_m,
+                                                       const long long* __restrict__ img_blk, int log2NE, WeightOut wo,
+                                                       Cand cand) {
+    __shared__ WShared sh;
+    const int img = blockIdx.x;
+    const long long b0 = img_blk[img];
+    const size_t count = (size_t)(img_blk[img + 1] - b0) << log2NE;
+    const double* Di = D2 + ((size_t)b0 << log2NE);
+    auto get = [&](size_t e, double& d, int& m, int& t) -> bool {
+        const long long b = b0 + (long long)(e >> log2NE);
+        m = task_m[blk_task[b]];
+        t = blk_t[b];
+        d = Di[e];
+        return true;
+    };
+    weight_pass(img, count, 0.0, wp, get, wo, sh, nullptr, 1, cand, nullptr);
+}
+
+}  // namespace
+
+hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const double* C2c,
+                                 const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                                 const CandArgs& c) {
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, C2c, emin_key, o, sigmask, TW, Cand{c.w, c.du, c.e, c.cap});
+    return hipGetLastError();
+}
+
+hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
+                               const int* blk_t, const int* task_m, const long long* img_blk, int NE,
+                               const WeightOutArgs& wo, const CandArgs& c) {
+    int log2NE = 0;
+    while ((1 << log2NE) < NE) ++log2NE;
+    if ((1 << log2NE) != NE) return hipErrorInvalidValue;
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    k_weights_fine<<<n, 1024, 0, s>>>(wp, D2, blk_task, blk_t, task_m, img_blk, log2NE, o, Cand{c.w, c.du, c.e, c.cap});
+    return hipGetLastError();
+}
+
+}  // namespace rome
