@@ -832,8 +832,11 @@ int or_estep(const OrConfig* cfg, const OrImages* im, OrOutputs* out, char* errb
                 }
             }
             min_diff2[i] = mymin;
-            double* dump = coarse ? out->coarse_diff2 : out->fine_diff2;
-            if (dump) {
+            // single pass: the one pass is both the coarse and the last ("fine") pass
+            double* dumps[2] = {coarse ? out->coarse_diff2 : out->fine_diff2,
+                                (coarse && npasses == 1) ? out->fine_diff2 : nullptr};
+            for (double* dump : dumps) {
+                if (!dump) continue;
                 const size_t per = (size_t)K * D * PSI * T * NOR * NOT;
                 double* di = dump + (size_t)i * per;
                 for (size_t q = 0; q < per; ++q) di[q] = std::numeric_limits<double>::quiet_NaN();

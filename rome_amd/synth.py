@@ -210,7 +210,10 @@ def make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, psi_ste
     rng = np.random.default_rng(seed)
     ndim = 3 if is3d else 2
     radius = 0.35 * ori
-    vols = [blob_volume(ori, nblobs, (1.0, max(1.5, ori / 16)), radius, rng, ndim) for _ in range(K)]
+    if is3d and ori >= 64:
+        vols = [blob_volume_fast(ori, nblobs, (2.0, 6.0), radius, rng) for _ in range(K)]
+    else:
+        vols = [blob_volume(ori, nblobs, (1.0, max(1.5, ori / 16)), radius, rng, ndim) for _ in range(K)]
     grids = []
     pad = None
     for v in vols:
