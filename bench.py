@@ -11,7 +11,11 @@ current_size 128, coarse_size 24, backprojection into 4 grids of 259x259x130.
 A step = one E-step (coarse pass, fine pass, weights, significance, sigma2/norm sums,
 backprojection, per-image sums) over one batch of particles already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--batch B] [--n-images N]
+Two arms run on the same particles: FP32 (coarse cross term on FP32 MFMA) and mixed (coarse cross term
+on BF16 or FP16 MFMA; fine pass, weights and sums FP32/FP64).  The headline value is the mixed arm's
+when its max-posterior poses agree with the FP32 arm's on >= 99.5 % of the particles, else the FP32 arm's.
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--n-images N] [--precision bf16|fp16|fp32]
 """
 import argparse
 import json
@@ -24,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # MI355X dense matrix peaks
 HBM_PEAK_GBS = 8000.0
 
 
@@ -74,6 +78,28 @@ def cpu_baseline(prob, F, ctf, idx, threads):
     return n / dt, dt
 
 
+def pose_differences(meta_a, meta_b, amax_a, amax_b):
+    """Where two arms pick different max-posterior samples: class changes, the angle between the two
+    orientations, the distance between the two offsets and both arms' PMAX."""
+    from rome_amd import synth
+    rows = []
+    for b in meta_a:
+        if b not in meta_b:
+            continue
+        for i in np.nonzero(~np.all(amax_a[b] == amax_b[b], axis=1))[0]:
+            ma, mb = meta_a[b][i], meta_b[b][i]
+            Ra, Rb = synth.euler_matrix(*ma[2:5]), synth.euler_matrix(*mb[2:5])
+            c = np.clip((np.trace(Ra.T @ Rb) - 1) / 2, -1, 1)
+            rows.append((ma[1] != mb[1], np.degrees(np.arccos(c)), np.hypot(*(ma[5:7] - mb[5:7])), ma[0], mb[0]))
+    if not rows:
+        return {"count": 0}
+    r = np.array(rows, dtype=float)
+    return {"count": len(rows), "class_changed": int(r[:, 0].sum()), "angle_deg_max": round(float(r[:, 1].max()), 3),
+            "angle_deg_median": round(float(np.median(r[:, 1])), 3), "shift_px_max": round(float(r[:, 2].max()), 3),
+            "pmax_fp32_median": round(float(np.median(r[:, 3])), 4),
+            "pmax_mixed_median": round(float(np.median(r[:, 4])), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +111,8 @@ def main():
     ap.add_argument("--perturbation", type=float, default=0.25)
     ap.add_argument("--cpu-sample", type=int, default=0, help="images timed on the CPU oracle (0: 2 per thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "bf16", "fp16"], default="bf16",
+                    help="mixed arm timed beside the FP32 arm (fp32: FP32 arm only)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -108,9 +136,6 @@ def main():
     F, ctf, truth, sigma2 = synth.make_particles_device(eng, prob["nr_classes"], n_img, 128, 128, 1.5, 5.0, 0.05,
                                                         args.seed + 1)
     prob["sigma2_noise"] = np.full((1, prob["ori_size"] // 2 + 1), sigma2)
-    eng.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
-                  prob["prior_offy"])
-    eng.reset_sums()
     dev = F.device
     oriF = prob["ori_size"] // 2 + 1
     hxi2 = torch.zeros(n_img, dtype=torch.float64, device=dev)
@@ -132,33 +157,60 @@ def main():
             hxi2.data_ptr() + o * 8, power.data_ptr() + o * oriF * 8, offx.data_ptr() + o * 8,
             offy.data_ptr() + o * 8, norm.data_ptr() + o * 8, group.data_ptr() + o * 4, results=True)
 
-    for i in range(args.warmup):
-        step(i)
-        log(f"warmup {i} done", eng.batch_stats())
-    eng.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    agree = []
-    for i in range(args.steps):
-        r = step(args.warmup + i)
-        b = (args.warmup + i) % nb
-        tr = truth[b * B:(b + 1) * B]
-        agree.append(float(np.mean(r["meta"][:, 1] - 1 == tr[:, 0])))
-        log(f"step {i} done", eng.batch_stats())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    prof = eng.profile()
-    stats = eng.batch_stats()
-    value = args.steps * B / dt
-    # roofline of the dominant kernel
+    def run_arm(prec):
+        synth.load_engine(eng, prob, precision=prec)
+        for i in range(args.warmup):
+            step(i)
+            log(f"[{prec}] warmup {i} done", eng.batch_stats())
+        eng.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        agree, amax, meta = [], {}, {}
+        for i in range(args.steps):
+            r = step(args.warmup + i)
+            b = (args.warmup + i) % nb
+            tr = truth[b * B:(b + 1) * B]
+            agree.append(float(np.mean(r["meta"][:, 1] - 1 == tr[:, 0])))
+            amax[b] = r["argmax"].copy()
+            meta[b] = r["meta"].copy()
+            log(f"[{prec}] step {i} done", eng.batch_stats())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        prof = eng.profile()
+        eng.set_profiling(False)
+        return dict(value=args.steps * B / dt, dt=dt, prof=prof, stats=eng.batch_stats(), amax=amax, meta=meta,
+                    truth_agree=float(np.mean(agree)))
+
+    arms = {"fp32": run_arm("fp32")}
+    if args.precision != "fp32":
+        arms[args.precision] = run_arm(args.precision)
+    # pose agreement of the mixed arm with the FP32 arm: same max-posterior (class, direction, psi,
+    # fine rotation, translation, fine translation) on the same particles
+    pose_agree = None
+    head = "fp32"
+    if args.precision != "fp32":
+        a32, alo = arms["fp32"]["amax"], arms[args.precision]["amax"]
+        same = [np.all(a32[b] == alo[b], axis=1) for b in a32 if b in alo]
+        pose_agree = float(np.mean(np.concatenate(same)))
+        diffs = pose_differences(arms["fp32"]["meta"], arms[args.precision]["meta"], a32, alo)
+        if pose_agree >= 0.995:
+            head = args.precision
+    H = arms[head]
+    prof, stats, dt = H["prof"], H["stats"], H["dt"]
+    value = H["value"]
+    # roofline of the coarse cross-term kernel of the headline arm
     dom = max(prof, key=lambda k: prof[k][0])
     M, T, Pc = stats["M"], stats["T"], stats["Pc"]
     gemm_ms, gemm_calls = prof["coarse_cross"]
     gemm_flops = 2.0 * M * (B * T) * (2 * Pc)  # algorithmic: Re<CF, XS> = 2 real MACs per complex pixel
     gemm_tflops = gemm_flops * gemm_calls / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
-    roofline = {"kernel": "k_coarse_gemm (coarse cross term, FP32 MFMA)", "bound": "mfma",
-                "achieved": round(gemm_tflops, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(gemm_tflops / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+    peak = MFMA_PEAK_TFLOPS[head]
+    kname = {"fp32": "k_coarse_gemm (coarse cross term, FP32 MFMA)",
+             "bf16": "k_gemm_lowp<bf16> (coarse cross term, BF16 MFMA)",
+             "fp16": "k_gemm_lowp<fp16> (coarse cross term, FP16 MFMA)"}[head]
+    roofline = {"kernel": kname, "bound": "mfma",
+                "achieved": round(gemm_tflops, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(gemm_tflops / peak, 4), "traffic": None,
                 "avg_launch_ms": round(gemm_ms / max(gemm_calls, 1), 3), "dominant_stage": dom}
     cpu = None
     if not args.no_cpu_baseline:
@@ -167,17 +219,23 @@ def main():
         ips, cdt = cpu_baseline(prob, F, ctf, np.arange(ns), threads)
         cpu = {"value": round(ips, 3), "unit": "images/s", "cores": threads, "kind": "port",
                "sample": f"{ns} particles of the same workload through the FP64 oracle E-step ({cdt:.1f}s)"}
+    dtype = {"fp32": "fp32", "bf16": "bf16+fp32", "fp16": "fp16+fp32"}[head]
     out = {
         "metric": "E-step images/sec (3D 128^2, K=4) at >=99.5% pose agreement; mixed vs FP32",
         "value": round(value, 2), "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 2), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic",
         "config": {"workload": "plain MAP3D E-step: 128^2, K=4, hp2 x 24 psi (4608) x 81 shifts, oversampling 1",
                    "batch_images": B, "n_images": n_img, "current_size": 128, "coarse_size": 24,
-                   "perturbation": args.perturbation, "precision_arm": "plain-fp32"},
+                   "perturbation": args.perturbation, "precision_arm": head},
         "roofline": roofline, "cpu_baseline": cpu,
-        "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 3) for k, v in prof.items()},
-        "batch_stats": stats, "class_agreement_vs_truth": round(float(np.mean(agree)), 4),
+        "arms": {k: {"value": round(v["value"], 2), "ms_per_step": round(v["dt"] / args.steps * 1e3, 2),
+                     "class_agreement_vs_truth": round(v["truth_agree"], 4),
+                     "stages_ms_per_step": {s_: round(x[0] / max(x[1], 1), 3) for s_, x in v["prof"].items()}}
+                 for k, v in arms.items()},
+        "pose_agreement_mixed_vs_fp32": None if pose_agree is None else round(pose_agree, 5),
+        "pose_differences": None if pose_agree is None else diffs,
+        "batch_stats": stats,
     }
     print(json.dumps(out), flush=True)
 

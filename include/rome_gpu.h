@@ -40,7 +40,15 @@ typedef struct {
     int nr_dir, nr_psi, nr_trans;  /* coarse grid; nr_dir = 1 in 2D */
     double adaptive_fraction, sigma2_fudge, sigma2_offset, avg_norm_correction;
     int do_ctf_correction, refs_are_ctf_corrected, do_norm_correction, do_map;
+    /* Arithmetic of the coarse cross term Re(CF * conj(Minv CTF X S)) (no reference counterpart:
+     * ROME computes it in FP64 on the CPU): 0 FP32 MFMA (default), 1 BF16 MFMA, 2 FP16 MFMA, both
+     * with FP32 accumulation and power-of-two operand scaling.  Only the coarse pass (which picks the
+     * significant samples) changes; the fine pass, weights, sums and maps keep FP32/FP64.  BF16/FP16
+     * need adaptive_oversampling = 1 (ROME_ERR_UNSUPPORTED otherwise). */
+    int precision;
 } rome_config;
+
+enum { ROME_PREC_FP32 = 0, ROME_PREC_BF16 = 1, ROME_PREC_FP16 = 2 };
 
 /* One pool of prepared images, the state ParticleModel::setFourierTransforms leaves
  * (src/exp_model.h:397-639): masked / unmasked Fourier transforms windowed to

@@ -42,7 +42,9 @@ class rome_config(C.Structure):
                [(n, C.c_double) for n in ("adaptive_fraction", "sigma2_fudge", "sigma2_offset",
                                            "avg_norm_correction")] + \
                [(n, C.c_int) for n in ("do_ctf_correction", "refs_are_ctf_corrected", "do_norm_correction",
-                                        "do_map")]
+                                        "do_map", "precision")]
+
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 
 
 class rome_images(C.Structure):

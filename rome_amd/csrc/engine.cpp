@@ -86,6 +86,7 @@ struct rome_ctx {
     DevBuf<float2> Sc, Sf;
     DevBuf<double> sigma2, pdf_dir, lpdf_dir, prior_x, prior_y, logsigma2, pdf_m, lpdf_m;
     DevBuf<int> k_m;
+    DevBuf<float2> mrow;
     double lpdf_max = 0.;
     DevBuf<int> emin_key;
     DevBuf<double> cand_w, cand_du;
@@ -93,6 +94,11 @@ struct rome_ctx {
     static constexpr int kCandCap = 8192;
     DevBuf<unsigned char> valid;
     DevBuf<float> FcT;
+    // mixed-precision arm: scaled BF16/FP16 copies of the coarse table and of the image operand
+    DevBuf<int> fct_absmax;
+    DevBuf<uint16_t> A16, W16;
+    DevBuf<float> inv_scale;
+    int K16() const { return (2 * Pc_pad + 63) / 64 * 64; }
     // sums
     DevBuf<float> bp_data, bp_weight;
     DevBuf<double> wsum_sigma2, wsum_pdf_class, wsum_pdf_dir, wsum_sumw_group, wsum_prior_x, wsum_prior_y, wsum_scalars;
@@ -240,6 +246,16 @@ int ensure_ready(rome_ctx* ctx) {
         HIPCHK(launch_project_coarse(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d,
                                      ctx->Ai_c.p, ctx->R, ctx->Mtot, ctx->c_xy.p, ctx->Pc, ctx->Pc_pad, my_r_max,
                                      ctx->Mpad, ctx->FcT.p));
+        if (ctx->cfg.precision != ROME_PREC_FP32) {
+            const int K16 = ctx->K16();
+            const size_t nf = (size_t)2 * ctx->Pc_pad * ctx->Mpad;
+            HIPCHK(ctx->fct_absmax.reserve(1));
+            HIPCHK(hipMemsetAsync(ctx->fct_absmax.p, 0, sizeof(int), ctx->stream));
+            HIPCHK(launch_absmax(ctx->stream, ctx->FcT.p, nf, ctx->fct_absmax.p));
+            HIPCHK(ctx->A16.reserve((size_t)ctx->Mpad * K16));
+            HIPCHK(launch_fct_to_lowp(ctx->stream, ctx->cfg.precision, ctx->FcT.p, 2 * ctx->Pc_pad, ctx->Mpad,
+                                      ctx->Mtot, ctx->fct_absmax.p, ctx->A16.p, K16));
+        }
         ctx->fct_dirty = false;
     }
     return ROME_OK;
@@ -257,6 +273,7 @@ WeightParams weight_params(rome_ctx* ctx) {
     wp.pdf_m = ctx->pdf_m.p;
     wp.lpdf_m = ctx->lpdf_m.p;
     wp.k_m = ctx->k_m.p;
+    wp.mrow = ctx->mrow.p;
     wp.lpdf_max = ctx->lpdf_max;
     wp.trans_x = ctx->d_tx.p;
     wp.trans_y = ctx->d_ty.p;
@@ -279,9 +296,18 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     const int Ncols = n * T;
     const int Npad = (Ncols + 127) / 128 * 128;
     const int Kdim = 2 * ctx->Pc_pad;
+    const int prec = c.precision;
+    const int K16 = ctx->K16();
     // ---- image preparation
-    HIPCHK(ctx->WcT.reserve((size_t)Kdim * Npad));
-    HIPCHK(hipMemsetAsync(ctx->WcT.p, 0, (size_t)Kdim * Npad * sizeof(float), s));
+    if (prec == ROME_PREC_FP32) {
+        HIPCHK(ctx->WcT.reserve((size_t)Kdim * Npad));
+        HIPCHK(hipMemsetAsync(ctx->WcT.p, 0, (size_t)Kdim * Npad * sizeof(float), s));
+    } else {
+        HIPCHK(ctx->W16.reserve((size_t)Npad * K16));
+        HIPCHK(ctx->inv_scale.reserve(n));
+        if (Npad > Ncols)  // rows past the last column only feed discarded outputs; keep them finite
+            HIPCHK(hipMemsetAsync(ctx->W16.p + (size_t)Ncols * K16, 0, (size_t)(Npad - Ncols) * K16 * 2, s));
+    }
     HIPCHK(ctx->MC2c.reserve((size_t)n * ctx->Pc));
     HIPCHK(ctx->C2c.reserve(n));
     HIPCHK(ctx->C2f.reserve(n));
@@ -315,7 +341,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     pa.Pf = ctx->Pf;
     pa.f_nfine = ctx->f_nfine.p;
     pa.f_ires = ctx->f_ires.p;
-    pa.WcT = ctx->WcT.p;
+    pa.WcT = prec == ROME_PREC_FP32 ? ctx->WcT.p : nullptr;
     pa.MC2c = ctx->MC2c.p;
     pa.C2c = ctx->C2c.p;
     pa.Xf = ctx->Xf.p;
@@ -326,19 +352,50 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     pa.MC2bp = ctx->MC2bp.p;
     stage_begin(ctx, 0);
     HIPCHK(launch_prep_images(s, n, pa));
+    if (prec != ROME_PREC_FP32) {
+        PrepLowpArgs pl;
+        pl.fmask = ctx->in_fmask.p;
+        pl.fctf = ctx->in_fctf.p;
+        pl.group = ctx->in_group.p;
+        pl.sigma2 = ctx->sigma2.p;
+        pl.ori_Fsize = ctx->oriF;
+        pl.F2f = ctx->F2f;
+        pl.sigma2_fudge = c.sigma2_fudge;
+        pl.ctf_in_diff = pa.ctf_in_diff;
+        pl.Pc = ctx->Pc;
+        pl.Pc_pad = ctx->Pc_pad;
+        pl.T = T;
+        pl.K16 = K16;
+        pl.c_nfine = ctx->c_nfine.p;
+        pl.c_ires = ctx->c_ires.p;
+        pl.Sc = ctx->Sc.p;
+        pl.fct_absmax = ctx->fct_absmax.p;
+        pl.W16 = ctx->W16.p;
+        pl.inv_scale = ctx->inv_scale.p;
+        HIPCHK(launch_prep_lowp(s, prec, n, pl));
+    }
     stage_end(ctx, 0);
     // ---- coarse cross term + norms
-    HIPCHK(ctx->At.reserve((size_t)n * M));
+    HIPCHK(ctx->At.reserve((size_t)n * ctx->Mpad));
     stage_begin(ctx, 1);
     HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
     stage_end(ctx, 1);
     HIPCHK(ctx->E.reserve((size_t)n * M * T));
     HIPCHK(ctx->emin_key.reserve(n));
     HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
+    // an L2-resident slice of the rotation operand per XCD: about 2.5 MB of its 4 MB L2
+    const int nbm = ctx->Mpad / 128;
     GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, ctx->E.p, ctx->valid.p, ctx->emin_key.p, Kdim, ctx->Mpad, Npad, M,
-                Ncols, T};
+                Ncols, T, gemm_sub_band(nbm, (size_t)128 * Kdim * 4, (size_t)5 << 19)};
     stage_begin(ctx, 2);
-    HIPCHK(launch_coarse_gemm(s, ga));
+    if (prec == ROME_PREC_FP32) {
+        HIPCHK(launch_coarse_gemm(s, ga));
+    } else {
+        LowpGemmArgs la{ctx->A16.p, ctx->W16.p, ctx->At.p, ctx->inv_scale.p, ctx->E.p, ctx->valid.p,
+                        ctx->emin_key.p, K16, ctx->Mpad, Npad, M, Ncols, T,
+                        gemm_sub_band(nbm, (size_t)128 * K16 * 2, (size_t)5 << 19)};
+        HIPCHK(launch_gemm_lowp(s, prec, la));
+    }
     stage_end(ctx, 2);
     // ---- coarse weights and significance
     for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_maxw, &ctx->c_thr, &ctx->f_dmin, &ctx->f_sumw, &ctx->f_maxw,
@@ -667,6 +724,10 @@ int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
         return fail(ctx, ROME_ERR_UNSUPPORTED, "nr_trans %d outside the supported range", c.nr_trans);
     if (c.ori_size / 2 + 1 > kMaxShell) return fail(ctx, ROME_ERR_UNSUPPORTED, "ori_size too large");
     if (c.ref_dim == 2 && c.nr_dir != 1) return fail(ctx, ROME_ERR_ARG, "2D requires nr_dir == 1");
+    if (c.precision < ROME_PREC_FP32 || c.precision > ROME_PREC_FP16)
+        return fail(ctx, ROME_ERR_ARG, "precision must be 0 (FP32), 1 (BF16) or 2 (FP16)");
+    if (c.precision != ROME_PREC_FP32 && c.adaptive_oversampling == 0)  // the coarse diff2 would be final
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "BF16/FP16 coarse pass needs adaptive_oversampling = 1");
     ctx->cfg = c;
     ctx->is3d = c.ref_dim == 3;
     ctx->K = c.nr_classes;
@@ -805,9 +866,10 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
     ctx->prior_x_h.assign(prior_offx, prior_offx + K);
     ctx->prior_y_h.assign(prior_offy, prior_offy + K);
     std::vector<double> lp((size_t)K * D);
-    std::vector<unsigned char> valid((size_t)ctx->Mtot);
+    std::vector<unsigned char> valid((size_t)ctx->Mpad, 0);
     std::vector<double> pm((size_t)ctx->Mtot), lpm((size_t)ctx->Mtot);
     std::vector<int> km((size_t)ctx->Mtot);
+    std::vector<float2> mrow((size_t)ctx->Mtot);
     for (int k = 0; k < K; ++k)
         for (int d = 0; d < D; ++d) {
             const double p = pdf_direction[(size_t)k * D + d];
@@ -821,11 +883,16 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
                 pm[m] = ok ? p : 0.;
                 lpm[m] = ok ? log(p) : -INFINITY;
                 km[m] = k;
+                const int kT = k * ctx->T;
+                float kTf;
+                std::memcpy(&kTf, &kT, sizeof kTf);
+                mrow[m] = make_float2(ok ? (float)lpm[m] : -INFINITY, kTf);
             }
         }
     HIPCHK(ctx->pdf_m.upload(pm.data(), pm.size(), ctx->stream));
     HIPCHK(ctx->lpdf_m.upload(lpm.data(), lpm.size(), ctx->stream));
     HIPCHK(ctx->k_m.upload(km.data(), km.size(), ctx->stream));
+    HIPCHK(ctx->mrow.upload(mrow.data(), mrow.size(), ctx->stream));
     ctx->lpdf_max = -INFINITY;
     for (double v : lpm) ctx->lpdf_max = std::max(ctx->lpdf_max, v);
     // logsigma2 per group (storeWeightedSums, src/map3d_optimizer.cpp:1998-2008): same loop order as the reference

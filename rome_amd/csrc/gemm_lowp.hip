@@ -1,0 +1,211 @@
+// Mixed-precision arm of the coarse cross term (K2): BF16 or FP16 MFMA with FP32 accumulation.
+//
+//   C[m][j] = sum_k A16[m][k] * B16[j][k],   E[img][m][t] = At[img][m] - C[m][img*T+t] / (sF * s_img)
+//
+// A16 = sF * coarse projections (power-of-two scale over the whole table), B16 = s_img * Minv*CTF*X*S
+// (power-of-two scale per image so its largest value is in [0.5, 1)): FP16 keeps 11 significant bits
+// over the image's dynamic range, BF16 8 bits over FP32's range.  The norm term At stays FP32 and the
+// per-image constant FP64, so only the cross term carries the reduced precision; the fine pass that
+// decides poses, weights and the maps is unchanged (FP32 products, FP64 sums).
+//
+// v_mfma_f32_32x32x16_{bf16,f16}: lane l supplies A[l%32][8*(l/32) .. +7] and B[8*(l/32) .. +7][l%32];
+// D as for the FP32 instruction (row 8*(r/4) + 4*(l/32) + r%4, col l%32).  K-contiguous operands so
+// each lane loads its 8 values with one 16-byte LDS read.  128x128x64 tiles, 4 waves, LDS double buffer.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_util.h"
+#include "gemm_common.h"
+#include "kernels.h"
+
+namespace rome {
+
+namespace {
+
+template <typename T>
+struct Lowp;
+template <>
+struct Lowp<__bf16> {
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    static DEV f16v mfma(v8 a, v8 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Lowp<_Float16> {
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    static DEV f16v mfma(v8 a, v8 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_lowp(LowpGemmArgs a) {
+    constexpr int BM = 128, BN = 128, BK = 64, LDW = BK + 8;
+    typedef typename Lowp<T>::v8 v8;
+    __shared__ __attribute__((aligned(16))) T As[2][BM][LDW];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BN][LDW];
+    __shared__ int slot_min[kEpiSlots];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    int bm, bn;
+    gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
+    const int m0 = bm * BM, n0 = bn * BN;
+    const T* A = reinterpret_cast<const T*>(a.A);
+    const T* B = reinterpret_cast<const T*>(a.B);
+    f16v acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    const int nk = a.K16 / BK;
+    uint4 ra[4], rb[4];
+    auto load = [&](int kc) {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int idx = tid + 256 * h;
+            const int row = idx >> 3, c8 = idx & 7;
+            ra[h] = *reinterpret_cast<const uint4*>(A + (size_t)(m0 + row) * a.K16 + kc * BK + c8 * 8);
+            rb[h] = *reinterpret_cast<const uint4*>(B + (size_t)(n0 + row) * a.K16 + kc * BK + c8 * 8);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int idx = tid + 256 * h;
+            const int row = idx >> 3, c8 = idx & 7;
+            *reinterpret_cast<uint4*>(&As[buf][row][c8 * 8]) = ra[h];
+            *reinterpret_cast<uint4*>(&Bs[buf][row][c8 * 8]) = rb[h];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kc = 0; kc < nk; ++kc) {
+        const int cur = kc & 1;
+        if (kc + 1 < nk) load(kc + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            v8 av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                av[i] = *reinterpret_cast<const v8*>(&As[cur][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                bv[j] = *reinterpret_cast<const v8*>(&Bs[cur][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = Lowp<T>::mfma(av[i], bv[j], acc[i][j]);
+        }
+        if (kc + 1 < nk) store(cur ^ 1);
+        __syncthreads();
+    }
+    gemm_epilogue<true>(acc, m0, n0, wm, wn, lane, tid, a.At, a.inv_scale, a.E, a.valid, a.emin_key, a.Mtot,
+                        a.Mpad, a.Ncols, a.T, slot_min);
+}
+
+// sF: power of two that brings max |FcT| into [0.5, 1) (exact rescaling)
+DEV float table_scale(const int* absmax_bits) {
+    const float mx = __int_as_float(*absmax_bits);
+    return mx > 0.f ? ldexpf(1.f, -ilogbf(mx) - 1) : 1.f;
+}
+
+// A16[m][k] = sF * FcT[k][m] for k < Kfp = 2*Pc_pad, zero beyond and for rows m >= Mtot.
+template <typename T>
+__global__ void k_fct_to_lowp(const float* __restrict__ FcT, int Kfp, int Mpad, int Mtot,
+                              const int* __restrict__ absmax_bits, T* __restrict__ A16, int K16) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)Mpad * K16) return;
+    const float sF = table_scale(absmax_bits);
+    const int m = (int)(idx / K16), k = (int)(idx % K16);
+    const float v = (m < Mtot && k < Kfp) ? FcT[(size_t)k * Mpad + m] * sF : 0.f;
+    A16[idx] = static_cast<T>(v);
+}
+
+// max |x| over an array as float bits (non-negative floats order like their bit patterns); *out starts at 0
+__global__ void k_absmax(const float* __restrict__ x, size_t n, int* __restrict__ out) {
+    __shared__ float red[16];
+    float m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(x[i]));
+    m = block_reduce(m, [](float u, float v) { return fmaxf(u, v); }, red);
+    if (threadIdx.x == 0) atomicMax(out, __float_as_int(m));
+}
+
+// W16[img*T + t][k]: s_img * Minv*CTF*X*S (Re at k = p, Im at k = Pc_pad + p, zero in the padding);
+// one block per image.  s_img is the power of two that brings max |Minv*CTF*X| into [0.5, 1).
+template <typename T>
+__global__ void k_prep_lowp(PrepLowpArgs a) {
+    __shared__ float red[16];
+    const int img = blockIdx.x;
+    const int g = a.group[img];
+    const double* s2 = a.sigma2 + (size_t)g * a.ori_Fsize;
+    const float2* fm = a.fmask + (size_t)img * a.F2f;
+    const float* fc = a.fctf + (size_t)img * a.F2f;
+    float mx = 0.f;
+    for (int p = threadIdx.x; p < a.Pc; p += blockDim.x) {
+        const int nf = a.c_nfine[p];
+        const float2 X = fm[nf];
+        const float mc = (float)(1.0 / (a.sigma2_fudge * s2[a.c_ires[p]])) * (a.ctf_in_diff ? fc[nf] : 1.f);
+        mx = fmaxf(mx, fabsf(mc) * sqrtf(X.x * X.x + X.y * X.y));
+    }
+    mx = block_reduce(mx, [](float u, float v) { return fmaxf(u, v); }, red);
+    const float s = mx > 0.f ? ldexpf(1.f, -ilogbf(mx) - 1) : 1.f;
+    if (threadIdx.x == 0) a.inv_scale[img] = 1.f / (s * table_scale(a.fct_absmax));
+    T* W = reinterpret_cast<T*>(a.W16) + (size_t)img * a.T * a.K16;
+    for (int p = threadIdx.x; p < a.Pc; p += blockDim.x) {
+        const int nf = a.c_nfine[p];
+        const float2 X = fm[nf];
+        const float mc = (float)(1.0 / (a.sigma2_fudge * s2[a.c_ires[p]])) * (a.ctf_in_diff ? fc[nf] : 1.f) * s;
+        for (int t = 0; t < a.T; ++t) {
+            const float2 XS = cmul(X, a.Sc[(size_t)t * a.Pc + p]);
+            W[(size_t)t * a.K16 + p] = static_cast<T>(mc * XS.x);
+            W[(size_t)t * a.K16 + a.Pc_pad + p] = static_cast<T>(mc * XS.y);
+        }
+    }
+    // padding columns: [Pc, Pc_pad), [Pc_pad + Pc, 2 Pc_pad), [2 Pc_pad, K16)
+    const int npad = a.K16 - 2 * a.Pc;
+    for (int q = threadIdx.x; q < a.T * npad; q += blockDim.x) {
+        const int t = q / npad, j = q - t * npad;
+        const int d = a.Pc_pad - a.Pc;
+        const int k = j < d ? a.Pc + j : (j < 2 * d ? a.Pc_pad + a.Pc + (j - d) : 2 * a.Pc_pad + (j - 2 * d));
+        W[(size_t)t * a.K16 + k] = static_cast<T>(0.f);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a) {
+    if (a.Mpad % 128 || a.Npad % 128 || a.K16 % 64 || a.K16 <= 0) return hipErrorInvalidValue;
+    const int tiles = (a.Mpad / 128) * (a.Npad / 128);
+    if (precision == 1) k_gemm_lowp<__bf16><<<tiles, 256, 0, s>>>(a);
+    else if (precision == 2) k_gemm_lowp<_Float16><<<tiles, 256, 0, s>>>(a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Kfp, int Mpad, int Mtot,
+                              const int* fct_absmax, void* A16, int K16) {
+    const size_t tot = (size_t)Mpad * K16;
+    const unsigned blocks = (unsigned)((tot + 255) / 256);
+    if (precision == 1)
+        k_fct_to_lowp<__bf16><<<blocks, 256, 0, s>>>(FcT, Kfp, Mpad, Mtot, fct_absmax, (__bf16*)A16, K16);
+    else if (precision == 2)
+        k_fct_to_lowp<_Float16><<<blocks, 256, 0, s>>>(FcT, Kfp, Mpad, Mtot, fct_absmax, (_Float16*)A16, K16);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(hipStream_t s, const float* x, size_t n, int* out) {
+    k_absmax<<<512, 256, 0, s>>>(x, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_lowp(hipStream_t s, int precision, int n, const PrepLowpArgs& a) {
+    if (2 * a.Pc_pad > a.K16 || a.Pc > a.Pc_pad) return hipErrorInvalidValue;
+    if (precision == 1) k_prep_lowp<__bf16><<<n, 256, 0, s>>>(a);
+    else if (precision == 2) k_prep_lowp<_Float16><<<n, 256, 0, s>>>(a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace rome

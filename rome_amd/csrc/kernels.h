@@ -47,11 +47,52 @@ struct PrepArgs {
 struct GemmArgs {
     const float* A;   // [K][Mpad]
     const float* B;   // [K][Npad]
-    const float* At;  // [n][Mtot]
+    const float* At;  // [n][Mpad]
     float* E;         // [n][Mtot][T]
-    const unsigned char* valid;  // [Mtot] rotation has a positive prior
+    const unsigned char* valid;  // [Mpad] rotation has a positive prior (0 in the padding)
     int* emin_key;    // [n] ordered-int key of the per-image minimum of E (init INT_MAX)
     int K, Mpad, Npad, Mtot, Ncols, T;
+    int sband;        // m-tiles per L2-resident sub-band of the XCD-banded tile order (0: plain order)
+};
+
+// Host side: the largest divisor of the per-XCD band of m-tiles whose slice of the rotation operand
+// (bytes_per_mtile each) fits in `budget` bytes of L2; 0 disables the banded order.
+inline int gemm_sub_band(int nbm, size_t bytes_per_mtile, size_t budget) {
+    if (nbm % 8) return 0;
+    const int band = nbm / 8;
+    for (int s = band; s >= 1; --s)
+        if (band % s == 0 && (size_t)s * bytes_per_mtile <= budget) return s;
+    return 1;
+}
+
+// Mixed-precision coarse cross term (gemm_lowp.hip).  Operands are BF16 or FP16, K-contiguous rows.
+struct LowpGemmArgs {
+    const void* A;            // [Mpad][K16] scaled coarse projections
+    const void* B;            // [Npad][K16] scaled Minv*CTF*X*S, row = img*T + t
+    const float* At;          // [n][Mpad]
+    const float* inv_scale;   // [n] 1 / (sF * s_img)
+    float* E;                 // [n][Mtot][T]
+    const unsigned char* valid;
+    int* emin_key;
+    int K16, Mpad, Npad, Mtot, Ncols, T;
+    int sband;
+};
+
+struct PrepLowpArgs {
+    const float2* fmask;
+    const float* fctf;
+    const int* group;
+    const double* sigma2;
+    int ori_Fsize, F2f;
+    double sigma2_fudge;
+    int ctf_in_diff;
+    int Pc, Pc_pad, T, K16;
+    const int* c_nfine;
+    const int* c_ires;
+    const float2* Sc;
+    const int* fct_absmax;    // float bits of max |FcT| (sets the table's scale sF)
+    void* W16;                // [Npad][K16]
+    float* inv_scale;         // [n]
 };
 
 struct WeightParams {
@@ -61,6 +102,7 @@ struct WeightParams {
     const double* lpdf_m;    // [Mtot] log(pdf_m) or -inf
     double lpdf_max;         // max over rotations of lpdf_m
     const int* k_m;          // [Mtot] class of the rotation
+    const float2* mrow;      // [Mtot] {(float) lpdf_m or -inf, k*T as int bits}: FP32 pre-test of the scan
     const double* trans_x;   // [T] unperturbed translations
     const double* trans_y;
     const double* old_offx;  // [n]
@@ -197,5 +239,12 @@ hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, 
 hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
                                 const int* blk_t, const int* task_m, int n, int NE, int2* out);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
+
+// precision: 1 BF16, 2 FP16
+hipError_t launch_absmax(hipStream_t s, const float* x, size_t n, int* out);
+hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Kfp, int Mpad, int Mtot,
+                              const int* fct_absmax, void* A16, int K16);
+hipError_t launch_prep_lowp(hipStream_t s, int precision, int n, const PrepLowpArgs& a);
+hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a);
 
 }  // namespace rome

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "device_util.h"
+#include "gemm_common.h"
 #include "kernels.h"
 
 namespace rome {
@@ -130,6 +131,7 @@ __global__ void k_prep_images(PrepArgs a) {
         const float mc = (float)minv * ctf;
         a.MC2c[(size_t)img * a.Pc + p] = mc * ctf;
         x2 += minv * ((double)X.x * X.x + (double)X.y * X.y);
+        if (!a.WcT) continue;  // the mixed-precision arm writes its own operand
         for (int t = 0; t < a.T; ++t) {
             const float2 S = a.Sc[(size_t)t * a.Pc + p];
             const float2 XS = cmul(X, S);
@@ -167,42 +169,56 @@ __global__ void k_prep_images(PrepArgs a) {
     if (threadIdx.x == 0) a.C2f[img] = 0.5 * (s + a.hxi2[img]);
 }
 
-// A-term: At[img][m] = 0.5 * sum_p MC2c[img][p] |F_m(p)|^2
-__global__ void k_coarse_aterm(const float* __restrict__ FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
-                               const float* __restrict__ MC2c, float* __restrict__ At) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int img = blockIdx.y;
-    if (m >= Mtot) return;
-    const float* w = MC2c + (size_t)img * Pc;
-    float acc = 0.f;
-    for (int p = 0; p < Pc; ++p) {
-        const float fr = FcT[(size_t)p * Mpad + m], fi = FcT[(size_t)(Pc_pad + p) * Mpad + m];
-        acc += w[p] * (fr * fr + fi * fi);
+// A-term: At[img][m] = 0.5 * sum_p MC2c[img][p] |F_m(p)|^2, rows padded to Mpad (zero beyond Mtot).
+// Block = 256 rotations x kATermImgs images: |F_m(p)|^2 is formed once per pixel and reused for every
+// image of the block; the images' weights pass through LDS in chunks of 256 pixels.
+constexpr int kATermImgs = 16;
+__global__ __launch_bounds__(256) void k_coarse_aterm(const float* __restrict__ FcT, int Mtot, int Mpad, int Pc,
+                                                      int Pc_pad, int n, const float* __restrict__ MC2c,
+                                                      float* __restrict__ At) {
+    __shared__ float w[kATermImgs][256];
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = blockIdx.y * kATermImgs;
+    const int ni = min(kATermImgs, n - i0);
+    float acc[kATermImgs];
+#pragma unroll
+    for (int i = 0; i < kATermImgs; ++i) acc[i] = 0.f;
+    for (int p0 = 0; p0 < Pc; p0 += 256) {
+        const int np = min(256, Pc - p0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < kATermImgs * 256; q += 256) {
+            const int i = q >> 8, p = q & 255;
+            w[i][p] = (i < ni && p < np) ? MC2c[(size_t)(i0 + i) * Pc + p0 + p] : 0.f;
+        }
+        __syncthreads();
+        if (m < Mtot) {
+            for (int p = 0; p < np; ++p) {
+                const float fr = FcT[(size_t)(p0 + p) * Mpad + m], fi = FcT[(size_t)(Pc_pad + p0 + p) * Mpad + m];
+                const float q = fr * fr + fi * fi;
+#pragma unroll
+                for (int i = 0; i < kATermImgs; ++i) acc[i] += w[i][p] * q;
+            }
+        }
     }
-    At[(size_t)img * Mtot + m] = 0.5f * acc;
+    if (m < Mpad) {
+#pragma unroll
+        for (int i = 0; i < kATermImgs; ++i)
+            if (i < ni) At[(size_t)(i0 + i) * Mpad + m] = m < Mtot ? 0.5f * acc[i] : 0.f;
+    }
 }
 
 // ------------------------------------------------------------------ coarse cross term (K2), FP32 MFMA
 // C[m][j] = sum_kk FcT[kk][m] * WcT[kk][j];  E[img][m][t] = At[img][m] - C[m][img*T+t].
 // 256 threads, 128x128 tile, BK = 16, v_mfma_f32_32x32x2f32, LDS double buffer.
-typedef float f16v __attribute__((ext_vector_type(16)));
-
 __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
     constexpr int BM = 128, BN = 128, BK = 16;
     __shared__ float As[2][BK][BM];
     __shared__ float Bs[2][BK][BN];
-    __shared__ int colmin[BN];
+    __shared__ int slot_min[kEpiSlots];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    // XCD-aware tile order: consecutive workgroups go round-robin over the 8 XCDs, so
-    // remap such that each XCD owns a contiguous band of tiles (shared operand panels).
-    const int nbm = a.Mpad / BM, nbn = a.Npad / BN, ntiles = nbm * nbn;
-    int bid = blockIdx.x;
-    if ((ntiles & 7) == 0) {  // bijection: bid = 8*loc + xcd  ->  xcd*(ntiles/8) + loc
-        const int per = ntiles >> 3;
-        bid = (bid & 7) * per + (bid >> 3);
-    }
-    const int bm = bid % nbm, bn = bid / nbm;
+    int bm, bn;
+    gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
     const int m0 = bm * BM, n0 = bn * BN;
     // global load mapping: 16 rows x 128 cols = 2048 floats = 512 float4 -> 2 per thread per operand
     const int lr = tid >> 5, lc = (tid & 31) * 4;  // rows lr and lr+8
@@ -252,46 +268,8 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
         if (kc + 1 < nk) store(cur ^ 1, ra, rb);
         __syncthreads();
     }
-    // epilogue: lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
-    if (tid < BN) colmin[tid] = 0x7fffffff;
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-            if (col >= a.Ncols) continue;
-            const int img = col / a.T, t = col - img * a.T;
-            float emin = INFINITY;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row < a.Mtot) {
-                    const float at = a.At[(size_t)img * a.Mtot + row];
-                    const float e = at - acc[i][j][r];
-                    a.E[((size_t)img * a.Mtot + row) * a.T + t] = e;
-                    if (a.valid[row]) emin = fminf(emin, e);
-                }
-            }
-            // per-column minimum in LDS, per-image minimum below (weight pass input)
-            if (emin < INFINITY) atomicMin(&colmin[wn * 64 + j * 32 + (lane & 31)], float_order_key(emin));
-        }
-    __syncthreads();
-    if (tid == 0) {  // at most a few images per tile: one global atomic each
-        int cur_img = -1, key = 0x7fffffff;
-        for (int c = 0; c < BN; ++c) {
-            const int col = n0 + c;
-            if (col >= a.Ncols) break;
-            const int img = col / a.T;
-            if (img != cur_img) {
-                if (cur_img >= 0 && key != 0x7fffffff) atomicMin(&a.emin_key[cur_img], key);
-                cur_img = img;
-                key = 0x7fffffff;
-            }
-            key = min(key, colmin[c]);
-        }
-        if (cur_img >= 0 && key != 0x7fffffff) atomicMin(&a.emin_key[cur_img], key);
-    }
+    gemm_epilogue<false>(acc, m0, n0, wm, wn, lane, tid, a.At, nullptr, a.E, a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols,
+                         a.T, slot_min);
 }
 
 // ------------------------------------------------------------------ task building
@@ -794,8 +772,8 @@ hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a) {
 
 hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
                                const float* MC2c, float* At) {
-    dim3 grid((Mtot + 255) / 256, n);
-    k_coarse_aterm<<<grid, 256, 0, s>>>(FcT, Mtot, Mpad, Pc, Pc_pad, MC2c, At);
+    dim3 grid((Mpad + 255) / 256, (n + kATermImgs - 1) / kATermImgs);
+    k_coarse_aterm<<<grid, 256, 0, s>>>(FcT, Mtot, Mpad, Pc, Pc_pad, n, MC2c, At);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -50,6 +50,7 @@ struct Cand {
 struct WShared {
     double poff[kMaxKT];
     double lpoff[kMaxKT];
+    float lpoff_f[kMaxKT];
     double hsum[kBinCount];
     unsigned int hcnt[kBinCount];
     double cw[kSortCap];
@@ -65,9 +66,11 @@ struct WShared {
     long long nsig;
 };
 
-template <typename Get>
-__device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, WeightOut wo,
-                            WShared& sh, uint32_t* sigmask, int TW, Cand cand, const int* emin_key) {
+// Pre(e, bound, lpoff_f): cheap FP32 test that is false only for entries whose exact FP64
+// du = u - u0 is >= kWin when bound = (float)(u0 + kWin) + margin (margin covers the FP32 rounding).
+template <typename Get, typename Pre>
+__device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, Pre pre,
+                            WeightOut wo, WShared& sh, uint32_t* sigmask, int TW, Cand cand, const int* emin_key) {
     constexpr int NT = 1024;
     const int tid = threadIdx.x;
     const int T = wp.T;
@@ -78,6 +81,7 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
                                      wp.prior_y[k], wp.sigma2_offset);
         sh.poff[q] = po;
         sh.lpoff[q] = po > 0 ? log(po) : -INFINITY;
+        sh.lpoff_f[q] = (float)sh.lpoff[q];
         lpo_max = fmax(lpo_max, sh.lpoff[q]);
     }
     lpo_max = block_reduce(lpo_max, [](double a, double b) { return fmax(a, b); }, sh.red_d);
@@ -102,6 +106,12 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
     }
     const double dmin = mn + cimg;  // (d + c) is monotone in d
     double u0 = mn - (wp.lpdf_max + lpo_max);
+    // FP32 pre-test bound: rounding of d - lpdf - lpoff and of u0 + kWin stays far below the margin
+    // (relative 2^-22 of magnitudes that the 1e-4 term covers; log priors are > -746)
+    auto pre_bound = [&](double a0) {
+        return (float)(a0 + kWin) + 1.0f + 1e-4f * (float)(fabs(a0) + fabs(mn) + kWin + fabs(wp.lpdf_max) + fabs(lpo_max));
+    };
+    float bound = pre_bound(u0);
     // ---- scan
     double sum = 0., mx = -1.;
     long long am = -1;
@@ -117,11 +127,12 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         am = -1;
         double umin = INFINITY;
         for (size_t e = tid; e < count; e += NT) {
+            if (!pre(e, bound, sh.lpoff_f)) continue;
             double d;
             int m, t;
             if (!get(e, d, m, t)) continue;
             const double u = u_of(d, m, t);
-            umin = fmin(umin, u);
+            umin = fmin(umin, u);  // over the entries that pass the pre-test
             const double du = u - u0;
             if (!(du < kWin)) continue;
             const double w = w_of(d, m, t, dmin);
@@ -144,8 +155,20 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
             }
         }
         umin = block_reduce(umin, [](double a, double b) { return fmin(a, b); }, sh.red_d);
+        // every entry with du < kWin passed the pre-test, so umin is the exact minimum of u whenever
+        // umin - u0 < kWin; in particular umin - u0 <= kAnchorSlack holds iff it holds for all entries
         if (!(umin - u0 > kAnchorSlack)) break;
+        if (!(umin - u0 < kWin)) {  // nothing inside the window: exact minimum from a full pass
+            umin = INFINITY;
+            for (size_t e = tid; e < count; e += NT) {
+                double d;
+                int m, t;
+                if (get(e, d, m, t)) umin = fmin(umin, u_of(d, m, t));
+            }
+            umin = block_reduce(umin, [](double a, double b) { return fmin(a, b); }, sh.red_d);
+        }
         u0 = umin;  // priors spread u widely: anchor the window at the true minimum and rescan
+        bound = pre_bound(u0);
     }
     sum = block_reduce(sum, [](double a, double b) { return a + b; }, sh.red_d);
     {
@@ -167,6 +190,7 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
             }
         } else {
             for (size_t e = tid; e < count; e += NT) {
+                if (!pre(e, bound, sh.lpoff_f)) continue;
                 double d;
                 int m, t;
                 if (!get(e, d, m, t)) continue;
@@ -332,7 +356,14 @@ __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const 
         d = (double)Ei[e];
         return true;
     };
-    weight_pass(img, (size_t)wp.Mtot * T, C2c[img], wp, get, wo, sh, sigmask, TW, cand, emin_key);
+    // d - lpdf_m - lpoff in FP32 from the per-rotation row {lpdf_m or -inf, k*T}
+    auto pre = [&](size_t e, float bound, const float* lpoff_f) -> bool {
+        const int m = (int)(((unsigned long long)e * magic) >> 40);
+        const int t = (int)e - m * T;
+        const float2 mr = wp.mrow[m];
+        return Ei[e] - mr.x - lpoff_f[__float_as_int(mr.y) + t] < bound;
+    };
+    weight_pass(img, (size_t)wp.Mtot * T, C2c[img], wp, get, pre, wo, sh, sigmask, TW, cand, emin_key);
 }
 
 __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
@@ -352,7 +383,8 @@ __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const do
         d = Di[e];
         return true;
     };
-    weight_pass(img, count, 0.0, wp, get, wo, sh, nullptr, 1, cand, nullptr);
+    auto pre = [](size_t, float, const float*) { return true; };
+    weight_pass(img, count, 0.0, wp, get, pre, wo, sh, nullptr, 1, cand, nullptr);
 }
 
 }  // namespace

@@ -11,7 +11,7 @@ perturbed sampling tables of the product's own HealpixSampler port.
 """
 import numpy as np
 
-from . import pose_table2d, pose_table3d, shift_table, translations
+from . import PRECISIONS, pose_table2d, pose_table3d, shift_table, translations
 
 PI = 3.14159265358979323846
 
@@ -380,27 +380,29 @@ def make_particles_device(engine, nr_classes, n, size, ori, angpix, offset_range
     return F, ctf, truth, sigma2
 
 
-def engine_config(prob):
+def engine_config(prob, precision="fp32"):
     keys = ("ori_size", "current_size", "coarse_size", "nr_classes", "nr_groups", "adaptive_oversampling",
             "nr_dir", "nr_psi", "nr_trans", "adaptive_fraction", "sigma2_fudge", "sigma2_offset",
             "avg_norm_correction", "do_ctf_correction", "refs_are_ctf_corrected", "do_norm_correction", "do_map")
     cfg = {k: prob[k] for k in keys}
     cfg["ref_dim"] = 3 if prob["is_3d"] else 2
+    cfg["precision"] = PRECISIONS[precision]
     return cfg
 
 
-def load_engine_tables(engine, prob):
+def load_engine_tables(engine, prob, precision="fp32"):
     """configure + sampling tables + reference grids (everything but the model)."""
-    engine.configure(**engine_config(prob))
+    engine.configure(**engine_config(prob, precision))
     engine.set_sampling(prob["pose_coarse"], prob["pose_fine"], prob["shift_coarse_x"], prob["shift_coarse_y"],
                         prob["shift_fine_x"], prob["shift_fine_y"], prob["trans_x"], prob["trans_y"])
     for k in range(prob["nr_classes"]):
         engine.set_reference(k, prob["grids"][k])
 
 
-def load_engine(engine, prob):
-    """Configure an Engine with a problem dict (same tables the oracle receives)."""
-    load_engine_tables(engine, prob)
+def load_engine(engine, prob, precision="fp32"):
+    """Configure an Engine with a problem dict (same tables the oracle receives); precision is the
+    arithmetic of the coarse cross term ("fp32", "bf16" or "fp16")."""
+    load_engine_tables(engine, prob, precision)
     engine.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
                      prob["prior_offy"])
     engine.reset_sums()
