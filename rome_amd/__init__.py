@@ -29,7 +29,7 @@ EXPORTED = (
 )
 
 STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
-          "fine_weights", "post", "store")
+          "fine_weights", "post", "store", "backproject")
 
 
 class RomeError(RuntimeError):

@@ -122,8 +122,14 @@ struct rome_ctx {
     DevBuf<float2> Pfine;
     DevBuf<double> D2;
     DevBuf<int2> amax_mt;
+    // binned backprojection
+    static constexpr size_t kBinnedBytes = (size_t)24 << 30;  // slice values + records per batch
+    DevBuf<float2> slice_val;
+    DevBuf<double> slice_w;
+    DevBuf<unsigned> bp_count, bp_off, bp_cursor;
+    DevBuf<unsigned long long> bp_rec;
     // stage timing with HIP events on the engine stream (rome_set_profiling)
-    static constexpr int kStages = 10;
+    static constexpr int kStages = 11;
     bool profiling = false;
     hipEvent_t ev[kStages][2] = {};
     bool ev_used[kStages] = {};
@@ -294,7 +300,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     hipStream_t s = ctx->stream;
     const int T = ctx->T, M = ctx->Mtot;
     const int Ncols = n * T;
-    const int Npad = (Ncols + 127) / 128 * 128;
+    const int Npad = (Ncols + 255) / 256 * 256;
     const int Kdim = 2 * ctx->Pc_pad;
     const int prec = c.precision;
     const int K16 = ctx->K16();
@@ -393,7 +399,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     } else {
         LowpGemmArgs la{ctx->A16.p, ctx->W16.p, ctx->At.p, ctx->inv_scale.p, ctx->E.p, ctx->valid.p,
                         ctx->emin_key.p, K16, ctx->Mpad, Npad, M, Ncols, T,
-                        gemm_sub_band(nbm, (size_t)128 * K16 * 2, (size_t)5 << 19)};
+                        gemm_sub_band(ctx->Mpad / 256, (size_t)256 * K16 * 2, (size_t)5 << 19)};
         HIPCHK(launch_gemm_lowp(s, prec, la));
     }
     stage_end(ctx, 2);
@@ -557,9 +563,58 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.wsum_prior_y = ctx->wsum_prior_y.p;
         pp.wsum_scalars = ctx->wsum_scalars.p;
         pp.wnorm = ctx->wnorm.p;
+        // 3D: slices to HBM, then the binned scatter (bounded by a slice buffer of kBinnedBytes)
+        const long long S = ntasks * NOR;
+        const bool binned = ctx->is3d && S > 0 && (unsigned long long)S * ctx->Pf < (1ULL << 32) &&
+                            (size_t)S * ctx->Pf * 16 <= rome_ctx::kBinnedBytes;
+        pp.slice_val = nullptr;
+        pp.slice_w = nullptr;
+        if (binned) {
+            HIPCHK(ctx->slice_val.reserve((size_t)S * ctx->Pf));
+            HIPCHK(ctx->slice_w.reserve((size_t)S));
+            pp.slice_val = ctx->slice_val.p;
+            pp.slice_w = ctx->slice_w.p;
+        }
         stage_begin(ctx, 8);
         HIPCHK(launch_post(s, n, NOR, NOT, pp));
         stage_end(ctx, 8);
+        if (binned) {
+            BpArgs ba;
+            ba.K = ctx->K;
+            ba.Pf = ctx->Pf;
+            ba.NOR = NOR;
+            ba.R = ctx->R;
+            ba.pad = ctx->pad;
+            ba.padF = ctx->padF;
+            ba.c = ctx->cgrid;
+            ba.r_max = ctx->r_max;
+            ba.grid_stride = ctx->grid_n;
+            ba.nbx = (ctx->padF + 7) / 8;
+            ba.nby = (ctx->pad + 7) / 8;
+            ba.nbz = (ctx->pad + 7) / 8;
+            ba.nb_class = ba.nbx * ba.nby * ba.nbz;
+            const size_t nb = (size_t)ba.nb_class * ctx->K;
+            HIPCHK(ctx->bp_count.reserve(nb));
+            HIPCHK(ctx->bp_off.reserve(nb + 1));
+            HIPCHK(ctx->bp_cursor.reserve(nb));
+            HIPCHK(ctx->bp_rec.reserve((size_t)S * ctx->Pf));
+            ba.task_img = ctx->task_img.p;
+            ba.task_m = ctx->task_m.p;
+            ba.Ai_fine = ctx->Ai_f.p;
+            ba.f_xy = ctx->f_xy.p;
+            ba.MC2bp = ctx->MC2bp.p;
+            ba.slice_w = ctx->slice_w.p;
+            ba.slice_val = ctx->slice_val.p;
+            ba.count = ctx->bp_count.p;
+            ba.off = ctx->bp_off.p;
+            ba.cursor = ctx->bp_cursor.p;
+            ba.rec = ctx->bp_rec.p;
+            ba.bp_data = ctx->bp_data.p;
+            ba.bp_weight = ctx->bp_weight.p;
+            stage_begin(ctx, 10);
+            HIPCHK(launch_backproject_binned(s, S, ba));
+            stage_end(ctx, 10);
+        }
     } else {
         HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
     }
@@ -736,7 +791,7 @@ int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
     ctx->T = c.nr_trans;
     ctx->R = c.nr_dir * c.nr_psi;
     ctx->Mtot = ctx->K * ctx->R;
-    ctx->Mpad = (ctx->Mtot + 127) / 128 * 128;
+    ctx->Mpad = (ctx->Mtot + 255) / 256 * 256;  // 256-row tiles of the BF16/FP16 cross term
     ctx->two_pass = c.adaptive_oversampling > 0;
     ctx->post_enabled = !(ctx->is3d && !ctx->two_pass);  // 3D stores no fine weights without oversampling
     ctx->NORf = ctx->two_pass ? (ctx->is3d ? 8 : 2) : 1;

@@ -30,26 +30,29 @@ DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
     }
 }
 
-constexpr int kEpiSlots = 130;  // images one 128-column tile can touch (T >= 1): 128 + 2
+// Images one tile of TN columns can touch (T >= 1): TN + 2.
+template <int TN>
+constexpr int epi_slots() { return TN + 2; }
 
 // E[img][m][t] = At[img][m] - s_img * C[m][img*T + t] (s_img = inv_scale[img] when SCALED, else 1)
-// (At rows and the valid flags padded to Mpad, 16-byte aligned)
-// for the 128x128 tile at (m0, n0), and the per-image minimum of E over rotations with a positive
+// for a tile whose first column is n0 (TN columns); this wave holds NI x NJ accumulator blocks
+// starting at row r0 and column c0.  Also the per-image minimum of E over rotations with a positive
 // prior into emin_key (ordered-int keys): LDS atomics per image slot, then one global atomic per
-// image the tile touches.  Called by all threads of the block (contains barriers).
-template <bool SCALED>
-DEV void gemm_epilogue(const f16v (&acc)[2][2], int m0, int n0, int wm, int wn, int lane, int tid,
+// image the tile touches.  At rows and the valid flags are padded to Mpad (16-byte aligned rows).
+// Called by all threads of the block (contains barriers).
+template <bool SCALED, int NI, int NJ, int TN>
+DEV void gemm_epilogue(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
                        const float* __restrict__ At, const float* __restrict__ inv_scale, float* __restrict__ E,
                        const unsigned char* __restrict__ valid, int* __restrict__ emin_key, int Mtot, int Mpad,
                        int Ncols, int T, int* slot_min) {
     const int img0 = n0 / T;
-    const int last = min(n0 + 128, Ncols) - 1;
+    const int last = min(n0 + TN, Ncols) - 1;
     const int nslot = last >= n0 ? last / T - img0 + 1 : 0;
-    if (tid < nslot) slot_min[tid] = INT_MAX;
+    for (int q = tid; q < nslot; q += blockDim.x) slot_min[q] = INT_MAX;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+    for (int j = 0; j < NJ; ++j) {
+        const int col = c0 + j * 32 + (lane & 31);
         if (col >= Ncols) continue;
         const int img = col / T, t = col - img * T;
         const float s = SCALED ? inv_scale[img] : 1.f;
@@ -57,11 +60,11 @@ DEV void gemm_epilogue(const f16v (&acc)[2][2], int m0, int n0, int wm, int wn, 
         float* Er = E + (size_t)img * Mtot * T + t;
         float emin = INFINITY;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 // registers 4q..4q+3 hold four consecutive rows: one 16-byte At load, one 4-byte valid load
-                const int base = m0 + wm * 64 + i * 32 + 8 * q + 4 * (lane >> 5);
+                const int base = r0 + i * 32 + 8 * q + 4 * (lane >> 5);
                 const float4 at = *reinterpret_cast<const float4*>(Ar + base);
                 const uint32_t v4 = *reinterpret_cast<const uint32_t*>(valid + base);
                 const float atv[4] = {at.x, at.y, at.z, at.w};
@@ -79,7 +82,8 @@ DEV void gemm_epilogue(const f16v (&acc)[2][2], int m0, int n0, int wm, int wn, 
         if (emin < INFINITY) atomicMin(&slot_min[img - img0], float_order_key(emin));
     }
     __syncthreads();
-    if (tid < nslot && slot_min[tid] != INT_MAX) atomicMin(&emin_key[img0 + tid], slot_min[tid]);
+    for (int q = tid; q < nslot; q += blockDim.x)
+        if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img0 + q], slot_min[q]);
 }
 
 }  // namespace rome

@@ -10,7 +10,7 @@
 //
 // v_mfma_f32_32x32x16_{bf16,f16}: lane l supplies A[l%32][8*(l/32) .. +7] and B[8*(l/32) .. +7][l%32];
 // D as for the FP32 instruction (row 8*(r/4) + 4*(l/32) + r%4, col l%32).  K-contiguous operands so
-// each lane loads its 8 values with one 16-byte LDS read.  128x128x64 tiles, 4 waves, LDS double buffer.
+// each lane loads its 8 values with one 16-byte LDS read.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,72 +35,89 @@ struct Lowp<_Float16> {
     static DEV f16v mfma(v8 a, v8 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
 };
 
+// 256x256 output tile per block of 8 waves (2 along rotations x 4 along columns); each wave owns
+// 128x64 = 4x2 accumulator blocks (128 accumulator registers, two waves per SIMD).  BK = 64: per
+// 16-k step a wave reads 6 fragments from LDS for 8 MFMAs; the global loads of chunk c+1 (held in
+// eight named 16-byte registers: an array here would live in scratch) are in flight while chunk c is
+// multiplied by both waves of the SIMD.  Two LDS stages of 72 KB, one block per CU.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_lowp(LowpGemmArgs a) {
-    constexpr int BM = 128, BN = 128, BK = 64, LDW = BK + 8;
+__global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
+    constexpr int BM = 256, BN = 256, BK = 64, LDW = BK + 8;  // 144-byte LDS rows: conflict-free b128 reads
     typedef typename Lowp<T>::v8 v8;
     __shared__ __attribute__((aligned(16))) T As[2][BM][LDW];
     __shared__ __attribute__((aligned(16))) T Bs[2][BN][LDW];
-    __shared__ int slot_min[kEpiSlots];
+    __shared__ int slot_min[epi_slots<BN>()];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave >> 2, wn = wave & 3;
     int bm, bn;
     gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
     const int m0 = bm * BM, n0 = bn * BN;
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
-    f16v acc[2][2];
+    f16v acc[4][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
     const int nk = a.K16 / BK;
-    uint4 ra[4], rb[4];
-    auto load = [&](int kc) {
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const int idx = tid + 256 * h;
-            const int row = idx >> 3, c8 = idx & 7;
-            ra[h] = *reinterpret_cast<const uint4*>(A + (size_t)(m0 + row) * a.K16 + kc * BK + c8 * 8);
-            rb[h] = *reinterpret_cast<const uint4*>(B + (size_t)(n0 + row) * a.K16 + kc * BK + c8 * 8);
-        }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const int idx = tid + 256 * h;
-            const int row = idx >> 3, c8 = idx & 7;
-            *reinterpret_cast<uint4*>(&As[buf][row][c8 * 8]) = ra[h];
-            *reinterpret_cast<uint4*>(&Bs[buf][row][c8 * 8]) = rb[h];
-        }
-    };
-    load(0);
-    store(0);
+    // chunk = 256 rows x 64 k of each operand = 2048 16-byte pieces: 4 per thread per operand
+    // (rows tid / 8 + 64 h, piece tid % 8)
+    const int prow = tid >> 3, pcol = (tid & 7) * 8;
+    const T* Ag = A + (size_t)(m0 + prow) * a.K16 + pcol;
+    const T* Bg = B + (size_t)(n0 + prow) * a.K16 + pcol;
+    const size_t q64 = (size_t)64 * a.K16;
+#define LOWP_LOAD(kc)                                                        \
+    do {                                                                             \
+        ra0 = *reinterpret_cast<const uint4*>(Ag + (kc) * BK);                     \
+        ra1 = *reinterpret_cast<const uint4*>(Ag + q64 + (kc) * BK);               \
+        ra2 = *reinterpret_cast<const uint4*>(Ag + 2 * q64 + (kc) * BK);           \
+        ra3 = *reinterpret_cast<const uint4*>(Ag + 3 * q64 + (kc) * BK);           \
+        rb0 = *reinterpret_cast<const uint4*>(Bg + (kc) * BK);                     \
+        rb1 = *reinterpret_cast<const uint4*>(Bg + q64 + (kc) * BK);               \
+        rb2 = *reinterpret_cast<const uint4*>(Bg + 2 * q64 + (kc) * BK);           \
+        rb3 = *reinterpret_cast<const uint4*>(Bg + 3 * q64 + (kc) * BK);           \
+    } while (0)
+#define LOWP_STORE(st)                                                       \
+    do {                                                                             \
+        *reinterpret_cast<uint4*>(&As[st][prow][pcol]) = ra0;                      \
+        *reinterpret_cast<uint4*>(&As[st][prow + 64][pcol]) = ra1;                 \
+        *reinterpret_cast<uint4*>(&As[st][prow + 128][pcol]) = ra2;                \
+        *reinterpret_cast<uint4*>(&As[st][prow + 192][pcol]) = ra3;                \
+        *reinterpret_cast<uint4*>(&Bs[st][prow][pcol]) = rb0;                      \
+        *reinterpret_cast<uint4*>(&Bs[st][prow + 64][pcol]) = rb1;                 \
+        *reinterpret_cast<uint4*>(&Bs[st][prow + 128][pcol]) = rb2;                \
+        *reinterpret_cast<uint4*>(&Bs[st][prow + 192][pcol]) = rb3;                \
+    } while (0)
+#define LOWP_COMPUTE(st)                                                                                    \
+    do {                                                                                                    \
+        _Pragma("unroll") for (int kk = 0; kk < BK; kk += 16) {                                             \
+            v8 av[4], bv[2];                                                                                \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i) av[i] =                                           \
+                *reinterpret_cast<const v8*>(&As[st][wm * 128 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]); \
+            _Pragma("unroll") for (int j = 0; j < 2; ++j) bv[j] =                                           \
+                *reinterpret_cast<const v8*>(&Bs[st][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);  \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                                   \
+                _Pragma("unroll") for (int j = 0; j < 2; ++j) acc[i][j] = Lowp<T>::mfma(av[i], bv[j], acc[i][j]); \
+        }                                                                                                   \
+    } while (0)
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+    LOWP_LOAD(0);
+    LOWP_STORE(0);
     __syncthreads();
     for (int kc = 0; kc < nk; ++kc) {
-        const int cur = kc & 1;
-        if (kc + 1 < nk) load(kc + 1);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            v8 av[2], bv[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-                av[i] = *reinterpret_cast<const v8*>(&As[cur][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                bv[j] = *reinterpret_cast<const v8*>(&Bs[cur][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = Lowp<T>::mfma(av[i], bv[j], acc[i][j]);
-        }
-        if (kc + 1 < nk) store(cur ^ 1);
+        const int st = kc & 1;
+        if (kc + 1 < nk) LOWP_LOAD(kc + 1);
+        LOWP_COMPUTE(st);
+        if (kc + 1 < nk) LOWP_STORE(st ^ 1);
         __syncthreads();
     }
-    gemm_epilogue<true>(acc, m0, n0, wm, wn, lane, tid, a.At, a.inv_scale, a.E, a.valid, a.emin_key, a.Mtot,
-                        a.Mpad, a.Ncols, a.T, slot_min);
+#undef LOWP_LOAD
+#undef LOWP_STORE
+#undef LOWP_COMPUTE
+    gemm_epilogue<true, 4, 2, BN>(acc, m0 + wm * 128, n0 + wn * 64, n0, lane, tid, a.At, a.inv_scale, a.E, a.valid,
+                                  a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
 }
 
 // sF: power of two that brings max |FcT| into [0.5, 1) (exact rescaling)
@@ -175,10 +192,10 @@ __global__ void k_prep_lowp(PrepLowpArgs a) {
 }  // namespace
 
 hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a) {
-    if (a.Mpad % 128 || a.Npad % 128 || a.K16 % 64 || a.K16 <= 0) return hipErrorInvalidValue;
-    const int tiles = (a.Mpad / 128) * (a.Npad / 128);
-    if (precision == 1) k_gemm_lowp<__bf16><<<tiles, 256, 0, s>>>(a);
-    else if (precision == 2) k_gemm_lowp<_Float16><<<tiles, 256, 0, s>>>(a);
+    if (a.Mpad % 256 || a.Npad % 256 || a.K16 % 64 || a.K16 <= 0) return hipErrorInvalidValue;
+    const int tiles = (a.Mpad / 256) * (a.Npad / 256);
+    if (precision == 1) k_gemm_lowp<__bf16><<<tiles, 512, 0, s>>>(a);
+    else if (precision == 2) k_gemm_lowp<_Float16><<<tiles, 512, 0, s>>>(a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -183,6 +183,29 @@ struct PostArgs {
     double* wsum_prior_y;
     double* wsum_scalars;       // [4]
     double* wnorm;              // [n]
+    // binned 3D backprojection: slices go to HBM instead of the grids (null: direct scatter)
+    float2* slice_val;          // [ntask * NOR][Pf]
+    double* slice_w;            // [ntask * NOR]
+};
+
+// Binned 3D backprojection (backproject.hip)
+struct BpArgs {
+    int K, Pf, NOR, R, pad, padF, c, r_max;
+    size_t grid_stride;
+    int nbx, nby, nbz, nb_class;  // bricks of 8 base voxels per axis (x over padF, y/z over pad)
+    const int* task_img;
+    const int* task_m;
+    const double* Ai_fine;        // [R][NOR][6]
+    const int* f_xy;
+    const float* MC2bp;           // [n][Pf]
+    const double* slice_w;        // [S] sum of the slice's normalised significant weights (0: none)
+    const float2* slice_val;      // [S][Pf]
+    unsigned* count;              // [nb]
+    unsigned* off;                // [nb + 1]
+    unsigned* cursor;             // [nb]
+    unsigned long long* rec;      // [records] slice << 32 | pixel
+    float* bp_data;
+    float* bp_weight;
 };
 
 struct StoreArgs {
@@ -239,6 +262,7 @@ hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, 
 hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
                                 const int* blk_t, const int* task_m, int n, int NE, int2* out);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
+hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a);
 
 // precision: 1 BF16, 2 FP16
 hipError_t launch_absmax(hipStream_t s, const float* x, size_t n, int* out);

@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
     constexpr int BM = 128, BN = 128, BK = 16;
     __shared__ float As[2][BK][BM];
     __shared__ float Bs[2][BK][BN];
-    __shared__ int slot_min[kEpiSlots];
+    __shared__ int slot_min[epi_slots<BN>()];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int bm, bn;
@@ -231,28 +231,28 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
     const int nk = a.K / BK;
-    auto load = [&](int kc, float4 (&ra)[2], float4 (&rb)[2]) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int row = kc * BK + lr + 8 * h;
-            ra[h] = *reinterpret_cast<const float4*>(a.A + (size_t)row * a.Mpad + m0 + lc);
-            rb[h] = *reinterpret_cast<const float4*>(a.B + (size_t)row * a.Npad + n0 + lc);
-        }
-    };
-    auto store = [&](int buf, const float4 (&ra)[2], const float4 (&rb)[2]) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            *reinterpret_cast<float4*>(&As[buf][lr + 8 * h][lc]) = ra[h];
-            *reinterpret_cast<float4*>(&Bs[buf][lr + 8 * h][lc]) = rb[h];
-        }
-    };
-    float4 ra[2], rb[2];
-    load(0, ra, rb);
-    store(0, ra, rb);
+    // staging in named registers (arrays of them end up in scratch)
+    const float* Ag = a.A + (size_t)lr * a.Mpad + m0 + lc;
+    const float* Bg = a.B + (size_t)lr * a.Npad + n0 + lc;
+    const size_t a8 = (size_t)8 * a.Mpad, b8 = (size_t)8 * a.Npad;
+    const size_t astep = (size_t)BK * a.Mpad, bstep = (size_t)BK * a.Npad;
+    float4 ra0 = *reinterpret_cast<const float4*>(Ag), ra1 = *reinterpret_cast<const float4*>(Ag + a8);
+    float4 rb0 = *reinterpret_cast<const float4*>(Bg), rb1 = *reinterpret_cast<const float4*>(Bg + b8);
+    *reinterpret_cast<float4*>(&As[0][lr][lc]) = ra0;
+    *reinterpret_cast<float4*>(&As[0][lr + 8][lc]) = ra1;
+    *reinterpret_cast<float4*>(&Bs[0][lr][lc]) = rb0;
+    *reinterpret_cast<float4*>(&Bs[0][lr + 8][lc]) = rb1;
     __syncthreads();
     for (int kc = 0; kc < nk; ++kc) {
         const int cur = kc & 1;
-        if (kc + 1 < nk) load(kc + 1, ra, rb);
+        if (kc + 1 < nk) {
+            const float* an = Ag + (size_t)(kc + 1) * astep;
+            const float* bnp = Bg + (size_t)(kc + 1) * bstep;
+            ra0 = *reinterpret_cast<const float4*>(an);
+            ra1 = *reinterpret_cast<const float4*>(an + a8);
+            rb0 = *reinterpret_cast<const float4*>(bnp);
+            rb1 = *reinterpret_cast<const float4*>(bnp + b8);
+        }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
             float av[2], bv[2];
@@ -265,11 +265,16 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
 #pragma unroll
                 for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
-        if (kc + 1 < nk) store(cur ^ 1, ra, rb);
+        if (kc + 1 < nk) {
+            *reinterpret_cast<float4*>(&As[cur ^ 1][lr][lc]) = ra0;
+            *reinterpret_cast<float4*>(&As[cur ^ 1][lr + 8][lc]) = ra1;
+            *reinterpret_cast<float4*>(&Bs[cur ^ 1][lr][lc]) = rb0;
+            *reinterpret_cast<float4*>(&Bs[cur ^ 1][lr + 8][lc]) = rb1;
+        }
         __syncthreads();
     }
-    gemm_epilogue<false>(acc, m0, n0, wm, wn, lane, tid, a.At, nullptr, a.E, a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols,
-                         a.T, slot_min);
+    gemm_epilogue<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr, a.E, a.valid,
+                                   a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
 }
 
 // ------------------------------------------------------------------ task building
@@ -599,7 +604,18 @@ __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
                     }
                 }
             }
-            if (wslice > 0.) {
+            if (a.slice_val) {  // binned 3D backprojection: the slice goes to HBM, the grids later
+                const size_t sl = (size_t)q * NOR + o;
+                if (wslice > 0.) {
+                    float2* V = a.slice_val + sl * a.Pf;
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        const int p = tid + j * NT;
+                        if (p < a.Pf) V[p] = acc[j];
+                    }
+                }
+                if (tid == 0) a.slice_w[sl] = wslice;
+            } else if (wslice > 0.) {
                 // trilinear scatter of the slice (MyBackProjector::backproject, src/reconstruct.cpp:977-1164)
                 float* gd = a.bp_data + a.grid_stride * 2 * k;
                 float* gw = a.bp_weight + a.grid_stride * k;

@@ -66,8 +66,9 @@ struct WShared {
     long long nsig;
 };
 
-// Pre(e, bound, lpoff_f): cheap FP32 test that is false only for entries whose exact FP64
-// du = u - u0 is >= kWin when bound = (float)(u0 + kWin) + margin (margin covers the FP32 rounding).
+// Pre(e0, bound, lpoff_f) -> bit mask over the Pre::VW entries e0 .. e0 + VW - 1 (those < count): a
+// cheap FP32 test whose bit is clear only for entries whose exact FP64 du = u - u0 is >= kWin when
+// bound = (float)(u0 + kWin) + margin (the margin covers the FP32 rounding).
 template <typename Get, typename Pre>
 __device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, Pre pre,
                             WeightOut wo, WShared& sh, uint32_t* sigmask, int TW, Cand cand, const int* emin_key) {
@@ -112,6 +113,25 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         return (float)(a0 + kWin) + 1.0f + 1e-4f * (float)(fabs(a0) + fabs(mn) + kWin + fabs(wp.lpdf_max) + fabs(lpo_max));
     };
     float bound = pre_bound(u0);
+    // every entry that may lie inside the window, each thread in increasing e (two groups of VW in
+    // flight per step): fn(e)
+    auto for_window = [&](float bnd, auto&& fn) {
+        constexpr size_t VW = Pre::VW, STEP = (size_t)NT * VW;
+        for (size_t e0 = (size_t)tid * VW; e0 < count; e0 += 2 * STEP) {
+            unsigned m0 = pre(e0, bnd, sh.lpoff_f);
+            unsigned m1 = pre(e0 + STEP, bnd, sh.lpoff_f);
+            while (m0) {
+                const int u = __builtin_ctz(m0);
+                m0 &= m0 - 1;
+                fn(e0 + u);
+            }
+            while (m1) {
+                const int u = __builtin_ctz(m1);
+                m1 &= m1 - 1;
+                fn(e0 + STEP + u);
+            }
+        }
+    };
     // ---- scan
     double sum = 0., mx = -1.;
     long long am = -1;
@@ -126,18 +146,17 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         mx = -1.;
         am = -1;
         double umin = INFINITY;
-        for (size_t e = tid; e < count; e += NT) {
-            if (!pre(e, bound, sh.lpoff_f)) continue;
+        for_window(bound, [&](size_t e) {
             double d;
             int m, t;
-            if (!get(e, d, m, t)) continue;
+            if (!get(e, d, m, t)) return;
             const double u = u_of(d, m, t);
             umin = fmin(umin, u);  // over the entries that pass the pre-test
             const double du = u - u0;
-            if (!(du < kWin)) continue;
+            if (!(du < kWin)) return;
             const double w = w_of(d, m, t, dmin);
             sum += w;
-            if (w > mx) {  // strided visit in increasing e: keeps this thread's first maximum
+            if (w > mx) {  // each thread visits increasing e: keeps this thread's first maximum
                 mx = w;
                 am = (long long)e;
             }
@@ -153,7 +172,7 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
                     cand.e[o] = (long long)e;
                 }
             }
-        }
+        });
         umin = block_reduce(umin, [](double a, double b) { return fmin(a, b); }, sh.red_d);
         // every entry with du < kWin passed the pre-test, so umin is the exact minimum of u whenever
         // umin - u0 < kWin; in particular umin - u0 <= kAnchorSlack holds iff it holds for all entries
@@ -189,16 +208,15 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
                 if (du >= lo && du < hi) fn(cand.w[o], du, cand.e[o]);
             }
         } else {
-            for (size_t e = tid; e < count; e += NT) {
-                if (!pre(e, bound, sh.lpoff_f)) continue;
+            for_window(bound, [&](size_t e) {
                 double d;
                 int m, t;
-                if (!get(e, d, m, t)) continue;
+                if (!get(e, d, m, t)) return;
                 const double du = u_of(d, m, t) - u0;
-                if (!(du >= lo && du < hi && du < kWin)) continue;
+                if (!(du >= lo && du < hi && du < kWin)) return;
                 const double w = w_of(d, m, t, dmin);
                 if (w > 0.) fn(w, du, (long long)e);
-            }
+            });
         }
     };
     // ---- select
@@ -340,6 +358,50 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
     });
 }
 
+// Coarse pre-test: d - lpdf_m - lpoff in FP32 from the per-rotation row {lpdf_m or -inf, k*T}, four
+// consecutive entries per call (one 16-byte load of E when the image's row is 16-byte aligned).
+struct CoarsePre {
+    static constexpr int VW = 4;
+    const float* Ei;
+    size_t count;
+    unsigned long long magic;
+    int T;
+    const float2* mrow;
+    bool vec;
+    DEV unsigned operator()(size_t e0, float bound, const float* lpoff_f) const {
+        if (e0 >= count) return 0u;
+        float v[4];
+        if (vec) {
+            const float4 q = *reinterpret_cast<const float4*>(Ei + e0);
+            v[0] = q.x;
+            v[1] = q.y;
+            v[2] = q.z;
+            v[3] = q.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = e0 + u < count ? Ei[e0 + u] : 0.f;
+        }
+        unsigned msk = 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t e = e0 + u;
+            if (e >= count) break;
+            const int m = (int)(((unsigned long long)e * magic) >> 40);
+            const int t = (int)e - m * T;
+            const float2 mr = mrow[m];
+            if (v[u] - mr.x - lpoff_f[__float_as_int(mr.y) + t] < bound) msk |= 1u << u;
+        }
+        return msk;
+    }
+};
+
+// Fine pass: every entry goes to the exact test.
+struct AllPre {
+    static constexpr int VW = 1;
+    size_t count;
+    DEV unsigned operator()(size_t e0, float, const float*) const { return e0 < count ? 1u : 0u; }
+};
+
 __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const float* __restrict__ E,
                                                          const double* __restrict__ C2c,
                                                          const int* __restrict__ emin_key, WeightOut wo,
@@ -356,13 +418,7 @@ __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const 
         d = (double)Ei[e];
         return true;
     };
-    // d - lpdf_m - lpoff in FP32 from the per-rotation row {lpdf_m or -inf, k*T}
-    auto pre = [&](size_t e, float bound, const float* lpoff_f) -> bool {
-        const int m = (int)(((unsigned long long)e * magic) >> 40);
-        const int t = (int)e - m * T;
-        const float2 mr = wp.mrow[m];
-        return Ei[e] - mr.x - lpoff_f[__float_as_int(mr.y) + t] < bound;
-    };
+    const CoarsePre pre{Ei, (size_t)wp.Mtot * T, magic, T, wp.mrow, ((size_t)wp.Mtot * T) % 4 == 0};
     weight_pass(img, (size_t)wp.Mtot * T, C2c[img], wp, get, pre, wo, sh, sigmask, TW, cand, emin_key);
 }
 
@@ -383,7 +439,7 @@ __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const do
         d = Di[e];
         return true;
     };
-    auto pre = [](size_t, float, const float*) { return true; };
+    const AllPre pre{count};
     weight_pass(img, count, 0.0, wp, get, pre, wo, sh, nullptr, 1, cand, nullptr);
 }
 
