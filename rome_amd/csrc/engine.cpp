@@ -275,7 +275,7 @@ WeightParams weight_params(rome_ctx* ctx) {
     wp.R = ctx->R;
     wp.Mtot = ctx->Mtot;
     wp.T = ctx->T;
-    wp.magicT = ((1ULL << 40) + (unsigned long long)ctx->T - 1) / (unsigned long long)ctx->T;
+    wp.magicM = ((1ULL << 48) + (unsigned long long)ctx->Mtot - 1) / (unsigned long long)ctx->Mtot;
     wp.pdf_m = ctx->pdf_m.p;
     wp.lpdf_m = ctx->lpdf_m.p;
     wp.k_m = ctx->k_m.p;
@@ -792,6 +792,9 @@ int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
     ctx->R = c.nr_dir * c.nr_psi;
     ctx->Mtot = ctx->K * ctx->R;
     ctx->Mpad = (ctx->Mtot + 255) / 256 * 256;  // 256-row tiles of the BF16/FP16 cross term
+    if ((double)ctx->Mtot * ctx->Mtot * ctx->T >= 281474976710656.0)  // 2^48: coarse index division
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "coarse grid too large (%d rotations x %d translations)", ctx->Mtot,
+                    ctx->T);
     ctx->two_pass = c.adaptive_oversampling > 0;
     ctx->post_enabled = !(ctx->is3d && !ctx->two_pass);  // 3D stores no fine weights without oversampling
     ctx->NORf = ctx->two_pass ? (ctx->is3d ? 8 : 2) : 1;
@@ -1121,10 +1124,14 @@ int rome_get_coarse_diff2(rome_ctx* ctx, double* out) {
     HIPCHK(hipMemcpy(c2.data(), ctx->C2c.p, n * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<unsigned char> valid(ctx->Mtot);
     HIPCHK(hipMemcpy(valid.data(), ctx->valid.p, ctx->Mtot, hipMemcpyDeviceToHost));
+    // device layout [img][t][m] -> [img][m][t]
+    const int M = ctx->Mtot, T = ctx->T;
     for (int i = 0; i < n; ++i)
-        for (size_t q = 0; q < per; ++q)
-            out[(size_t)i * per + q] =
-                valid[q / ctx->T] ? (double)e[(size_t)i * per + q] + c2[i] : std::numeric_limits<double>::quiet_NaN();
+        for (int m = 0; m < M; ++m)
+            for (int t = 0; t < T; ++t)
+                out[(size_t)i * per + (size_t)m * T + t] =
+                    valid[m] ? (double)e[(size_t)i * per + (size_t)t * M + m] + c2[i]
+                             : std::numeric_limits<double>::quiet_NaN();
     return ROME_OK;
 }
 

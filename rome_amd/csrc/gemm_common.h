@@ -10,6 +10,7 @@
 namespace rome {
 
 typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 // blockIdx -> (bm, bn).  Workgroups are dispatched round-robin over the 8 XCDs (XCD = bid % 8).
 // When the m-tiles split evenly, XCD x owns the band [x*band, (x+1)*band) of m-tiles and walks it in
@@ -34,11 +35,13 @@ DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
 template <int TN>
 constexpr int epi_slots() { return TN + 2; }
 
-// E[img][m][t] = At[img][m] - s_img * C[m][img*T + t] (s_img = inv_scale[img] when SCALED, else 1)
+// E[img][t][m] = At[img][m] - s_img * C[m][img*T + t] (s_img = inv_scale[img] when SCALED, else 1)
 // for a tile whose first column is n0 (TN columns); this wave holds NI x NJ accumulator blocks
 // starting at row r0 and column c0.  Also the per-image minimum of E over rotations with a positive
 // prior into emin_key (ordered-int keys): LDS atomics per image slot, then one global atomic per
 // image the tile touches.  At rows and the valid flags are padded to Mpad (16-byte aligned rows).
+// Rotation-contiguous E rows make each 32-row accumulator block one whole 128-byte line per column,
+// written by one wave with 16-byte stores (an unaligned line would be read back from HBM to merge).
 // Called by all threads of the block (contains barriers).
 template <bool SCALED, int NI, int NJ, int TN>
 DEV void gemm_epilogue(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
@@ -57,7 +60,8 @@ DEV void gemm_epilogue(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int la
         const int img = col / T, t = col - img * T;
         const float s = SCALED ? inv_scale[img] : 1.f;
         const float* Ar = At + (size_t)img * Mpad;
-        float* Er = E + (size_t)img * Mtot * T + t;
+        float* Er = E + ((size_t)img * T + t) * Mtot;
+        const bool vec = (Mtot & 3) == 0;
         float emin = INFINITY;
 #pragma unroll
         for (int i = 0; i < NI; ++i)
@@ -68,19 +72,107 @@ DEV void gemm_epilogue(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int la
                 const float4 at = *reinterpret_cast<const float4*>(Ar + base);
                 const uint32_t v4 = *reinterpret_cast<const uint32_t*>(valid + base);
                 const float atv[4] = {at.x, at.y, at.z, at.w};
+                float ev[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int row = base + u;
-                    if (row < Mtot) {
-                        const float c = SCALED ? acc[i][j][4 * q + u] * s : acc[i][j][4 * q + u];
-                        const float e = atv[u] - c;
-                        Er[(size_t)row * T] = e;
-                        if ((v4 >> (8 * u)) & 0xff) emin = fminf(emin, e);
-                    }
+                    const float c = SCALED ? acc[i][j][4 * q + u] * s : acc[i][j][4 * q + u];
+                    ev[u] = atv[u] - c;
+                    if (base + u < Mtot && ((v4 >> (8 * u)) & 0xff)) emin = fminf(emin, ev[u]);
+                }
+                if (vec && base + 3 < Mtot) {
+                    *reinterpret_cast<float4*>(Er + base) = make_float4(ev[0], ev[1], ev[2], ev[3]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (base + u < Mtot) Er[base + u] = ev[u];
                 }
             }
         if (emin < INFINITY) atomicMin(&slot_min[img - img0], float_order_key(emin));
     }
+    __syncthreads();
+    for (int q = tid; q < nslot; q += blockDim.x)
+        if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img0 + q], slot_min[q]);
+}
+
+// Same results as gemm_epilogue, with the E rows staged through LDS: for each 32-row accumulator block
+// the wave writes its values column-major into `stage` (NJ*32 columns x 36 floats, this wave's own
+// region), then eight lanes store one column's 32 rows, one whole 128-byte line per store instruction,
+// as non-temporal (streaming) stores: complete lines that neither merge in L2 nor evict the operand
+// panels there.  All threads of the block call it (barriers).
+template <bool SCALED, int NI, int NJ, int TN>
+DEV void gemm_epilogue_staged(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
+                              const float* __restrict__ At, const float* __restrict__ inv_scale,
+                              float* __restrict__ E, const unsigned char* __restrict__ valid,
+                              int* __restrict__ emin_key, int Mtot, int Mpad, int Ncols, int T, int* slot_min,
+                              float* stage) {
+    constexpr int LD = 36;  // column stride in floats (16-byte aligned)
+    const int img0 = n0 / T;
+    const int last = min(n0 + TN, Ncols) - 1;
+    const int nslot = last >= n0 ? last / T - img0 + 1 : 0;
+    for (int q = tid; q < nslot; q += blockDim.x) slot_min[q] = INT_MAX;
+    const bool vec = (Mtot & 3) == 0;
+    const int h = lane >> 5;
+    int img_j[NJ];
+    float s_j[NJ], emin_j[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int col = min(c0 + j * 32 + (lane & 31), Ncols - 1);  // clamped: outputs of padding columns are dropped
+        img_j[j] = col / T;
+        s_j[j] = SCALED ? inv_scale[img_j[j]] : 1.f;
+        emin_j[j] = INFINITY;
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        // registers -> LDS (column-major), running minimum of this lane's column
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const float* Ar = At + (size_t)img_j[j] * Mpad;
+            const bool colok = c0 + j * 32 + (lane & 31) < Ncols;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int base = r0 + i * 32 + 8 * q + 4 * h;
+                const float4 at = *reinterpret_cast<const float4*>(Ar + base);
+                const uint32_t v4 = *reinterpret_cast<const uint32_t*>(valid + base);
+                const float atv[4] = {at.x, at.y, at.z, at.w};
+                float ev[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float c = SCALED ? acc[i][j][4 * q + u] * s_j[j] : acc[i][j][4 * q + u];
+                    ev[u] = atv[u] - c;
+                    if (colok && base + u < Mtot && ((v4 >> (8 * u)) & 0xff)) emin_j[j] = fminf(emin_j[j], ev[u]);
+                }
+                *reinterpret_cast<float4*>(&stage[(j * 32 + (lane & 31)) * LD + 8 * q + 4 * h]) =
+                    make_float4(ev[0], ev[1], ev[2], ev[3]);
+            }
+        }
+        __syncthreads();
+        // LDS -> E: lane group of 8 = one column's 32 rows = one 128-byte line
+        const int row0 = r0 + i * 32 + 4 * (lane & 7);
+#pragma unroll
+        for (int p = 0; p < NJ * 4; ++p) {
+            const int c = p * 8 + (lane >> 3);
+            const int col = c0 + c;
+            if (col < Ncols) {
+                const int img = col / T, t = col - img * T;
+                const float4 v = *reinterpret_cast<const float4*>(&stage[c * LD + 4 * (lane & 7)]);
+                float* Er = E + ((size_t)img * T + t) * Mtot + row0;
+                if (vec && row0 + 3 < Mtot) {
+                    const f4v w = {v.x, v.y, v.z, v.w};
+                    __builtin_nontemporal_store(w, reinterpret_cast<f4v*>(Er));
+                } else {
+                    const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (row0 + u < Mtot) Er[u] = vv[u];
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+        if (c0 + j * 32 + (lane & 31) < Ncols && emin_j[j] < INFINITY)
+            atomicMin(&slot_min[img_j[j] - img0], float_order_key(emin_j[j]));
     __syncthreads();
     for (int q = tid; q < nslot; q += blockDim.x)
         if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img0 + q], slot_min[q]);

@@ -116,8 +116,11 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
 #undef LOWP_LOAD
 #undef LOWP_STORE
 #undef LOWP_COMPUTE
-    gemm_epilogue<true, 4, 2, BN>(acc, m0 + wm * 128, n0 + wn * 64, n0, lane, tid, a.At, a.inv_scale, a.E, a.valid,
-                                  a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
+    // the operand stages are free now: 8 waves x 64 columns x 36 floats of staging = the A stages
+    float* stage = reinterpret_cast<float*>(&As[0][0][0]) + wave * (64 * 36);
+    static_assert(sizeof(As) >= 8 * 64 * 36 * sizeof(float), "epilogue staging does not fit");
+    gemm_epilogue_staged<true, 4, 2, BN>(acc, m0 + wm * 128, n0 + wn * 64, n0, lane, tid, a.At, a.inv_scale, a.E,
+                                         a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min, stage);
 }
 
 // sF: power of two that brings max |FcT| into [0.5, 1) (exact rescaling)

@@ -97,7 +97,7 @@ struct PrepLowpArgs {
 
 struct WeightParams {
     int K, D, PSI, R, Mtot, T;
-    unsigned long long magicT;  // ceil(2^40 / T): e / T == (e * magicT) >> 40 for e < 2^40 / T^2
+    unsigned long long magicM;  // ceil(2^48 / Mtot): e / Mtot == (e * magicM) >> 48 (e < Mtot * T, Mtot^2 T < 2^48)
     const double* pdf_m;     // [Mtot] pdf_direction of the rotation's (class, direction); 0 if skipped
     const double* lpdf_m;    // [Mtot] log(pdf_m) or -inf
     double lpdf_max;         // max over rotations of lpdf_m

@@ -720,7 +720,7 @@ __global__ void k_fill_d2(const int* __restrict__ blk_task, const int* __restric
     if (b >= nblocks) return;
     const int q = blk_task[b];
     const int img = task_img[q];
-    D2[b] = (double)E[((size_t)img * M + task_m[q]) * T + blk_t[b]] + C2c[img];
+    D2[b] = (double)E[((size_t)img * T + blk_t[b]) * M + task_m[q]] + C2c[img];
 }
 
 // argmax entry of the fine list -> (rotation m, translation t)

@@ -69,8 +69,10 @@ struct WShared {
 // Pre(e0, bound, lpoff_f) -> bit mask over the Pre::VW entries e0 .. e0 + VW - 1 (those < count): a
 // cheap FP32 test whose bit is clear only for entries whose exact FP64 du = u - u0 is >= kWin when
 // bound = (float)(u0 + kWin) + margin (the margin covers the FP32 rounding).
-template <typename Get, typename Pre>
-__device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, Pre pre,
+// Key(e, m, t): the entry's position in the reference's loop order; the max-posterior entry is the
+// first maximum in that order (ties between equal weights).
+template <typename Get, typename Pre, typename Key>
+__device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, Pre pre, Key key,
                             WeightOut wo, WShared& sh, uint32_t* sigmask, int TW, Cand cand, const int* emin_key) {
     constexpr int NT = 1024;
     const int tid = threadIdx.x;
@@ -156,9 +158,10 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
             if (!(du < kWin)) return;
             const double w = w_of(d, m, t, dmin);
             sum += w;
-            if (w > mx) {  // each thread visits increasing e: keeps this thread's first maximum
+            const long long ke = key(e, m, t);
+            if (w > mx || (w == mx && ke < am)) {
                 mx = w;
-                am = (long long)e;
+                am = ke;
             }
             if (w > 0.) {
                 const int b = du > 0. ? (int)(du * kBinScale) : 0;
@@ -358,38 +361,46 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
     });
 }
 
+// Coarse entries e = t * Mtot + m (E rows are rotation-contiguous).
+DEV void coarse_mt(size_t e, unsigned long long magic, int Mtot, int& m, int& t) {
+    t = (int)(((unsigned long long)e * magic) >> 48);
+    m = (int)e - t * Mtot;
+}
+
 // Coarse pre-test: d - lpdf_m - lpoff in FP32 from the per-rotation row {lpdf_m or -inf, k*T}, four
-// consecutive entries per call (one 16-byte load of E when the image's row is 16-byte aligned).
+// consecutive entries per call (one 16-byte load of E and one translation when Mtot % 4 == 0).
 struct CoarsePre {
     static constexpr int VW = 4;
     const float* Ei;
     size_t count;
     unsigned long long magic;
-    int T;
+    int Mtot;
     const float2* mrow;
     bool vec;
     DEV unsigned operator()(size_t e0, float bound, const float* lpoff_f) const {
         if (e0 >= count) return 0u;
-        float v[4];
+        unsigned msk = 0u;
         if (vec) {
-            const float4 q = *reinterpret_cast<const float4*>(Ei + e0);
-            v[0] = q.x;
-            v[1] = q.y;
-            v[2] = q.z;
-            v[3] = q.w;
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v q = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(Ei + e0));  // read once
+            const float v[4] = {q[0], q[1], q[2], q[3]};
+            int m0, t;
+            coarse_mt(e0, magic, Mtot, m0, t);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float2 mr = mrow[m0 + u];
+                if (v[u] - mr.x - lpoff_f[__float_as_int(mr.y) + t] < bound) msk |= 1u << u;
+            }
         } else {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = e0 + u < count ? Ei[e0 + u] : 0.f;
-        }
-        unsigned msk = 0u;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const size_t e = e0 + u;
-            if (e >= count) break;
-            const int m = (int)(((unsigned long long)e * magic) >> 40);
-            const int t = (int)e - m * T;
-            const float2 mr = mrow[m];
-            if (v[u] - mr.x - lpoff_f[__float_as_int(mr.y) + t] < bound) msk |= 1u << u;
+            for (int u = 0; u < 4; ++u) {
+                const size_t e = e0 + u;
+                if (e >= count) break;
+                int m, t;
+                coarse_mt(e, magic, Mtot, m, t);
+                const float2 mr = mrow[m];
+                if (Ei[e] - mr.x - lpoff_f[__float_as_int(mr.y) + t] < bound) msk |= 1u << u;
+            }
         }
         return msk;
     }
@@ -409,17 +420,18 @@ __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const 
     __shared__ WShared sh;
     const int img = blockIdx.x;
     const int T = wp.T;
-    const unsigned long long magic = wp.magicT;
-    const float* Ei = E + (size_t)img * wp.Mtot * T;
+    const unsigned long long magic = wp.magicM;
+    const int Mtot = wp.Mtot;
+    const float* Ei = E + (size_t)img * Mtot * T;
     auto get = [&](size_t e, double& d, int& m, int& t) -> bool {
-        m = (int)(((unsigned long long)e * magic) >> 40);
-        t = (int)e - m * T;
+        coarse_mt(e, magic, Mtot, m, t);
         if (!(wp.pdf_m[m] > 0.)) return false;
         d = (double)Ei[e];
         return true;
     };
-    const CoarsePre pre{Ei, (size_t)wp.Mtot * T, magic, T, wp.mrow, ((size_t)wp.Mtot * T) % 4 == 0};
-    weight_pass(img, (size_t)wp.Mtot * T, C2c[img], wp, get, pre, wo, sh, sigmask, TW, cand, emin_key);
+    const CoarsePre pre{Ei, (size_t)Mtot * T, magic, Mtot, wp.mrow, (Mtot & 3) == 0};
+    auto key = [&](size_t, int m, int t) { return (long long)m * T + t; };  // (class, dir, psi, t) order
+    weight_pass(img, (size_t)Mtot * T, C2c[img], wp, get, pre, key, wo, sh, sigmask, TW, cand, emin_key);
 }
 
 __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
@@ -440,7 +452,8 @@ __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const do
         return true;
     };
     const AllPre pre{count};
-    weight_pass(img, count, 0.0, wp, get, pre, wo, sh, nullptr, 1, cand, nullptr);
+    auto key = [](size_t e, int, int) { return (long long)e; };
+    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, nullptr, 1, cand, nullptr);
 }
 
 }  // namespace
