@@ -172,6 +172,37 @@ int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, in
  * rotations x classes, translations. */
 int rome_get_batch_stats(rome_ctx* ctx, long long* out8);
 
+/* ---- M-step: model parameters other than the maps ----
+ * maximizationOtherParameters (src/map3d_optimizer.cpp:2620-2798; 2D src/map2d_optimizer.cpp:1985-2045)
+ * for iterations without CC mode and without scale correction: from the E-step sums to
+ *   pdf_class = wsum_pdf_class / sum_w, pdf_direction = wsum_pdf_direction / sum_w,
+ *   prior offsets (2D) = wsum_prior_offset / wsum_pdf_class (0 for an empty class),
+ *   sigma2_offset = wsum_sigma2_offset / (2 sum_w),
+ *   sigma2_noise[g][n] = wsum_sigma2_noise[g][n] / (2 wsum_sumw_group[g] Npix_per_shell[n])
+ *     (Npix_per_shell of MAPModel::updateImageSizeAndResolutionPointers, src/map_model.cpp:522-538),
+ *   avg_norm_correction = wsum_avg_norm_correction / sum_w (when do_norm_correction),
+ *   LL = wsum_LL, ave_Pmax = wsum_ave_Pmax / sum_w,
+ * with sum_w = sum_k wsum_pdf_class[k].  Host FP64 arithmetic in the reference's order. */
+typedef struct {
+    double* sigma2_noise;   /* [G][ori/2+1] */
+    double* pdf_class;      /* [K] */
+    double* pdf_direction;  /* [K][nr_dir] */
+    double* prior_offx;     /* [K] */
+    double* prior_offy;     /* [K] */
+    double* scalars;        /* [4] sigma2_offset, avg_norm_correction, LL, ave_Pmax */
+} rome_model;
+
+/* Context-free form: sums in (rome_sums layout, const), model out.  avg_norm_in and the prior
+ * offsets in are kept where the reference keeps them (3D priors, do_norm_correction = 0). */
+int rome_maximization_other(int ref_dim, int ori_size, int nr_classes, int nr_groups, int nr_dir,
+                            int do_norm_correction, double avg_norm_in, const double* prior_offx_in,
+                            const double* prior_offy_in, const rome_sums* sums, rome_model* out);
+
+/* The same from the context's own sums and model; install != 0 makes the result the model of the
+ * next rome_expectation (as rome_set_model would, plus sigma2_offset, avg_norm_correction and, with
+ * CTF correction, refs_are_ctf_corrected = 1). */
+int rome_update_model(rome_ctx* ctx, int install, rome_model* out);
+
 /* ---- host sampling helpers (HealpixSampler, C1) ---- */
 int rome_sampling_pose3d(int healpix_order, int oversampling, double perturbation, double* out);
 int rome_sampling_pose2d(double psi_step, int oversampling, double perturbation, double* out);

@@ -25,7 +25,7 @@ EXPORTED = (
     "rome_get_backprojector", "rome_get_coarse_diff2", "rome_get_fine_count", "rome_get_fine_list", "rome_project",
     "rome_sampling_pose3d", "rome_sampling_pose2d", "rome_sampling_translations", "rome_sampling_shifts",
     "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
-    "rome_get_batch_stats",
+    "rome_get_batch_stats", "rome_maximization_other", "rome_update_model",
 )
 
 STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
@@ -68,6 +68,33 @@ class rome_results(C.Structure):
 class rome_sums(C.Structure):
     _fields_ = [(n, _dp) for n in ("wsum_sigma2_noise", "wsum_sumw_group", "wsum_pdf_class", "wsum_pdf_direction",
                                    "wsum_prior_offx", "wsum_prior_offy", "wsum_scalars")]
+
+
+class rome_model(C.Structure):
+    _fields_ = [(n, _dp) for n in ("sigma2_noise", "pdf_class", "pdf_direction", "prior_offx", "prior_offy",
+                                   "scalars")]
+
+
+def _model_arrays(K, G, D, F):
+    return {"sigma2_noise": np.zeros((G, F)), "pdf_class": np.zeros(K), "pdf_direction": np.zeros((K, D)),
+            "prior_offx": np.zeros(K), "prior_offy": np.zeros(K), "scalars": np.zeros(4)}
+
+
+def maximization_other(ref_dim, ori_size, sums, nr_dir, do_norm_correction, avg_norm_in, prior_offx, prior_offy):
+    """rome_maximization_other (host FP64): the model update of maximizationOtherParameters from E-step
+    sums (same dict layout as Engine.get_sums).  scalars = sigma2_offset, avg_norm_correction, LL, ave_Pmax."""
+    K = len(sums["wsum_pdf_class"])
+    G, F = np.asarray(sums["wsum_sigma2_noise"]).shape
+    keep = {k: _f64(v) for k, v in sums.items()}
+    cs = rome_sums(**{k: _p(v) for k, v in keep.items()})
+    out = _model_arrays(K, G, nr_dir, F)
+    cm = rome_model(**{k: _p(v) for k, v in out.items()})
+    px, py = _f64(prior_offx), _f64(prior_offy)
+    rc = lib().rome_maximization_other(ref_dim, ori_size, K, G, nr_dir, int(do_norm_correction),
+                                       C.c_double(avg_norm_in), _p(px), _p(py), C.byref(cs), C.byref(cm))
+    if rc != 0:
+        raise RomeError(f"rome_maximization_other failed ({rc})")
+    return out
 
 
 def lib():
@@ -241,6 +268,13 @@ class Engine:
         cs = rome_sums(**{k: _p(v) for k, v in s.items()})
         self._check(lib().rome_get_sums(self._h, C.byref(cs)), "rome_get_sums")
         return s
+
+    def update_model(self, install=True):
+        """rome_update_model: M-step model parameters from this context's sums (optionally installed)."""
+        out = _model_arrays(self.K, self.G, self.D, self.oriF)
+        cm = rome_model(**{k: _p(v) for k, v in out.items()})
+        self._check(lib().rome_update_model(self._h, int(install), C.byref(cm)), "rome_update_model")
+        return out
 
     def get_backprojector(self, iclass):
         d = np.zeros(2 * self.grid_n)

@@ -1263,3 +1263,39 @@ int rome_healpix_pix2ang(int order, int pix, double* z, double* phi) {
 }
 
 }  // extern "C"
+
+int rome_update_model(rome_ctx* ctx, int install, rome_model* out) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (!ctx->configured || !ctx->have_model) return fail(ctx, ROME_ERR_ARG, "no configured model to update");
+    const rome_config& c = ctx->cfg;
+    const int K = ctx->K, D = ctx->D, G = c.nr_groups, F = ctx->oriF;
+    std::vector<double> ws((size_t)G * F), wg(G), wpc(K), wpd((size_t)K * D), wpx(K), wpy(K), wsc(4);
+    rome_sums s{ws.data(), wg.data(), wpc.data(), wpd.data(), wpx.data(), wpy.data(), wsc.data()};
+    int rc = rome_get_sums(ctx, &s);
+    if (rc) return rc;
+    std::vector<double> sig((size_t)G * F), pc(K), pd((size_t)K * D), px(K), py(K), sc(4);
+    rome_model m{sig.data(), pc.data(), pd.data(), px.data(), py.data(), sc.data()};
+    rc = rome_maximization_other(c.ref_dim, c.ori_size, K, G, D, c.do_norm_correction, c.avg_norm_correction,
+                                 ctx->prior_x_h.data(), ctx->prior_y_h.data(), &s, &m);
+    if (rc == ROME_ERR_NUMERIC) return fail(ctx, rc, "sum of the class weights is not positive");
+    if (rc) return fail(ctx, rc, "rome_maximization_other failed");
+    if (out) {
+        auto put = [](double* dst, const std::vector<double>& v) {
+            if (dst) std::copy(v.begin(), v.end(), dst);
+        };
+        put(out->sigma2_noise, sig);
+        put(out->pdf_class, pc);
+        put(out->pdf_direction, pd);
+        put(out->prior_offx, px);
+        put(out->prior_offy, py);
+        put(out->scalars, sc);
+    }
+    if (install) {
+        rc = rome_set_model(ctx, sig.data(), pc.data(), pd.data(), px.data(), py.data());
+        if (rc) return rc;
+        ctx->cfg.sigma2_offset = sc[0];
+        ctx->cfg.avg_norm_correction = sc[1];
+        if (c.do_ctf_correction) ctx->cfg.refs_are_ctf_corrected = 1;  // after the first iteration
+    }
+    return ROME_OK;
+}

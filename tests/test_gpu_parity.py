@@ -147,3 +147,18 @@ def _check_sums(r, s, ref, prob, engine, name):
             continue
         assert np.linalg.norm(dg - od) <= 1e-5 * np.linalg.norm(od), name
         assert np.linalg.norm(wg - ow) <= 1e-5 * np.linalg.norm(ow), name
+
+
+def test_update_model_matches_oracle(engine, case):
+    """M-step model parameters from the engine's own sums vs the oracle's update of the oracle's sums."""
+    import oracle.mstep as OM
+    name, prob, batch, ref = case
+    _run(engine, prob, batch)
+    got = engine.update_model(install=False)
+    ref_sums = {k: ref[k] for k in ("wsum_pdf_class", "wsum_pdf_direction", "wsum_sigma2_noise", "wsum_sumw_group",
+                                    "wsum_prior_offx", "wsum_prior_offy", "wsum_scalars")}
+    want = OM.maximization_other(3 if prob["is_3d"] else 2, prob["ori_size"], ref_sums, prob["nr_dir"],
+                                 prob["do_norm_correction"], prob["avg_norm_correction"], prob["prior_offx"],
+                                 prob["prior_offy"])
+    for k in want:
+        np.testing.assert_allclose(got[k], np.reshape(want[k], got[k].shape), rtol=1e-4, atol=1e-12, err_msg=k)
