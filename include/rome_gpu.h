@@ -130,6 +130,37 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
  * [pad][pad/2+1] (2D), pad = 2*(2*min(cur/2,ori/2)+1)+1. */
 int rome_set_reference(rome_ctx* ctx, int iclass, const double* grid);
 
+/* MyProjector::computeFourierTransformMap (src/reconstruct.cpp:106-223) with griddingCorrect
+ * (src/reconstruct.cpp:61-103) on the device, the way MAPModel::setFourierTransformMaps
+ * (src/map_model.cpp:136-160) calls it: vol = the class map, [ori][ori][ori] (3D) or [ori][ori] (2D)
+ * FP64; padding factor 2, trilinear, r_max = min(current_size, ori_size)/2 of rome_configure.  FFT in
+ * FP64 (hipFFT D2Z).  The grid becomes reference iclass (what rome_set_reference would install);
+ * power_spectrum [ori/2+1] receives the radial power (tau2 input), corrected_map the gridding-corrected
+ * map (the reference corrects its input in place) and grid_out the [pad][pad][pad/2+1] (3D) or
+ * [pad][pad/2+1] (2D) complex FP64 grid, interleaved; any of the three may be NULL. */
+int rome_fourier_map(rome_ctx* ctx, int iclass, const double* vol, int do_gridding, double* power_spectrum,
+                     double* corrected_map, double* grid_out);
+
+/* MyBackProjector::reconstruct (src/reconstruct.cpp:1929-2522) as MAPModel::reconstruction
+ * (src/map_model.cpp:233-239) calls it from maximization (src/map3d_optimizer.cpp:2507-2617): the class's
+ * backprojector grids -> the new map, on the device in FP64 (hipFFT).  Grids: the context's own
+ * accumulated backprojector of iclass when data_in and weight_in are NULL, else FP64 host grids in the
+ * layout of rome_get_backprojector.  C1 symmetry, trilinear, padding 2, blob order 0 / radius 1.9 /
+ * alpha 15 (MyBackProjector defaults, src/reconstruct.h:650-655).  tau2 [ori/2+1] is read (and
+ * rewritten when update_tau2_with_fsc); fsc [ori/2+1] is read only with update_tau2_with_fsc; sigma2 and
+ * data_vs_prior [ori/2+1] are written (data_vs_prior stays 0 without do_map and FSC update); vol_out is
+ * [ori][ori][ori] (3D) or [ori][ori] (2D). */
+typedef struct {
+    int max_iter_preweight;     /* gridding iterations (rome_map3d: 10) */
+    int do_map, minres_map;     /* MAPModel default minres_map 5 */
+    int update_tau2_with_fsc, is_whole_instead_of_half;
+    double tau2_fudge, normalise;  /* maximization passes normalise = wsum_pdf_class[iclass] */
+} rome_recon_params;
+
+int rome_reconstruct(rome_ctx* ctx, int iclass, const rome_recon_params* params, const double* data_in,
+                     const double* weight_in, double* tau2, const double* fsc, double* sigma2,
+                     double* data_vs_prior, double* vol_out);
+
 /* MLModel::resetZero of the E-step sums + zeroed backprojector grids (MAPModel). */
 int rome_reset_sums(rome_ctx* ctx);
 

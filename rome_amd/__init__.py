@@ -25,7 +25,8 @@ EXPORTED = (
     "rome_get_backprojector", "rome_get_coarse_diff2", "rome_get_fine_count", "rome_get_fine_list", "rome_project",
     "rome_sampling_pose3d", "rome_sampling_pose2d", "rome_sampling_translations", "rome_sampling_shifts",
     "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
-    "rome_get_batch_stats", "rome_maximization_other", "rome_update_model",
+    "rome_get_batch_stats", "rome_maximization_other", "rome_update_model", "rome_fourier_map",
+    "rome_reconstruct",
 )
 
 STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
@@ -95,6 +96,12 @@ def maximization_other(ref_dim, ori_size, sums, nr_dir, do_norm_correction, avg_
     if rc != 0:
         raise RomeError(f"rome_maximization_other failed ({rc})")
     return out
+
+
+class rome_recon_params(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("max_iter_preweight", "do_map", "minres_map", "update_tau2_with_fsc",
+                                        "is_whole_instead_of_half")] + \
+               [(n, C.c_double) for n in ("tau2_fudge", "normalise")]
 
 
 def lib():
@@ -268,6 +275,43 @@ class Engine:
         cs = rome_sums(**{k: _p(v) for k, v in s.items()})
         self._check(lib().rome_get_sums(self._h, C.byref(cs)), "rome_get_sums")
         return s
+
+    def fourier_map(self, iclass, vol, do_gridding=True, want_grid=True):
+        """rome_fourier_map: projector grid of a real map, installed as reference iclass.
+        Returns (power_spectrum, corrected_map, grid or None); grid is complex128 [pad][pad][pad/2+1]
+        (3D) or [pad][pad/2+1] (2D)."""
+        v = _f64(vol)
+        ps = np.zeros(self.oriF)
+        corr = np.zeros_like(v)
+        grid = np.zeros(2 * self.grid_n) if want_grid else None
+        self._check(lib().rome_fourier_map(self._h, iclass, _p(v), int(do_gridding), _p(ps), _p(corr), _p(grid)),
+                    "rome_fourier_map")
+        if grid is not None:
+            padF = self.pad // 2 + 1
+            shape = (self.pad, self.pad, padF) if self.is3d else (self.pad, padF)
+            grid = (grid[0::2] + 1j * grid[1::2]).reshape(shape)
+        return ps, corr, grid
+
+    def reconstruct(self, iclass, tau2, data=None, weight=None, fsc=None, max_iter_preweight=10, do_map=True,
+                    minres_map=5, update_tau2_with_fsc=False, is_whole_instead_of_half=False, tau2_fudge=1.0,
+                    normalise=1.0):
+        """rome_reconstruct: the class map from the backprojector grids (the context's own, or the given
+        complex data / real weight grids).  Returns (vol, tau2, sigma2, data_vs_prior)."""
+        prm = rome_recon_params(max_iter_preweight, int(do_map), minres_map, int(update_tau2_with_fsc),
+                                int(is_whole_instead_of_half), tau2_fudge, normalise)
+        t2 = _f64(tau2).copy()
+        sig = np.zeros(self.oriF)
+        dvp = np.zeros(self.oriF)
+        ori = self.cfg["ori_size"]
+        vol = np.zeros((ori, ori, ori) if self.is3d else (ori, ori))
+        d = w = None
+        if data is not None:
+            d = np.ascontiguousarray(np.stack([np.real(data), np.imag(data)], axis=-1), dtype=np.float64)
+            w = _f64(weight)
+        f = None if fsc is None else _f64(fsc)
+        self._check(lib().rome_reconstruct(self._h, iclass, C.byref(prm), _p(d), _p(w), _p(t2), _p(f), _p(sig),
+                                           _p(dvp), _p(vol)), "rome_reconstruct")
+        return vol, t2, sig, dvp
 
     def update_model(self, install=True):
         """rome_update_model: M-step model parameters from this context's sums (optionally installed)."""

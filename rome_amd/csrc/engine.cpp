@@ -4,6 +4,7 @@
 //   fine pass    : getAllSquaredDifferencesFine -> convert -> significance
 //   updateOtherParams + backProjection (one fused kernel), storeWeightedSums.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -128,6 +129,20 @@ struct rome_ctx {
     DevBuf<double> slice_w;
     DevBuf<unsigned> bp_count, bp_off, bp_cursor;
     DevBuf<unsigned long long> bp_rec;
+    // hipFFT plans by (type, rank, size), kept for the context's lifetime
+    struct FftPlan {
+        int type, rank, n;
+        hipfftHandle h;
+    };
+    std::vector<FftPlan> fft_plans;
+    // projector grids built on the device (rome_fourier_map)
+    DevBuf<double> fm_vol, fm_Mc, fm_corr, fm_pw, fm_cnt;
+    DevBuf<double2> fm_F, fm_grid64;
+    // M-step reconstruction buffers (rome_reconstruct)
+    DevBuf<double2> rc_data, rc_Fc, rc_Ft;
+    DevBuf<double> rc_weight, rc_Fw, rc_Fnw, rc_M, rc_Mout, rc_vol, rc_tab, rc_sums, rc_tau2;
+    DevBuf<int> rc_bad;
+    std::vector<double> rc_tab_h;
     // stage timing with HIP events on the engine stream (rome_set_profiling)
     static constexpr int kStages = 11;
     bool profiling = false;
@@ -758,6 +773,7 @@ void rome_destroy(rome_ctx* ctx) {
     for (int i = 0; i < rome_ctx::kStages; ++i)
         for (int j = 0; j < 2; ++j)
             if (ctx->ev[i][j]) (void)hipEventDestroy(ctx->ev[i][j]);
+    for (auto& p : ctx->fft_plans) (void)hipfftDestroy(p.h);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1297,5 +1313,239 @@ int rome_update_model(rome_ctx* ctx, int install, rome_model* out) {
         ctx->cfg.avg_norm_correction = sc[1];
         if (c.do_ctf_correction) ctx->cfg.refs_are_ctf_corrected = 1;  // after the first iteration
     }
+    return ROME_OK;
+}
+
+// cached hipFFT plan (type D2Z / Z2D, rank 2 or 3, n per axis) bound to the context's stream
+static int fft_plan(rome_ctx* ctx, hipfftType type, int rank, int n, hipfftHandle* out) {
+    for (auto& p : ctx->fft_plans)
+        if (p.type == (int)type && p.rank == rank && p.n == n) {
+            *out = p.h;
+            return ROME_OK;
+        }
+    hipfftHandle h = 0;
+    const hipfftResult r = rank == 3 ? hipfftPlan3d(&h, n, n, n, type) : hipfftPlan2d(&h, n, n, type);
+    if (r != HIPFFT_SUCCESS) return fail(ctx, ROME_ERR_HIP, "hipfftPlan (type %d, %d^%d) failed: %d", (int)type, n, rank, (int)r);
+    if (hipfftSetStream(h, ctx->stream) != HIPFFT_SUCCESS) {
+        (void)hipfftDestroy(h);
+        return fail(ctx, ROME_ERR_HIP, "hipfftSetStream failed");
+    }
+    ctx->fft_plans.push_back({(int)type, rank, n, h});
+    *out = h;
+    return ROME_OK;
+}
+
+int rome_fourier_map(rome_ctx* ctx, int iclass, const double* vol, int do_gridding, double* power_spectrum,
+                     double* corrected_map, double* grid_out) {
+    if (!ctx || !vol) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int pf = 2, ori = ctx->cfg.ori_size, padori = pf * ori, dim3 = ctx->is3d ? 1 : 0;
+    const size_t nvol = dim3 ? (size_t)ori * ori * ori : (size_t)ori * ori;
+    const size_t npad = dim3 ? (size_t)padori * padori * padori : (size_t)padori * padori;
+    const size_t nF = (dim3 ? (size_t)padori : 1) * padori * (padori / 2 + 1);
+    const int oriF = ctx->oriF;
+    hipfftHandle plan;
+    int rc = fft_plan(ctx, HIPFFT_D2Z, dim3 ? 3 : 2, padori, &plan);
+    if (rc) return rc;
+    HIPCHK(ctx->fm_vol.upload(vol, nvol, s));
+    HIPCHK(ctx->fm_Mc.reserve(npad));
+    HIPCHK(hipMemsetAsync(ctx->fm_Mc.p, 0, npad * sizeof(double), s));
+    if (corrected_map) HIPCHK(ctx->fm_corr.reserve(nvol));
+    HIPCHK(launch_fm_pad(s, ctx->fm_vol.p, ori, padori, dim3, do_gridding ? 1 : 0, pf, ctx->fm_Mc.p,
+                         corrected_map ? ctx->fm_corr.p : nullptr));
+    HIPCHK(ctx->fm_F.reserve(nF));
+    const hipfftResult fr = hipfftExecD2Z(plan, ctx->fm_Mc.p, reinterpret_cast<hipfftDoubleComplex*>(ctx->fm_F.p));
+    if (fr != HIPFFT_SUCCESS) return fail(ctx, ROME_ERR_HIP, "hipfftExecD2Z failed: %d", (int)fr);
+    HIPCHK(ctx->fm_grid64.reserve(ctx->grid_n));
+    HIPCHK(hipMemsetAsync(ctx->fm_grid64.p, 0, ctx->grid_n * sizeof(double2), s));
+    float2* g32 = ctx->grids.p + ctx->grid_n * iclass;
+    HIPCHK(hipMemsetAsync(g32, 0, ctx->grid_n * sizeof(float2), s));
+    HIPCHK(ctx->fm_pw.reserve(oriF));
+    HIPCHK(ctx->fm_cnt.reserve(oriF));
+    HIPCHK(hipMemsetAsync(ctx->fm_pw.p, 0, oriF * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(ctx->fm_cnt.p, 0, oriF * sizeof(double), s));
+    const double normfft = dim3 ? (double)(pf * pf * pf * ori) : (double)(pf * pf);
+    HIPCHK(launch_fm_window(s, ctx->fm_F.p, padori, dim3, pf, ctx->r_max, ctx->pad, oriF, normfft, ctx->fm_grid64.p,
+                            g32, ctx->fm_pw.p, ctx->fm_cnt.p));
+    std::vector<double> pw(oriF), cnt(oriF);
+    HIPCHK(hipMemcpyAsync(pw.data(), ctx->fm_pw.p, oriF * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->fm_cnt.p, oriF * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (corrected_map)
+        HIPCHK(hipMemcpyAsync(corrected_map, ctx->fm_corr.p, nvol * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (grid_out)
+        HIPCHK(hipMemcpyAsync(grid_out, ctx->fm_grid64.p, ctx->grid_n * sizeof(double2), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (power_spectrum)  // radial average (src/reconstruct.cpp:213-221)
+        for (int i = 0; i < oriF; ++i) power_spectrum[i] = cnt[i] < 1. ? 0. : pw[i] / cnt[i];
+    ctx->have_ref[iclass] = 1;
+    ctx->fct_dirty = true;
+    return ROME_OK;
+}
+
+// Fourier transform of the order-0 Kaiser-Bessel blob, tabulated as TabFtBlob::init does
+// (src/reconstruct.h:615-627, kaiser_Fourier_value src/bessel.cpp:26-50): 10000 samples over [0, 1/2).
+// Only ratios to the value at 0 are used, so the constant factor (2 pi)^{3/2} a^3 / I0(alpha) only has
+// to be the same for every entry; I0 by its power series.
+static std::vector<double> kaiser_ft_table(double a, double alpha, int nr_elem, double* sampling) {
+    *sampling = 0.5 / (double)nr_elem;
+    double i0 = 1., term = 1.;
+    for (int k = 1; term > 1e-17 * i0; ++k) {
+        term *= (alpha / 2.) * (alpha / 2.) / ((double)k * k);
+        i0 += term;
+    }
+    const double pi = 3.14159265358979323846;
+    const double cst = std::pow(2. * pi, 1.5) * a * a * a;
+    std::vector<double> t(nr_elem);
+    for (int i = 0; i < nr_elem; ++i) {
+        const double w = (double)i * *sampling;
+        const double x = 2. * pi * a * w;
+        const double sig = std::sqrt(std::fabs(alpha * alpha - x * x));
+        double b;
+        if (x > alpha)  // spherical-order J_{3/2}
+            b = std::sqrt(2. / (pi * sig)) * (std::sin(sig) / sig - std::cos(sig));
+        else            // I_{3/2}
+            b = sig == 0. ? 0. : std::sqrt(2. / (pi * sig)) * (std::cosh(sig) - std::sinh(sig) / sig);
+        t[i] = cst * b / (i0 * std::pow(sig, 1.5));
+    }
+    return t;
+}
+
+int rome_reconstruct(rome_ctx* ctx, int iclass, const rome_recon_params* prm, const double* data_in,
+                     const double* weight_in, double* tau2, const double* fsc, double* sigma2,
+                     double* data_vs_prior, double* vol_out) {
+    if (!ctx || !prm || !tau2 || !sigma2 || !data_vs_prior || !vol_out) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
+    if ((data_in == nullptr) != (weight_in == nullptr)) return fail(ctx, ROME_ERR_ARG, "give both grids or neither");
+    if (prm->update_tau2_with_fsc && !fsc) return fail(ctx, ROME_ERR_ARG, "update_tau2_with_fsc needs fsc");
+    if (prm->max_iter_preweight < 0 || !(prm->normalise != 0.)) return fail(ctx, ROME_ERR_ARG, "bad parameters");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int pf = 2, ori = ctx->cfg.ori_size, oriF = ctx->oriF, r_max = ctx->r_max;
+    const int P = ctx->pad, PF = ctx->padF, Dz = ctx->is3d ? P : 1, rank = ctx->is3d ? 3 : 2;
+    const int N = pf * ori, NF = N / 2 + 1, NZ = ctx->is3d ? N : 1, OZ = ctx->is3d ? ori : 1;
+    const int max_r2 = r_max * r_max * pf * pf;
+    const size_t ng = (size_t)Dz * P * PF, nM = (size_t)Dz * P * P, nFt = (size_t)NZ * N * NF;
+    const size_t nMo = (size_t)NZ * N * N, nv = (size_t)OZ * ori * ori;
+    const double osc = ctx->is3d ? (double)(pf * pf * pf) : (double)(pf * pf);
+    hipfftHandle planPi, planPf, planNi;
+    int rc = fft_plan(ctx, HIPFFT_Z2D, rank, P, &planPi);
+    if (!rc) rc = fft_plan(ctx, HIPFFT_D2Z, rank, P, &planPf);
+    if (!rc) rc = fft_plan(ctx, HIPFFT_Z2D, rank, N, &planNi);
+    if (rc) return rc;
+    HIPCHK(ctx->rc_data.reserve(ng));
+    HIPCHK(ctx->rc_weight.reserve(ng));
+    HIPCHK(ctx->rc_Fw.reserve(ng));
+    HIPCHK(ctx->rc_Fnw.reserve(ng));
+    // rc_Fc and rc_M also take the transform of the output map (tau2 from the FSC)
+    HIPCHK(ctx->rc_Fc.reserve(std::max(ng, (size_t)OZ * ori * (ori / 2 + 1))));
+    HIPCHK(ctx->rc_M.reserve(std::max(nM, nv)));
+    HIPCHK(ctx->rc_Ft.reserve(nFt));
+    HIPCHK(ctx->rc_Mout.reserve(nMo));
+    HIPCHK(ctx->rc_vol.reserve(nv));
+    HIPCHK(ctx->rc_sums.reserve(4 * (size_t)std::max(oriF, ori)));
+    HIPCHK(ctx->rc_tau2.reserve(oriF));
+    HIPCHK(ctx->rc_bad.reserve(1));
+    if (data_in) {
+        HIPCHK(hipMemcpyAsync(ctx->rc_data.p, data_in, ng * sizeof(double2), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->rc_weight.p, weight_in, ng * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        HIPCHK(rc_from_float(s, ctx->bp_data.p ? reinterpret_cast<const float2*>(ctx->bp_data.p) + ctx->grid_n * iclass
+                                               : nullptr,
+                             ctx->bp_weight.p + ctx->grid_n * iclass, ctx->rc_data.p, ctx->rc_weight.p, ng));
+    }
+    // Hermitian fold of the x = 0 plane, then the weights in FFTW order
+    HIPCHK(rc_fold(s, ctx->rc_data.p, ctx->rc_weight.p, Dz, P, PF));
+    HIPCHK(rc_decenter_real(s, ctx->rc_weight.p, ctx->rc_Fw.p, Dz, P, PF, max_r2));
+    // radial average of the inverse noise power (:2019-2060)
+    double* s0 = ctx->rc_sums.p;
+    double* s1 = s0 + oriF;
+    HIPCHK(hipMemsetAsync(s0, 0, 2 * oriF * sizeof(double), s));
+    HIPCHK(rc_spectrum(s, ctx->rc_Fw.p, Dz, P, PF, max_r2, pf, osc, oriF, s0, s1));
+    std::vector<double> h0(oriF), h1(oriF);
+    HIPCHK(hipMemcpyAsync(h0.data(), s0, oriF * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h1.data(), s1, oriF * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < oriF; ++i) {
+        if (h0[i] > 1e-10) sigma2[i] = h1[i] / h0[i];
+        else if (h0[i] == 0.) sigma2[i] = 0.;
+        else return fail(ctx, ROME_ERR_NUMERIC, "unexpectedly small, yet non-zero sigma2 value (shell %d)", i);
+    }
+    if (prm->update_tau2_with_fsc) {  // tau2 from the FSC (:2062-2080)
+        for (int i = 0; i < oriF; ++i) {
+            double myfsc = std::max(0.001, fsc[i]);
+            if (prm->is_whole_instead_of_half) myfsc = std::sqrt(2. * myfsc / (myfsc + 1.));
+            myfsc = std::min(0.999, myfsc);
+            const double myssnr = myfsc / (1. - myfsc);
+            tau2[i] = myssnr * sigma2[i];
+            data_vs_prior[i] = myssnr;
+        }
+    }
+    if (prm->do_map) {
+        HIPCHK(hipMemcpyAsync(ctx->rc_tau2.p, tau2, oriF * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(s0, 0, 2 * oriF * sizeof(double), s));
+        HIPCHK(hipMemsetAsync(ctx->rc_bad.p, 0, sizeof(int), s));
+        HIPCHK(rc_map(s, ctx->rc_Fw.p, Dz, P, PF, max_r2, pf, osc, prm->tau2_fudge, ctx->rc_tau2.p, prm->minres_map,
+                      oriF, s0, s1, ctx->rc_bad.p));
+        int bad = 0;
+        HIPCHK(hipMemcpyAsync(h0.data(), s0, oriF * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h1.data(), s1, oriF * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&bad, ctx->rc_bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (bad) return fail(ctx, ROME_ERR_NUMERIC, "negative values in the tau2 spectrum");
+        if (!prm->update_tau2_with_fsc)
+            for (int i = 0; i < oriF; ++i)
+                data_vs_prior[i] = i > r_max ? 0. : (h1[i] < 0.001 ? 999. : h0[i] / h1[i]);
+    }
+    HIPCHK(rc_normalise(s, ctx->rc_Fw.p, ctx->rc_data.p, ng, prm->normalise));
+    // iterative gridding weights (Pipe & Menon), :2196-2274
+    if (ctx->rc_tab_h.empty()) {
+        double sampling = 0.;
+        ctx->rc_tab_h = kaiser_ft_table(1.9 * pf, 15.0, 10000, &sampling);
+        HIPCHK(ctx->rc_tab.upload(ctx->rc_tab_h.data(), ctx->rc_tab_h.size(), s));
+    }
+    const double sampling = 0.5 / 10000.;
+    HIPCHK(rc_newweight_init(s, ctx->rc_Fnw.p, Dz, P, PF, max_r2));
+    auto Z = [](double2* p) { return reinterpret_cast<hipfftDoubleComplex*>(p); };
+    for (int it = 0; it < prm->max_iter_preweight; ++it) {
+        HIPCHK(rc_fconv(s, ctx->rc_Fnw.p, ctx->rc_Fw.p, ctx->rc_Fc.p, ng));
+        if (hipfftExecZ2D(planPi, Z(ctx->rc_Fc.p), ctx->rc_M.p) != HIPFFT_SUCCESS)
+            return fail(ctx, ROME_ERR_HIP, "hipfftExecZ2D failed");
+        HIPCHK(rc_blob(s, ctx->rc_M.p, Dz, P, ori, pf, ctx->rc_tab.p, 10000, sampling, ctx->rc_tab_h[0]));
+        if (hipfftExecD2Z(planPf, ctx->rc_M.p, Z(ctx->rc_Fc.p)) != HIPFFT_SUCCESS)
+            return fail(ctx, ROME_ERR_HIP, "hipfftExecD2Z failed");
+        HIPCHK(rc_newweight(s, ctx->rc_Fc.p, ctx->rc_Fnw.p, Dz, P, PF, max_r2, (double)nM));
+    }
+    // weighted data -> window to 2 ori -> real space -> window to ori, soft mask, gridding correction
+    HIPCHK(rc_final_fconv(s, ctx->rc_data.p, ctx->rc_Fnw.p, ctx->rc_Fc.p, Dz, P, PF, max_r2));
+    HIPCHK(rc_window_ft(s, ctx->rc_Fc.p, ctx->rc_Ft.p, Dz, P, PF, NZ, N, NF));
+    if (hipfftExecZ2D(planNi, Z(ctx->rc_Ft.p), ctx->rc_Mout.p) != HIPFFT_SUCCESS)
+        return fail(ctx, ROME_ERR_HIP, "hipfftExecZ2D failed");
+    const double normfft = ctx->is3d ? (double)(pf * pf * pf * ori) : (double)(pf * pf);
+    HIPCHK(rc_center_window(s, ctx->rc_Mout.p, ctx->rc_vol.p, NZ, N, OZ, ori, normfft));
+    HIPCHK(hipMemsetAsync(s0, 0, 2 * sizeof(double), s));
+    HIPCHK(rc_bgsum(s, ctx->rc_vol.p, OZ, ori, (double)ori / 2., 3., s0));
+    HIPCHK(rc_mask_grid(s, ctx->rc_vol.p, OZ, ori, pf, (double)ori / 2., 3., s0));
+    if (prm->update_tau2_with_fsc) {  // new tau2 = power spectrum of the new map (:2473-2510)
+        hipfftHandle planO;
+        rc = fft_plan(ctx, HIPFFT_D2Z, rank, ori, &planO);
+        if (rc) return rc;
+        // the transform input must stay intact (it is the output map): transform a copy
+        HIPCHK(hipMemcpyAsync(ctx->rc_M.p, ctx->rc_vol.p, nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (hipfftExecD2Z(planO, ctx->rc_M.p, Z(ctx->rc_Fc.p)) != HIPFFT_SUCCESS)
+            return fail(ctx, ROME_ERR_HIP, "hipfftExecD2Z failed");
+        HIPCHK(hipMemsetAsync(s0, 0, 2 * ori * sizeof(double), s));
+        HIPCHK(rc_map_spectrum(s, ctx->rc_Fc.p, OZ, ori, (double)nv, s0, s0 + ori));
+        std::vector<double> sp(2 * ori);
+        HIPCHK(hipMemcpyAsync(sp.data(), s0, 2 * ori * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const double nf = ctx->is3d ? (double)ori * ori : 1.;
+        for (int i = 0; i < oriF; ++i) tau2[i] = prm->tau2_fudge * (sp[i] / sp[ori + i] * nf / 2.);
+    }
+    HIPCHK(hipMemcpyAsync(vol_out, ctx->rc_vol.p, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return ROME_OK;
 }

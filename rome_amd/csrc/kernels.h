@@ -263,6 +263,35 @@ hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long
                                 const int* blk_t, const int* task_m, int n, int NE, int2* out);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
 hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a);
+// M-step reconstruction (reconstruct.hip)
+hipError_t rc_fold(hipStream_t s, double2* data, double* weight, int Dz, int P, int PF);
+hipError_t rc_decenter_real(hipStream_t s, const double* cen, double* out, int Dz, int P, int PF, int max_r2);
+hipError_t rc_spectrum(hipStream_t s, const double* Fw, int Dz, int P, int PF, int max_r2, int pf, double osc, int oriF,
+                       double* s0, double* s1);
+hipError_t rc_map(hipStream_t s, double* Fw, int Dz, int P, int PF, int max_r2, int pf, double osc, double tau2_fudge,
+                  const double* tau2, int minres, int oriF, double* dvp, double* cnt, int* bad);
+hipError_t rc_normalise(hipStream_t s, double* Fw, double2* data, size_t n, double norm);
+hipError_t rc_newweight_init(hipStream_t s, double* Fnw, int Dz, int P, int PF, int max_r2);
+hipError_t rc_fconv(hipStream_t s, const double* Fnw, const double* Fw, double2* Fc, size_t n);
+hipError_t rc_blob(hipStream_t s, double* M, int Dz, int P, int ori, int pf, const double* tab, int ntab,
+                   double sampling, double norm);
+hipError_t rc_newweight(hipStream_t s, const double2* Fc, double* Fnw, int Dz, int P, int PF, int max_r2,
+                        double size3);
+hipError_t rc_final_fconv(hipStream_t s, const double2* data, const double* Fnw, double2* Fc, int Dz, int P, int PF,
+                          int max_r2);
+hipError_t rc_window_ft(hipStream_t s, const double2* Fc, double2* Ft, int Dz, int P, int PF, int NZ, int N, int NF);
+hipError_t rc_center_window(hipStream_t s, const double* Mout, double* vol, int NZ, int N, int OZ, int ori,
+                            double normfft);
+hipError_t rc_bgsum(hipStream_t s, const double* vol, int OZ, int ori, double radius, double width, double* sums);
+hipError_t rc_mask_grid(hipStream_t s, double* vol, int OZ, int ori, int pf, double radius, double width,
+                        const double* sums);
+hipError_t rc_map_spectrum(hipStream_t s, const double2* F, int OZ, int ori, double size3, double* spec, double* cnt);
+hipError_t rc_from_float(hipStream_t s, const float2* d32, const float* w32, double2* d64, double* w64, size_t n);
+// projector grid of a map (fourier_map.hip)
+hipError_t launch_fm_pad(hipStream_t s, const double* vol, int ori, int padori, int dim3, int do_gridding, int pf,
+                         double* Mc, double* corrected);
+hipError_t launch_fm_window(hipStream_t s, const double2* F, int padori, int dim3, int pf, int r_max, int pad,
+                            int oriF, double normfft, double2* grid64, float2* grid32, double* pw, double* cnt);
 
 // precision: 1 BF16, 2 FP16
 hipError_t launch_absmax(hipStream_t s, const float* x, size_t n, int* out);
