@@ -115,23 +115,21 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         return (float)(a0 + kWin) + 1.0f + 1e-4f * (float)(fabs(a0) + fabs(mn) + kWin + fabs(wp.lpdf_max) + fabs(lpo_max));
     };
     float bound = pre_bound(u0);
-    // every entry that may lie inside the window, each thread in increasing e (two groups of VW in
+    // every entry that may lie inside the window, each thread in increasing e (four groups of VW in
     // flight per step): fn(e)
     auto for_window = [&](float bnd, auto&& fn) {
         constexpr size_t VW = Pre::VW, STEP = (size_t)NT * VW;
-        for (size_t e0 = (size_t)tid * VW; e0 < count; e0 += 2 * STEP) {
-            unsigned m0 = pre(e0, bnd, sh.lpoff_f);
-            unsigned m1 = pre(e0 + STEP, bnd, sh.lpoff_f);
-            while (m0) {
-                const int u = __builtin_ctz(m0);
-                m0 &= m0 - 1;
-                fn(e0 + u);
-            }
-            while (m1) {
-                const int u = __builtin_ctz(m1);
-                m1 &= m1 - 1;
-                fn(e0 + STEP + u);
-            }
+        for (size_t e0 = (size_t)tid * VW; e0 < count; e0 += 4 * STEP) {
+            unsigned m[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) m[g] = pre(e0 + g * STEP, bnd, sh.lpoff_f);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                while (m[g]) {
+                    const int u = __builtin_ctz(m[g]);
+                    m[g] &= m[g] - 1;
+                    fn(e0 + g * STEP + u);
+                }
         }
     };
     // ---- scan
