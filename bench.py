@@ -175,7 +175,7 @@ def main():
         synth.load_engine(eng, prob, precision=prec)
         for i in range(args.warmup):
             step(i)
-            log(f"[{prec}] warmup {i} done", eng.batch_stats())
+            log(f"[{prec}] warmup {i} done", eng.batch_stats(), eng.coarse_stats())
         eng.set_profiling(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -187,12 +187,13 @@ def main():
             agree.append(float(np.mean(r["meta"][:, 1] - 1 == tr[:, 0])))
             amax[b] = r["argmax"].copy()
             meta[b] = r["meta"].copy()
-            log(f"[{prec}] step {i} done", eng.batch_stats())
+            log(f"[{prec}] step {i} done", eng.batch_stats(), eng.coarse_stats())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         prof = eng.profile()
         eng.set_profiling(False)
-        return dict(value=args.steps * B / dt, dt=dt, prof=prof, stats=eng.batch_stats(), amax=amax, meta=meta,
+        return dict(value=args.steps * B / dt, dt=dt, prof=prof, stats={**eng.batch_stats(), **eng.coarse_stats()},
+                    amax=amax, meta=meta,
                     truth_agree=float(np.mean(agree)))
 
     arms = {} if (args.only_mixed and args.precision != "fp32") else {"fp32": run_arm("fp32")}

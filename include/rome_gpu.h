@@ -177,7 +177,8 @@ int rome_get_sums(rome_ctx* ctx, rome_sums* out);
 int rome_get_backprojector(rome_ctx* ctx, int iclass, double* data, double* weight);
 
 /* Parity hooks: coarse-pass diff2 of the last pool [n][K][dir][psi][T] (NaN where the
- * reference stores no entry), and the fine-pass list as (key, diff2) with
+ * reference stores no entry; needs ROME_COARSE_FULL, rome_set_coarse_mode), and the fine-pass
+ * list as (key, diff2) with
  * key = ((((img*K + k)*D + dir)*PSI + psi)*T + t)*NOR*NOT + over_rot*NOT + over_trans. */
 int rome_get_coarse_diff2(rome_ctx* ctx, double* out);
 long long rome_get_fine_count(rome_ctx* ctx);
@@ -202,6 +203,21 @@ int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, in
 /* Last pool: n, fine tasks, fine blocks, unique rotations, coarse pixels, fine pixels,
  * rotations x classes, translations. */
 int rome_get_batch_stats(rome_ctx* ctx, long long* out8);
+
+/* Storage of the coarse pass (no counterpart in the reference, which keeps every coarse diff2 of a
+ * particle in exp_Mweight, src/map3d_optimizer.cpp:1403-1608, before
+ * convertSquaredDifferencesToWeights).  ROME_COARSE_LIST (default): the cross-term kernel keeps per
+ * image only the entries whose u = diff2 - ln(pdf_direction pdf_offset) can lie within 64 nats of
+ * the image's minimum u (the only ones whose weights reach the sums), so the full pose term never
+ * goes to HBM; an image with more than list_cap such entries sends the whole pool through the full
+ * path again (same results).  ROME_COARSE_FULL: the whole pose term in HBM, needed by
+ * rome_get_coarse_diff2.  list_cap: entries per image, <= 0 for the default max(65536, a quarter of
+ * the image's coarse entries). */
+enum { ROME_COARSE_LIST = 0, ROME_COARSE_FULL = 1 };
+int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap);
+/* Last pool: out3 = {0 list / 1 full / 2 list overflowed and the pool ran full, largest list,
+ * total list entries}. */
+int rome_get_coarse_stats(rome_ctx* ctx, long long* out3);
 
 /* ---- M-step: model parameters other than the maps ----
  * maximizationOtherParameters (src/map3d_optimizer.cpp:2620-2798; 2D src/map2d_optimizer.cpp:1985-2045)

@@ -26,8 +26,10 @@ EXPORTED = (
     "rome_sampling_pose3d", "rome_sampling_pose2d", "rome_sampling_translations", "rome_sampling_shifts",
     "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
     "rome_get_batch_stats", "rome_maximization_other", "rome_update_model", "rome_fourier_map",
-    "rome_reconstruct",
+    "rome_reconstruct", "rome_set_coarse_mode", "rome_get_coarse_stats",
 )
+
+COARSE_LIST, COARSE_FULL = 0, 1
 
 STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
           "fine_weights", "post", "store", "backproject")
@@ -356,6 +358,17 @@ class Engine:
         out = np.zeros(8, np.int64)
         self._check(lib().rome_get_batch_stats(self._h, _p(out, C.c_longlong)), "rome_get_batch_stats")
         return dict(zip(("n", "tasks", "blocks", "unique_rot", "Pc", "Pf", "M", "T"), out.tolist()))
+
+    def set_coarse_mode(self, mode, list_cap=0):
+        """ROME_COARSE_LIST (keep only the entries within reach of the weights) or ROME_COARSE_FULL
+        (the whole coarse pose term, needed by get_coarse_diff2); list_cap <= 0: the default cap."""
+        m = {"list": COARSE_LIST, "full": COARSE_FULL}.get(mode, mode)
+        self._check(lib().rome_set_coarse_mode(self._h, int(m), int(list_cap)), "rome_set_coarse_mode")
+
+    def coarse_stats(self):
+        out = np.zeros(3, np.int64)
+        self._check(lib().rome_get_coarse_stats(self._h, _p(out, C.c_longlong)), "rome_get_coarse_stats")
+        return dict(zip(("mode", "list_max", "list_total"), out.tolist()))
 
     def project(self, iclass, size, A):
         A = _f64(A).reshape(-1, 9)

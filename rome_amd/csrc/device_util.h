@@ -89,6 +89,25 @@ DEV float2 gather(const float2* __restrict__ g, int pad, int padF, const Coords&
     return out;
 }
 
+// The same trilinear interpolation from the neighbourhood grid: record i holds the eight values
+// the flat gather reads for base voxel i (x0, x0+1 of rows (y0,z0), (y1,z0), (y0,z1), (y1,z1)),
+// so one sample reads 64 contiguous bytes instead of touching four 128-byte lines.
+DEV float2 gather_cube(const float4* __restrict__ cube, int pad, int padF, const Coords& q) {
+    const float4* rec = cube + 4 * (((size_t)q.z0 * pad + q.y0) * padF + q.x0);
+    const float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3];
+    float dx00r = linf(q.fx, a.x, a.z), dx00i = linf(q.fx, a.y, a.w);
+    float dx01r = linf(q.fx, c.x, c.z), dx01i = linf(q.fx, c.y, c.w);
+    float dx10r = linf(q.fx, b.x, b.z), dx10i = linf(q.fx, b.y, b.w);
+    float dx11r = linf(q.fx, d.x, d.z), dx11i = linf(q.fx, d.y, d.w);
+    float dxy0r = linf(q.fy, dx00r, dx10r), dxy0i = linf(q.fy, dx00i, dx10i);
+    float dxy1r = linf(q.fy, dx01r, dx11r), dxy1i = linf(q.fy, dx01i, dx11i);
+    float2 out;
+    out.x = linf(q.fz, dxy0r, dxy1r);
+    out.y = linf(q.fz, dxy0i, dxy1i);
+    if (q.neg) out.y = -out.y;
+    return out;
+}
+
 DEV int px_x(int xy) { return xy & 0xffff; }
 DEV int px_y(int xy) { return (int)(short)(xy >> 16); }
 

@@ -82,6 +82,7 @@ struct rome_ctx {
 
     // device persistent
     DevBuf<float2> grids;
+    DevBuf<float4> cube;  // 3D: 2x2x2 neighbourhood records of grids, rebuilt when a grid changes
     DevBuf<double> Ai_c, Ai_f, sintab, costab, d_sxc, d_syc, d_sxf, d_syf, d_tx, d_ty;
     DevBuf<int> c_nfine, c_ires, c_xy, f_nfine, f_ires, f_xy;
     DevBuf<float2> Sc, Sf;
@@ -90,6 +91,14 @@ struct rome_ctx {
     DevBuf<float2> mrow;
     double lpdf_max = 0.;
     DevBuf<int> emin_key;
+    // coarse list (rome_set_coarse_mode): entries within reach of the weights, per image
+    int coarse_mode = ROME_COARSE_LIST;
+    int list_cap = 0;  // 0: max(65536, a quarter of the image's coarse entries)
+    DevBuf<float> lpd_row, reach, list_val;  // lpd_row [Mpad]: ln pdf_direction per rotation (-inf: none)
+    DevBuf<int> vbound, list_cnt;
+    DevBuf<unsigned> list_ent;
+    long long last_list_max = 0, last_list_total = 0;
+    int last_coarse_full = 0;  // last batch: 0 list, 1 full pose term, 2 list overflowed -> full
     DevBuf<double> cand_w, cand_du;
     DevBuf<long long> cand_e;
     static constexpr int kCandCap = 8192;
@@ -267,6 +276,10 @@ int ensure_ready(rome_ctx* ctx) {
         HIPCHK(launch_project_coarse(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d,
                                      ctx->Ai_c.p, ctx->R, ctx->Mtot, ctx->c_xy.p, ctx->Pc, ctx->Pc_pad, my_r_max,
                                      ctx->Mpad, ctx->FcT.p));
+        if (ctx->is3d) {
+            HIPCHK(ctx->cube.reserve(4 * ctx->grid_n * ctx->K));
+            HIPCHK(launch_build_cube(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->K, ctx->pad, ctx->padF, ctx->cube.p));
+        }
         if (ctx->cfg.precision != ROME_PREC_FP32) {
             const int K16 = ctx->K16();
             const size_t nf = (size_t)2 * ctx->Pc_pad * ctx->Mpad;
@@ -398,27 +411,14 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     stage_end(ctx, 0);
     // ---- coarse cross term + norms
     HIPCHK(ctx->At.reserve((size_t)n * ctx->Mpad));
+    WeightParams wp = weight_params(ctx);
     stage_begin(ctx, 1);
     HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
-    stage_end(ctx, 1);
-    HIPCHK(ctx->E.reserve((size_t)n * M * T));
-    HIPCHK(ctx->emin_key.reserve(n));
-    HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
-    // an L2-resident slice of the rotation operand per XCD: about 2.5 MB of its 4 MB L2
-    const int nbm = ctx->Mpad / 128;
-    GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, ctx->E.p, ctx->valid.p, ctx->emin_key.p, Kdim, ctx->Mpad, Npad, M,
-                Ncols, T, gemm_sub_band(nbm, (size_t)128 * Kdim * 4, (size_t)5 << 19)};
-    stage_begin(ctx, 2);
-    if (prec == ROME_PREC_FP32) {
-        HIPCHK(launch_coarse_gemm(s, ga));
-    } else {
-        LowpGemmArgs la{ctx->A16.p, ctx->W16.p, ctx->At.p, ctx->inv_scale.p, ctx->E.p, ctx->valid.p,
-                        ctx->emin_key.p, K16, ctx->Mpad, Npad, M, Ncols, T,
-                        gemm_sub_band(ctx->Mpad / 256, (size_t)256 * K16 * 2, (size_t)5 << 19)};
-        HIPCHK(launch_gemm_lowp(s, prec, la));
+    if (ctx->coarse_mode == ROME_COARSE_LIST) {
+        HIPCHK(ctx->reach.reserve(n));
+        HIPCHK(launch_list_reach(s, n, wp, ctx->reach.p));
     }
-    stage_end(ctx, 2);
-    // ---- coarse weights and significance
+    stage_end(ctx, 1);
     for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_maxw, &ctx->c_thr, &ctx->f_dmin, &ctx->f_sumw, &ctx->f_maxw,
                     &ctx->f_thr, &ctx->wnorm})
         HIPCHK(b->reserve(n));
@@ -429,31 +429,89 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->c_status.reserve(n));
     HIPCHK(ctx->f_status.reserve(n));
     HIPCHK(ctx->sigmask.reserve((size_t)n * M * ctx->TW));
-    WeightParams wp = weight_params(ctx);
-    WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p, ctx->c_nsig.p,
-                     ctx->c_status.p};
-    stage_begin(ctx, 3);
+    HIPCHK(ctx->emin_key.reserve(n));
     HIPCHK(ctx->cand_w.reserve((size_t)n * rome_ctx::kCandCap));
     HIPCHK(ctx->cand_du.reserve((size_t)n * rome_ctx::kCandCap));
     HIPCHK(ctx->cand_e.reserve((size_t)n * rome_ctx::kCandCap));
-    const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
-    HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->C2c.p, ctx->emin_key.p, wc, ctx->sigmask.p, ctx->TW, cand));
-    stage_end(ctx, 3);
-    // ---- tasks (image, rotation) of the fine pass
     HIPCHK(ctx->ntask.reserve(n));
     HIPCHK(ctx->nblk.reserve(n));
     HIPCHK(ctx->task_off.reserve(n + 1));
     HIPCHK(ctx->blk_off.reserve(n + 1));
-    stage_begin(ctx, 4);
-    HIPCHK(launch_task_build(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p,
-                             ctx->blk_off.p));
-    stage_end(ctx, 4);
+    const WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p, ctx->c_nsig.p,
+                           ctx->c_status.p};
+    const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
     long long tot[2];
-    HIPCHK(hipMemcpyAsync(&tot[0], ctx->task_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&tot[1], ctx->blk_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
-    std::vector<int> cstatus(n);
-    HIPCHK(hipMemcpyAsync(cstatus.data(), ctx->c_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> cstatus(n), lcount;
+    // cross term -> list of the entries within reach of the weights (or the whole pose term E) ->
+    // weights and significance -> fine-pass tasks; returns after one host synchronisation
+    auto coarse = [&](bool list) -> int {
+        HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
+        ListArgs L{};
+        if (list) {
+            const size_t per = (size_t)M * T;
+            const int cap = ctx->list_cap > 0 ? ctx->list_cap : (int)std::max<size_t>(65536, (per + 3) / 4);
+            HIPCHK(ctx->vbound.reserve(n));
+            HIPCHK(ctx->list_cnt.reserve(n));
+            HIPCHK(ctx->list_ent.reserve((size_t)n * cap));
+            HIPCHK(ctx->list_val.reserve((size_t)n * cap));
+            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->vbound.p, 0x7fffffff, n, s));
+            HIPCHK(hipMemsetAsync(ctx->list_cnt.p, 0, (size_t)n * sizeof(int), s));
+            L = ListArgs{ctx->lpd_row.p, ctx->reach.p, ctx->vbound.p, ctx->list_ent.p, ctx->list_val.p,
+                         ctx->list_cnt.p, cap};
+        } else {
+            HIPCHK(ctx->E.reserve((size_t)n * M * T));
+        }
+        float* Ep = list ? nullptr : ctx->E.p;
+        stage_begin(ctx, 2);
+        if (prec == ROME_PREC_FP32) {
+            // an L2-resident slice of the rotation operand per XCD: about 2.5 MB of its 4 MB L2
+            const int nbm = ctx->Mpad / 128;
+            GemmArgs ga{ctx->FcT.p, ctx->WcT.p, ctx->At.p, Ep, ctx->valid.p, ctx->emin_key.p, Kdim, ctx->Mpad, Npad, M,
+                        Ncols, T, gemm_sub_band(nbm, (size_t)128 * Kdim * 4, (size_t)5 << 19), L};
+            HIPCHK(launch_coarse_gemm(s, ga));
+        } else {
+            LowpGemmArgs la{ctx->A16.p, ctx->W16.p, ctx->At.p, ctx->inv_scale.p, Ep, ctx->valid.p,
+                            ctx->emin_key.p, K16, ctx->Mpad, Npad, M, Ncols, T,
+                            gemm_sub_band(ctx->Mpad / 256, (size_t)256 * K16 * 2, (size_t)5 << 19), L};
+            HIPCHK(launch_gemm_lowp(s, prec, la));
+        }
+        stage_end(ctx, 2);
+        stage_begin(ctx, 3);
+        if (list)
+            HIPCHK(launch_weights_list(s, n, wp, L, ctx->C2c.p, ctx->emin_key.p, wc, ctx->sigmask.p, ctx->TW, cand));
+        else
+            HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->C2c.p, ctx->emin_key.p, wc, ctx->sigmask.p, ctx->TW,
+                                         cand));
+        stage_end(ctx, 3);
+        stage_begin(ctx, 4);
+        HIPCHK(launch_task_build(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p,
+                                 ctx->blk_off.p));
+        stage_end(ctx, 4);
+        HIPCHK(hipMemcpyAsync(&tot[0], ctx->task_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&tot[1], ctx->blk_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(cstatus.data(), ctx->c_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (list) {
+            lcount.resize(n);
+            HIPCHK(hipMemcpyAsync(lcount.data(), ctx->list_cnt.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        return ROME_OK;
+    };
+    const bool use_list = ctx->coarse_mode == ROME_COARSE_LIST;
+    rc = coarse(use_list);
+    if (rc) return rc;
+    ctx->last_list_max = 0;
+    ctx->last_list_total = 0;
+    for (int v : lcount) {
+        ctx->last_list_max = std::max<long long>(ctx->last_list_max, v);
+        ctx->last_list_total += v;
+    }
+    ctx->last_coarse_full = use_list ? 0 : 1;
+    if (use_list && std::any_of(cstatus.begin(), cstatus.end(), [](int v) { return (v & 8) != 0; })) {
+        ctx->last_coarse_full = 2;  // a list overflowed: the whole batch again with the full pose term
+        rc = coarse(false);
+        if (rc) return rc;
+    }
     for (int i = 0; i < n; ++i)
         if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
     const long long ntasks = tot[0], nblocks = tot[1];
@@ -481,7 +539,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->D2.reserve((size_t)nblocks * NE));
     HIPCHK(ctx->Pfine.reserve((size_t)U * NOR * ctx->Pf));
     stage_begin(ctx, 5);
-    HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d, ctx->Ai_f.p,
+    HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->is3d ? ctx->cube.p : nullptr, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d, ctx->Ai_f.p,
                                ctx->R, NOR, ctx->ulist.p, U, ctx->f_xy.p, ctx->Pf, ctx->r_max, ctx->Pfine.p));
     stage_end(ctx, 5);
     FineArgs fa;
@@ -943,7 +1001,7 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
     std::vector<unsigned char> valid((size_t)ctx->Mpad, 0);
     std::vector<double> pm((size_t)ctx->Mtot), lpm((size_t)ctx->Mtot);
     std::vector<int> km((size_t)ctx->Mtot);
-    std::vector<float2> mrow((size_t)ctx->Mtot);
+    std::vector<float2> mrow((size_t)ctx->Mpad, make_float2(-INFINITY, 0.f));  // padding rows: no prior
     for (int k = 0; k < K; ++k)
         for (int d = 0; d < D; ++d) {
             const double p = pdf_direction[(size_t)k * D + d];
@@ -967,6 +1025,9 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
     HIPCHK(ctx->lpdf_m.upload(lpm.data(), lpm.size(), ctx->stream));
     HIPCHK(ctx->k_m.upload(km.data(), km.size(), ctx->stream));
     HIPCHK(ctx->mrow.upload(mrow.data(), mrow.size(), ctx->stream));
+    std::vector<float> lpd_row(mrow.size());
+    for (size_t m = 0; m < mrow.size(); ++m) lpd_row[m] = mrow[m].x;
+    HIPCHK(ctx->lpd_row.upload(lpd_row.data(), lpd_row.size(), ctx->stream));
     ctx->lpdf_max = -INFINITY;
     for (double v : lpm) ctx->lpdf_max = std::max(ctx->lpdf_max, v);
     // logsigma2 per group (storeWeightedSums, src/map3d_optimizer.cpp:1998-2008): same loop order as the reference
@@ -1132,6 +1193,8 @@ int rome_get_coarse_diff2(rome_ctx* ctx, double* out) {
     if (!ctx || !out) return ROME_ERR_ARG;
     const int n = ctx->last_n;
     if (n == 0) return fail(ctx, ROME_ERR_ARG, "no E-step has run");
+    if (ctx->last_coarse_full == 0)
+        return fail(ctx, ROME_ERR_ARG, "the last pool kept a coarse list: set ROME_COARSE_FULL first");
     HIPCHK(hipSetDevice(ctx->device));
     const size_t per = (size_t)ctx->Mtot * ctx->T;
     std::vector<float> e((size_t)n * per);
@@ -1228,6 +1291,22 @@ int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, in
         if (stage_calls) stage_calls[i] = ctx->stage_calls[i];
     }
     return rome_ctx::kStages;
+}
+
+int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (mode != ROME_COARSE_LIST && mode != ROME_COARSE_FULL) return fail(ctx, ROME_ERR_ARG, "unknown coarse mode %d", mode);
+    ctx->coarse_mode = mode;
+    ctx->list_cap = list_cap > 0 ? list_cap : 0;
+    return ROME_OK;
+}
+
+int rome_get_coarse_stats(rome_ctx* ctx, long long* out) {
+    if (!ctx || !out) return ROME_ERR_ARG;
+    out[0] = ctx->last_coarse_full;
+    out[1] = ctx->last_list_max;
+    out[2] = ctx->last_list_total;
+    return ROME_OK;
 }
 
 int rome_get_batch_stats(rome_ctx* ctx, long long* out) {

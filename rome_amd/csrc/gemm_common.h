@@ -35,6 +35,19 @@ DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
 template <int TN>
 constexpr int epi_slots() { return TN + 2; }
 
+// List test of one E value e with ln pdf_direction l against the image's limit (the slack
+// 1e-6 (|e| + |l|) covers the FP32 rounding of e - l); one expression for counting and storing.
+DEV bool list_keeps(float e, float l, float lim) {
+    return __builtin_fmaf(-1e-6f, fabsf(e) + fabsf(l), e - l) < lim;
+}
+
+// One E value from an accumulator: At - C (FP32 arm) or At - C * inv_scale as one fused
+// multiply-add (scaled arms), the same rounding in every epilogue.
+template <bool SCALED>
+DEV float e_value(float at, float c, float s) {
+    return SCALED ? __builtin_fmaf(-c, s, at) : at - c;
+}
+
 // E[img][t][m] = At[img][m] - s_img * C[m][img*T + t] (s_img = inv_scale[img] when SCALED, else 1)
 // for a tile whose first column is n0 (TN columns); this wave holds NI x NJ accumulator blocks
 // starting at row r0 and column c0.  Also the per-image minimum of E over rotations with a positive
@@ -75,8 +88,7 @@ DEV void gemm_epilogue(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int la
                 float ev[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const float c = SCALED ? acc[i][j][4 * q + u] * s : acc[i][j][4 * q + u];
-                    ev[u] = atv[u] - c;
+                    ev[u] = e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s);
                     if (base + u < Mtot && ((v4 >> (8 * u)) & 0xff)) emin = fminf(emin, ev[u]);
                 }
                 if (vec && base + 3 < Mtot) {
@@ -137,8 +149,7 @@ DEV void gemm_epilogue_staged(const f16v (&acc)[NI][NJ], int r0, int c0, int n0,
                 float ev[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const float c = SCALED ? acc[i][j][4 * q + u] * s_j[j] : acc[i][j][4 * q + u];
-                    ev[u] = atv[u] - c;
+                    ev[u] = e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j]);
                     if (colok && base + u < Mtot && ((v4 >> (8 * u)) & 0xff)) emin_j[j] = fminf(emin_j[j], ev[u]);
                 }
                 *reinterpret_cast<float4*>(&stage[(j * 32 + (lane & 31)) * LD + 8 * q + 4 * h]) =
@@ -168,6 +179,142 @@ DEV void gemm_epilogue_staged(const f16v (&acc)[NI][NJ], int r0, int c0, int n0,
             }
         }
         __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+        if (c0 + j * 32 + (lane & 31) < Ncols && emin_j[j] < INFINITY)
+            atomicMin(&slot_min[img_j[j] - img0], float_order_key(emin_j[j]));
+    __syncthreads();
+    for (int q = tid; q < nslot; q += blockDim.x)
+        if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img0 + q], slot_min[q]);
+}
+
+// List epilogue (ListArgs): the per-image minimum of E as gemm_epilogue, and instead of E the
+// entries whose v = E - ln pdf_direction can lie within reach[img] of the image's minimum v.
+// v is evaluated in FP32 as vf with the slack tol = 1e-6 (|E| + |ln pdf_direction|), far above its
+// rounding.  Per 32-row accumulator block: the E values go to this wave's LDS region (column-major,
+// as gemm_epilogue_staged) while min(vf + tol) per image lowers the image's bound B >= min v (LDS
+// slot, then one global atomic per image); then eight lanes per column test the staged values
+// against B + reach and append the entries that pass: offsets from LDS counters per image, one
+// global atomic per image and block to reserve the range, then the stores.  B only falls, so every
+// entry the weights need passes, whatever the order in which tiles run.  All threads of the block
+// call it (barriers).  slot_* arrays hold epi_slots<TN>() entries.
+template <bool SCALED, int NI, int NJ, int TN>
+DEV void gemm_epilogue_list(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
+                            const float* __restrict__ At, const float* __restrict__ inv_scale,
+                            const unsigned char* __restrict__ valid, int* __restrict__ emin_key, int Mtot, int Mpad,
+                            int Ncols, int T, const ListArgs& L, int* slot_min, int* slot_vb, float* slot_lim,
+                            int* slot_cnt, float* stage) {
+    constexpr int LD = 36;  // column stride in floats (16-byte aligned)
+    const int img0 = n0 / T;
+    const int last = min(n0 + TN, Ncols) - 1;
+    const int nslot = last >= n0 ? last / T - img0 + 1 : 0;
+    for (int q = tid; q < nslot; q += blockDim.x) {
+        slot_min[q] = INT_MAX;
+        slot_vb[q] = INT_MAX;
+        slot_cnt[q] = 0;
+    }
+    __syncthreads();
+    const int h = lane >> 5;
+    int img_j[NJ];
+    float s_j[NJ], emin_j[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int col = min(c0 + j * 32 + (lane & 31), Ncols - 1);  // clamped: padding columns are dropped
+        img_j[j] = col / T;
+        s_j[j] = SCALED ? inv_scale[img_j[j]] : 1.f;
+        emin_j[j] = INFINITY;
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        // registers -> LDS, minimum of E over valid rows, min(vf + tol) of this lane's column
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const float* Ar = At + (size_t)img_j[j] * Mpad;
+            const bool colok = c0 + j * 32 + (lane & 31) < Ncols;
+            float vb = INFINITY;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int base = r0 + i * 32 + 8 * q + 4 * h;
+                const float4 at = *reinterpret_cast<const float4*>(Ar + base);
+                const float4 lp = *reinterpret_cast<const float4*>(L.lpd + base);
+                const uint32_t v4 = *reinterpret_cast<const uint32_t*>(valid + base);
+                const float atv[4] = {at.x, at.y, at.z, at.w};
+                const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
+                float ev[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ev[u] = e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j]);
+                    if (colok && base + u < Mtot && ((v4 >> (8 * u)) & 0xff)) emin_j[j] = fminf(emin_j[j], ev[u]);
+                    const float vf = ev[u] - lpd[u];
+                    vb = fminf(vb, vf + 1e-6f * (fabsf(ev[u]) + fabsf(lpd[u])));  // +inf without a prior
+                }
+                *reinterpret_cast<float4*>(&stage[(j * 32 + (lane & 31)) * LD + 8 * q + 4 * h]) =
+                    make_float4(ev[0], ev[1], ev[2], ev[3]);
+            }
+            if (colok && vb < INFINITY) atomicMin(&slot_vb[img_j[j] - img0], float_order_key(vb));
+        }
+        __syncthreads();
+        for (int q = tid; q < nslot; q += blockDim.x) {
+            const int img = img0 + q;
+            const int mine = slot_vb[q];
+            const int glob = __hip_atomic_load(&L.vbound[img], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (mine < glob) atomicMin(&L.vbound[img], mine);
+            const int b = min(mine, glob);  // a stale global value is larger: still a valid bound
+            slot_vb[q] = b;
+            const float bv = float_from_order_key(b);
+            slot_lim[q] = b == INT_MAX ? -INFINITY : bv + L.reach[img] + 1e-6f * fabsf(bv);  // covers the sum's rounding
+        }
+        __syncthreads();
+        // LDS -> list: lane group of 8 = one column's 32 rows, four per lane.  Count, reserve, store.
+        const int row0 = r0 + i * 32 + 4 * (lane & 7);
+        const float4 lp = *reinterpret_cast<const float4*>(L.lpd + row0);
+        const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
+        int off[NJ * 4];
+#pragma unroll
+        for (int p = 0; p < NJ * 4; ++p) {
+            off[p] = -1;
+            const int c = p * 8 + (lane >> 3);
+            const int col = c0 + c;
+            if (col >= Ncols) continue;
+            const int sl = col / T - img0;
+            const float lim = slot_lim[sl];
+            const float4 v = *reinterpret_cast<const float4*>(&stage[c * LD + 4 * (lane & 7)]);
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+            int cnt = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cnt += list_keeps(vv[u], lpd[u], lim);
+            if (cnt) off[p] = atomicAdd(&slot_cnt[sl], cnt);
+        }
+        __syncthreads();
+        for (int q = tid; q < nslot; q += blockDim.x) {
+            const int c = slot_cnt[q];
+            slot_cnt[q] = c ? atomicAdd(&L.count[img0 + q], c) : 0;  // now the base of this block's range
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < NJ * 4; ++p) {
+            if (off[p] < 0) continue;
+            const int c = p * 8 + (lane >> 3);
+            const int col = c0 + c;
+            const int img = col / T;
+            const float lim = slot_lim[img - img0];
+            const float4 v = *reinterpret_cast<const float4*>(&stage[c * LD + 4 * (lane & 7)]);
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+            int pos = slot_cnt[img - img0] + off[p];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!list_keeps(vv[u], lpd[u], lim)) continue;
+                if (pos < L.cap) {
+                    const size_t o = (size_t)img * L.cap + pos;
+                    L.ent[o] = (unsigned)(col - img * T) * (unsigned)Mtot + (unsigned)(row0 + u);
+                    L.val[o] = vv[u];
+                }
+                ++pos;
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < nslot; q += blockDim.x) slot_cnt[q] = 0;  // next row block counts afresh
     }
 #pragma unroll
     for (int j = 0; j < NJ; ++j)

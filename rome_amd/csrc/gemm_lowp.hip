@@ -35,30 +35,38 @@ struct Lowp<_Float16> {
     static DEV f16v mfma(v8 a, v8 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
 };
 
-// 256x256 output tile per block of 8 waves (2 along rotations x 4 along columns); each wave owns
-// 128x64 = 4x2 accumulator blocks (128 accumulator registers, two waves per SIMD).  BK = 64: per
+// 256x256 output tile per block of 8 waves (4 along rotations x 2 along columns); each wave owns
+// 64x128 = 2x4 accumulator blocks (128 accumulator registers, two waves per SIMD).  BK = 64: per
 // 16-k step a wave reads 6 fragments from LDS for 8 MFMAs; the global loads of chunk c+1 (held in
 // eight named 16-byte registers: an array here would live in scratch) are in flight while chunk c is
-// multiplied by both waves of the SIMD.  Two LDS stages of 72 KB, one block per CU.
-template <typename T>
+// multiplied by both waves of the SIMD.  Two LDS stages of 72 KB, one block per CU; after the main
+// loop the same 144 KB stage the E rows.  (4x2 waves of 64x128 beat 2x4 waves of 128x64 by 15 % in
+// the main loop at K = 512.)
+template <typename T, bool LIST>
 __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
     constexpr int BM = 256, BN = 256, BK = 64, LDW = BK + 8;  // 144-byte LDS rows: conflict-free b128 reads
     typedef typename Lowp<T>::v8 v8;
-    __shared__ __attribute__((aligned(16))) T As[2][BM][LDW];
-    __shared__ __attribute__((aligned(16))) T Bs[2][BN][LDW];
+    constexpr int TI = 2, TJ = 4;  // accumulator blocks per wave
+    // operand stages [A|B][stage][row][k]; the epilogue reuses the whole array
+    __shared__ __attribute__((aligned(16))) T ABs[2][2][BM][LDW];
     __shared__ int slot_min[epi_slots<BN>()];
+    __shared__ int slot_vb[LIST ? epi_slots<BN>() : 1];
+    __shared__ float slot_lim[LIST ? epi_slots<BN>() : 1];
+    __shared__ int slot_cnt[LIST ? epi_slots<BN>() : 1];
+    auto& As = ABs[0];
+    auto& Bs = ABs[1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 2, wn = wave & 3;
+    const int wm = wave >> 1, wn = wave & 1;
     int bm, bn;
     gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
     const int m0 = bm * BM, n0 = bn * BN;
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
-    f16v acc[4][2];
+    f16v acc[TI][TJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
     const int nk = a.K16 / BK;
@@ -93,13 +101,13 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
 #define LOWP_COMPUTE(st)                                                                                    \
     do {                                                                                                    \
         _Pragma("unroll") for (int kk = 0; kk < BK; kk += 16) {                                             \
-            v8 av[4], bv[2];                                                                                \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i) av[i] =                                           \
-                *reinterpret_cast<const v8*>(&As[st][wm * 128 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]); \
-            _Pragma("unroll") for (int j = 0; j < 2; ++j) bv[j] =                                           \
-                *reinterpret_cast<const v8*>(&Bs[st][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);  \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                                   \
-                _Pragma("unroll") for (int j = 0; j < 2; ++j) acc[i][j] = Lowp<T>::mfma(av[i], bv[j], acc[i][j]); \
+            v8 av[TI], bv[TJ];                                                                              \
+            _Pragma("unroll") for (int i = 0; i < TI; ++i) av[i] =                                          \
+                *reinterpret_cast<const v8*>(&As[st][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]); \
+            _Pragma("unroll") for (int j = 0; j < TJ; ++j) bv[j] =                                          \
+                *reinterpret_cast<const v8*>(&Bs[st][wn * 128 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]); \
+            _Pragma("unroll") for (int i = 0; i < TI; ++i)                                                  \
+                _Pragma("unroll") for (int j = 0; j < TJ; ++j) acc[i][j] = Lowp<T>::mfma(av[i], bv[j], acc[i][j]); \
         }                                                                                                   \
     } while (0)
     uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
@@ -116,11 +124,17 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
 #undef LOWP_LOAD
 #undef LOWP_STORE
 #undef LOWP_COMPUTE
-    // the operand stages are free now: 8 waves x 64 columns x 36 floats of staging = the A stages
-    float* stage = reinterpret_cast<float*>(&As[0][0][0]) + wave * (64 * 36);
-    static_assert(sizeof(As) >= 8 * 64 * 36 * sizeof(float), "epilogue staging does not fit");
-    gemm_epilogue_staged<true, 4, 2, BN>(acc, m0 + wm * 128, n0 + wn * 64, n0, lane, tid, a.At, a.inv_scale, a.E,
-                                         a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min, stage);
+    // the operand stages are free now: 8 waves x 128 columns x 36 floats of staging
+    float* stage = reinterpret_cast<float*>(&ABs[0][0][0][0]) + wave * (TJ * 32 * 36);
+    static_assert(sizeof(ABs) >= 8 * TJ * 32 * 36 * sizeof(float), "epilogue staging does not fit");
+    if constexpr (LIST) {
+        gemm_epilogue_list<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale,
+                                             a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min,
+                                             slot_vb, slot_lim, slot_cnt, stage);
+    } else {
+        gemm_epilogue_staged<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale,
+                                               a.E, a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min, stage);
+    }
 }
 
 // sF: power of two that brings max |FcT| into [0.5, 1) (exact rescaling)
@@ -197,9 +211,17 @@ __global__ void k_prep_lowp(PrepLowpArgs a) {
 hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a) {
     if (a.Mpad % 256 || a.Npad % 256 || a.K16 % 64 || a.K16 <= 0) return hipErrorInvalidValue;
     const int tiles = (a.Mpad / 256) * (a.Npad / 256);
-    if (precision == 1) k_gemm_lowp<__bf16><<<tiles, 512, 0, s>>>(a);
-    else if (precision == 2) k_gemm_lowp<_Float16><<<tiles, 512, 0, s>>>(a);
-    else return hipErrorInvalidValue;
+    const bool list = a.list.count != nullptr;
+    if (!list && !a.E) return hipErrorInvalidValue;
+    if (precision == 1) {
+        if (list) k_gemm_lowp<__bf16, true><<<tiles, 512, 0, s>>>(a);
+        else k_gemm_lowp<__bf16, false><<<tiles, 512, 0, s>>>(a);
+    } else if (precision == 2) {
+        if (list) k_gemm_lowp<_Float16, true><<<tiles, 512, 0, s>>>(a);
+        else k_gemm_lowp<_Float16, false><<<tiles, 512, 0, s>>>(a);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -44,15 +44,38 @@ struct PrepArgs {
     float* MC2bp;           // [n][Pf]
 };
 
+// Window of the coarse weights: entries with u = d - ln(pdf_direction * pdf_offset) below
+// min u + kWeightWindow nats are the only ones whose weights reach the sums (beyond it a weight is
+// below exp(-48) of the largest).
+constexpr double kWeightWindow = 64.0;
+
+// Coarse list output of the cross-term GEMMs (instead of the full pose term E): per image the
+// entries e = t*Mtot + m that can lie within kWeightWindow of the image's minimum u, with their E
+// value.  With v = E - ln pdf_direction(m) and D = max - min of ln pdf_offset over the image's
+// (class, translation) pairs, u = v - ln pdf_offset lies in [v - max, v - min], so every entry with
+// u < min u + W has v < min v + D + W: the kernel keeps those, against a running bound of min v
+// (every tile lowers it), a superset of the window.  Entries past `cap` are counted but not stored
+// (the caller then reruns with E).
+struct ListArgs {
+    const float* lpd;     // [Mpad] ln pdf_direction per rotation (-inf: no prior / padding)
+    const float* reach;   // [n] D + W + slack per image (+inf when some pdf_offset is zero)
+    int* vbound;          // [n] ordered-int key of an upper bound of min v (INT_MAX before the pass)
+    unsigned* ent;        // [n][cap]
+    float* val;           // [n][cap]
+    int* count;           // [n] entries found (zero before the pass)
+    int cap;
+};
+
 struct GemmArgs {
     const float* A;   // [K][Mpad]
     const float* B;   // [K][Npad]
     const float* At;  // [n][Mpad]
-    float* E;         // [n][Mtot][T]
+    float* E;         // [n][T][Mtot] (unused with a list)
     const unsigned char* valid;  // [Mpad] rotation has a positive prior (0 in the padding)
     int* emin_key;    // [n] ordered-int key of the per-image minimum of E (init INT_MAX)
     int K, Mpad, Npad, Mtot, Ncols, T;
     int sband;        // m-tiles per L2-resident sub-band of the XCD-banded tile order (0: plain order)
+    ListArgs list;    // list.count != nullptr: list output instead of E
 };
 
 // Host side: the largest divisor of the per-XCD band of m-tiles whose slice of the rotation operand
@@ -71,11 +94,12 @@ struct LowpGemmArgs {
     const void* B;            // [Npad][K16] scaled Minv*CTF*X*S, row = img*T + t
     const float* At;          // [n][Mpad]
     const float* inv_scale;   // [n] 1 / (sF * s_img)
-    float* E;                 // [n][Mtot][T]
+    float* E;                 // [n][T][Mtot] (unused with a list)
     const unsigned char* valid;
     int* emin_key;
     int K16, Mpad, Npad, Mtot, Ncols, T;
     int sband;
+    ListArgs list;            // list.count != nullptr: list output instead of E
 };
 
 struct PrepLowpArgs {
@@ -228,9 +252,11 @@ hipError_t launch_phase_table(hipStream_t s, const double* sx, const double* sy,
 hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
                                  const double* Ai, int R, int Mtot, const int* xy, int Pc, int Pc_pad, int my_r_max,
                                  int Mpad, float* FcT);
-hipError_t launch_project_fine(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
-                               const double* Ai_fine, int R, int NOR, const int* ulist, int U, const int* xy, int Pf,
-                               int r_max, float2* Pfine);
+// cube[K][grid_n] records of 4 float4: the 2x2x2 trilinear neighbourhood of every voxel of a 3D grid
+hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, int K, int pad, int padF, float4* cube);
+hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4* cube, size_t grid_stride, int pad,
+                               int padF, int c, int is3d, const double* Ai_fine, int R, int NOR, const int* ulist, int U,
+                               const int* xy, int Pf, int r_max, float2* Pfine);
 hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
                                 int nrot, int size, int my_r_max, float2* out);
 hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a);
@@ -247,6 +273,14 @@ struct CandArgs {  // per-image candidate list of the weight pass: [n][cap]
 hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const double* C2c,
                                  const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
                                  const CandArgs& cand);
+// The same weights and significance from the coarse list (ListArgs); an image whose list
+// overflowed gets status 8 and nothing else.
+hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, const ListArgs& list, const double* C2c,
+                               const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                               const CandArgs& cand);
+// reach[img] = kWeightWindow + (max - min of ln pdf_offset over (class, translation)) + slack,
+// +inf when a pdf_offset is zero
+hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float* reach);
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
                                const WeightOutArgs& wo, const CandArgs& cand);

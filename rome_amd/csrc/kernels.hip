@@ -67,10 +67,31 @@ __global__ void k_project_coarse(const float2* __restrict__ grids, size_t grid_s
     FcT[(size_t)(Pc_pad + p) * Mpad + m] = v.y;
 }
 
-// Fine projections for the unique rotations the batch needs: Pf[u][o][p].
-__global__ void k_project_fine(const float2* __restrict__ grids, size_t grid_stride, int pad, int padF, int c, int is3d,
-                               const double* __restrict__ Ai_fine, int R, int NOR, const int* __restrict__ ulist,
-                               const int* __restrict__ xy, int Pf, int r_max, float2* __restrict__ Pfine) {
+// Neighbourhood grid of a 3D class grid: record i = the eight values the trilinear gather reads
+// for base voxel i (flat indices i, i+1, i+padF, i+padF+1, then the same one z plane on), zero
+// past the end of the class grid.  64 bytes per voxel: 8x the grid's HBM for one 64-byte read per
+// fine-projection sample instead of four scattered 128-byte lines.  blockIdx.y = class.
+__global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad, int padF, float4* __restrict__ cube) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2* g = grids + n * blockIdx.y;
+    const size_t sy = (size_t)padF, sz = (size_t)pad * padF;
+    auto at = [&](size_t j) { return j < n ? g[j] : make_float2(0.f, 0.f); };
+    const float2 v0 = at(i), v1 = at(i + 1), v2 = at(i + sy), v3 = at(i + sy + 1);
+    const float2 v4 = at(i + sz), v5 = at(i + sz + 1), v6 = at(i + sz + sy), v7 = at(i + sz + sy + 1);
+    float4* o = cube + 4 * (n * blockIdx.y + i);
+    o[0] = make_float4(v0.x, v0.y, v1.x, v1.y);
+    o[1] = make_float4(v2.x, v2.y, v3.x, v3.y);
+    o[2] = make_float4(v4.x, v4.y, v5.x, v5.y);
+    o[3] = make_float4(v6.x, v6.y, v7.x, v7.y);
+}
+
+// Fine projections for the unique rotations the batch needs: Pf[u][o][p].  3D classes read the
+// neighbourhood grid (cube, same values and arithmetic as the flat gather); 2D the flat grid.
+__global__ void k_project_fine(const float2* __restrict__ grids, const float4* __restrict__ cube, size_t grid_stride,
+                               int pad, int padF, int c, int is3d, const double* __restrict__ Ai_fine, int R, int NOR,
+                               const int* __restrict__ ulist, const int* __restrict__ xy, int Pf, int r_max,
+                               float2* __restrict__ Pfine) {
     const int p = blockIdx.y * blockDim.x + threadIdx.x;
     const int uo = blockIdx.x;  // U*NOR can exceed the 65535 limit of gridDim.y
     if (p >= Pf) return;
@@ -81,7 +102,7 @@ __global__ void k_project_fine(const float2* __restrict__ grids, size_t grid_str
     float2 v = make_float2(0.f, 0.f);
     if (x * x + y * y <= r_max * r_max) {
         Coords q = map_coords(Ai_fine + 6 * ((size_t)r * NOR + o), x, y, c, is3d != 0);
-        v = gather(grids + grid_stride * k, pad, padF, q, is3d != 0);
+        v = cube ? gather_cube(cube + 4 * grid_stride * k, pad, padF, q) : gather(grids + grid_stride * k, pad, padF, q, false);
     }
     Pfine[((size_t)u * NOR + o) * Pf + p] = v;
 }
@@ -210,11 +231,19 @@ __global__ __launch_bounds__(256) void k_coarse_aterm(const float* __restrict__ 
 // ------------------------------------------------------------------ coarse cross term (K2), FP32 MFMA
 // C[m][j] = sum_kk FcT[kk][m] * WcT[kk][j];  E[img][m][t] = At[img][m] - C[m][img*T+t].
 // 256 threads, 128x128 tile, BK = 16, v_mfma_f32_32x32x2f32, LDS double buffer.
+template <bool LIST>
 __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
     constexpr int BM = 128, BN = 128, BK = 16;
-    __shared__ float As[2][BK][BM];
-    __shared__ float Bs[2][BK][BN];
+    // operand stages; with a list output the epilogue stages 4 waves x 64 columns x 36 floats here
+    constexpr int kOps = 2 * 2 * BK * BM, kStage = 4 * 64 * 36;
+    __shared__ __attribute__((aligned(16))) float smem[LIST && kStage > kOps ? kStage : kOps];
+    typedef float Stage[BK][BM];
+    Stage* As = reinterpret_cast<Stage*>(smem);
+    Stage* Bs = reinterpret_cast<Stage*>(smem + 2 * BK * BM);
     __shared__ int slot_min[epi_slots<BN>()];
+    __shared__ int slot_vb[LIST ? epi_slots<BN>() : 1];
+    __shared__ float slot_lim[LIST ? epi_slots<BN>() : 1];
+    __shared__ int slot_cnt[LIST ? epi_slots<BN>() : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int bm, bn;
@@ -273,8 +302,13 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
         }
         __syncthreads();
     }
-    gemm_epilogue<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr, a.E, a.valid,
-                                   a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
+    if constexpr (LIST)
+        gemm_epilogue_list<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr, a.valid,
+                                            a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb,
+                                            slot_lim, slot_cnt, smem + wave * (64 * 36));
+    else
+        gemm_epilogue<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr, a.E, a.valid,
+                                       a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
 }
 
 // ------------------------------------------------------------------ task building
@@ -759,13 +793,22 @@ hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid
     return hipSuccess;
 }
 
-hipError_t launch_project_fine(hipStream_t s, const float2* grids, size_t grid_stride, int pad, int padF, int c, int is3d,
-                               const double* Ai_fine, int R, int NOR, const int* ulist, int U, const int* xy, int Pf,
-                               int r_max, float2* Pfine) {
+hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, int K, int pad, int padF, float4* cube) {
+    if (grid_n == 0 || K == 0) return hipSuccess;
+    dim3 grid((unsigned)((grid_n + 255) / 256), K);
+    k_build_cube<<<grid, 256, 0, s>>>(grids, grid_n, pad, padF, cube);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4* cube, size_t grid_stride, int pad,
+                               int padF, int c, int is3d, const double* Ai_fine, int R, int NOR, const int* ulist, int U,
+                               const int* xy, int Pf, int r_max, float2* Pfine) {
     if (U == 0) return hipSuccess;
+    if (is3d && !cube) return hipErrorInvalidValue;  // 3D projections read the neighbourhood grid
     dim3 grid(U * NOR, (Pf + 255) / 256);
-    k_project_fine<<<grid, 256, 0, s>>>(grids, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist, xy, Pf, r_max,
-                                        Pfine);
+    k_project_fine<<<grid, 256, 0, s>>>(grids, is3d ? cube : nullptr, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR,
+                                        ulist, xy, Pf, r_max, Pfine);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -796,7 +839,9 @@ hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot,
 
 hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a) {
     const int tiles = (a.Mpad / 128) * (a.Npad / 128);
-    k_coarse_gemm<<<tiles, 256, 0, s>>>(a);
+    if (a.list.count) k_coarse_gemm<true><<<tiles, 256, 0, s>>>(a);
+    else if (a.E) k_coarse_gemm<false><<<tiles, 256, 0, s>>>(a);
+    else return hipErrorInvalidValue;
     LAUNCH_CHECK();
     return hipSuccess;
 }

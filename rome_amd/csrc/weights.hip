@@ -27,7 +27,7 @@ namespace rome {
 
 namespace {
 
-constexpr double kWin = 64.0;     // histogram window (2048 bins x 1/32 nat)
+constexpr double kWin = kWeightWindow;  // histogram window (2048 bins x 1/32 nat)
 constexpr double kAnchorSlack = 16.0;
 
 struct WeightOut {
@@ -432,6 +432,53 @@ __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const 
     weight_pass(img, (size_t)Mtot * T, C2c[img], wp, get, pre, key, wo, sh, sigmask, TW, cand, emin_key);
 }
 
+// Coarse weights from the list the cross-term GEMM kept (every entry within kWin of the minimum
+// u, and more): entries i = 0 .. count-1 of the image's list, argmax keys in the reference's order.
+__global__ __launch_bounds__(1024) void k_weights_list(WeightParams wp, ListArgs L, const double* __restrict__ C2c,
+                                                       const int* __restrict__ emin_key, WeightOut wo,
+                                                       uint32_t* __restrict__ sigmask, int TW, Cand cand) {
+    __shared__ WShared sh;
+    const int img = blockIdx.x;
+    const int cnt = L.count[img];
+    if (cnt > L.cap) {  // the list overflowed: the caller reruns the batch with the full pose term
+        if (threadIdx.x == 0) wo.status[img] = 8;
+        return;
+    }
+    const int T = wp.T;
+    const unsigned long long magic = wp.magicM;
+    const int Mtot = wp.Mtot;
+    const unsigned* ent = L.ent + (size_t)img * L.cap;
+    const float* val = L.val + (size_t)img * L.cap;
+    auto get = [&](size_t i, double& d, int& m, int& t) -> bool {
+        coarse_mt(ent[i], magic, Mtot, m, t);
+        if (!(wp.pdf_m[m] > 0.)) return false;
+        d = (double)val[i];
+        return true;
+    };
+    const AllPre pre{(size_t)cnt};
+    auto key = [&](size_t, int m, int t) { return (long long)m * T + t; };
+    weight_pass(img, (size_t)cnt, C2c[img], wp, get, pre, key, wo, sh, sigmask, TW, cand, emin_key);
+}
+
+// reach[img] = kWin + (max - min of ln pdf_offset over the image's (class, translation) pairs) + 1
+// nat of slack (+inf when a pdf_offset is zero: the list then takes every entry and overflows).
+__global__ void k_list_reach(WeightParams wp, float* __restrict__ reach) {
+    __shared__ double red[16];
+    const int img = blockIdx.x;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int q = threadIdx.x; q < wp.K * wp.T; q += blockDim.x) {
+        const int k = q / wp.T, t = q - k * wp.T;
+        const double po = pdf_offset(wp.trans_x[t], wp.trans_y[t], wp.old_offx[img], wp.old_offy[img], wp.prior_x[k],
+                                     wp.prior_y[k], wp.sigma2_offset);
+        const double l = po > 0 ? log(po) : -INFINITY;
+        lo = fmin(lo, l);
+        hi = fmax(hi, l);
+    }
+    lo = block_reduce(lo, [](double a, double b) { return fmin(a, b); }, red);
+    hi = block_reduce(hi, [](double a, double b) { return fmax(a, b); }, red);
+    if (threadIdx.x == 0) reach[img] = lo > -INFINITY ? (float)(kWin + (hi - lo) + 1.0) : INFINITY;
+}
+
 __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
                                                        const int* __restrict__ blk_task, const int* __restrict__ blk_t,
                                                        const int* __restrict__ task_m,
@@ -461,6 +508,21 @@ hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, c
                                  const CandArgs& c) {
     WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
     k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, C2c, emin_key, o, sigmask, TW, Cand{c.w, c.du, c.e, c.cap});
+    return hipGetLastError();
+}
+
+hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, const ListArgs& list, const double* C2c,
+                               const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                               const CandArgs& c) {
+    if (wp.K * wp.T > kMaxKT) return hipErrorInvalidValue;
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    k_weights_list<<<n, 1024, 0, s>>>(wp, list, C2c, emin_key, o, sigmask, TW, Cand{c.w, c.du, c.e, c.cap});
+    return hipGetLastError();
+}
+
+hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float* reach) {
+    if (n == 0) return hipSuccess;
+    k_list_reach<<<n, 256, 0, s>>>(wp, reach);
     return hipGetLastError();
 }
 

@@ -20,7 +20,7 @@ import pytest
 import oracle as O
 from rome_amd import synth
 
-from test_gpu_parity import _case
+from test_gpu_parity import _case, _run_both
 
 pytestmark = pytest.mark.gpu
 
@@ -46,13 +46,11 @@ def test_mixed_precision_estep(engine, case, prec):
     name, prob, batch, ref = case
     print(f"[mixed case {name} {prec}] n={batch['n']} D={prob['nr_dir']} PSI={prob['nr_psi']} "
           f"T={prob['nr_trans']}", flush=True)
-    synth.load_engine(engine, prob, precision=prec)
-    r = engine.expectation(batch)
-    s = engine.get_sums()
+    r, s, d = _run_both(engine, prob, batch, prec)
     n = batch["n"]
     u = U[prec]
     # coarse pass
-    d = engine.get_coarse_diff2(n).reshape(n, -1)
+    d = d.reshape(n, -1)
     o = ref["coarse_diff2"].reshape(n, -1)
     assert np.array_equal(np.isnan(d), np.isnan(o))
     worst = 0.0

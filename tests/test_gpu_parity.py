@@ -62,10 +62,40 @@ def case(request):
     return request.param, prob, batch, ref
 
 
-def _run(engine, prob, batch):
-    synth.load_engine(engine, prob)
+def _run(engine, prob, batch, precision="fp32"):
+    synth.load_engine(engine, prob, precision=precision)
     r = engine.expectation(batch)
     return r, engine.get_sums()
+
+
+def _run_both(engine, prob, batch, precision="fp32"):
+    """The batch with the whole coarse pose term in HBM (ROME_COARSE_FULL: its coarse diff2 is readable),
+    then through the product path, the coarse list (the default); the list must give the same E-step:
+    it holds every entry whose weight reaches the sums."""
+    n = batch["n"]
+    engine.set_coarse_mode("full")
+    try:
+        rf, sf = _run(engine, prob, batch, precision)
+        d = engine.get_coarse_diff2(n)
+        assert engine.coarse_stats()["mode"] == 1
+    finally:
+        engine.set_coarse_mode("list")
+    r, s = _run(engine, prob, batch, precision)
+    st = engine.coarse_stats()
+    assert st["mode"] == 0 and 0 < st["list_max"], st
+    _check_same(r, s, rf, sf)
+    return r, s, d
+
+
+def _check_same(r, s, rf, sf):
+    """Coarse list vs full pose term: the same E values and the same window, so the same poses and
+    significant sets; the weight sums differ only in FP64 summation order."""
+    assert np.array_equal(r["argmax"], rf["argmax"])
+    assert np.array_equal(r["nr_sign"], rf["nr_sign"])
+    np.testing.assert_allclose(r["min_diff2_coarse"], rf["min_diff2_coarse"], rtol=0, atol=0)
+    np.testing.assert_allclose(r["meta"], rf["meta"], rtol=1e-10, atol=1e-12)
+    for k in sf:
+        np.testing.assert_allclose(s[k], sf[k], rtol=1e-10, atol=1e-12, err_msg=k)
 
 
 def test_projector_matches_oracle(engine):
@@ -83,10 +113,9 @@ def test_projector_matches_oracle(engine):
 
 def test_estep_parity(engine, case):
     name, prob, batch, ref = case
-    r, s = _run(engine, prob, batch)
+    r, s, d = _run_both(engine, prob, batch)
     n = batch["n"]
     # coarse pass
-    d = engine.get_coarse_diff2(n)
     o = ref["coarse_diff2"]
     assert np.array_equal(np.isnan(d), np.isnan(o))
     m = ~np.isnan(o)
@@ -147,6 +176,27 @@ def _check_sums(r, s, ref, prob, engine, name):
             continue
         assert np.linalg.norm(dg - od) <= 1e-5 * np.linalg.norm(od), name
         assert np.linalg.norm(wg - ow) <= 1e-5 * np.linalg.norm(ow), name
+
+
+def test_list_overflow_reruns_full(engine):
+    """A coarse list past its cap sends the pool through the full pose term: same E-step."""
+    prob, batch, _ = _case("3d_window_groups_offsets")
+    engine.set_coarse_mode("full")
+    try:
+        rf, sf = _run(engine, prob, batch)
+    finally:
+        engine.set_coarse_mode("list")
+    engine.set_coarse_mode("list", 4)
+    try:
+        r, s = _run(engine, prob, batch)
+        st = engine.coarse_stats()
+    finally:
+        engine.set_coarse_mode("list")
+    assert st["mode"] == 2 and st["list_max"] > 4, st
+    assert np.array_equal(r["argmax"], rf["argmax"]) and np.array_equal(r["nr_sign"], rf["nr_sign"])
+    np.testing.assert_array_equal(r["meta"], rf["meta"])
+    for k in sf:  # FP64 atomic accumulation: order-dependent last bits only
+        np.testing.assert_allclose(s[k], sf[k], rtol=1e-12, atol=1e-15, err_msg=k)
 
 
 def test_update_model_matches_oracle(engine, case):
