@@ -93,7 +93,7 @@ struct rome_ctx {
     DevBuf<int> emin_key;
     // coarse list (rome_set_coarse_mode): entries within reach of the weights, per image
     int coarse_mode = ROME_COARSE_LIST;
-    int list_cap = 0;  // 0: max(65536, a quarter of the image's coarse entries)
+    int list_cap = 0;  // 0: all of an image's coarse entries if the pool's lists fit in 1 GiB, else max(65536, a quarter)
     DevBuf<float> lpd_row, reach, list_val;  // lpd_row [Mpad]: ln pdf_direction per rotation (-inf: none)
     DevBuf<int> vbound, list_cnt;
     DevBuf<unsigned> list_ent;
@@ -448,8 +448,12 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
         ListArgs L{};
         if (list) {
+            // every entry when the pool's lists fit in 1 GiB, else a quarter of them (a list past its
+            // cap sends the pool through the full pose term)
             const size_t per = (size_t)M * T;
-            const int cap = ctx->list_cap > 0 ? ctx->list_cap : (int)std::max<size_t>(65536, (per + 3) / 4);
+            const int cap = ctx->list_cap > 0                       ? ctx->list_cap
+                            : (size_t)n * per * 8 <= ((size_t)1 << 30) ? (int)per
+                                                                      : (int)std::max<size_t>(65536, (per + 3) / 4);
             HIPCHK(ctx->vbound.reserve(n));
             HIPCHK(ctx->list_cnt.reserve(n));
             HIPCHK(ctx->list_ent.reserve((size_t)n * cap));
