@@ -30,6 +30,22 @@ sys.path.insert(0, ROOT)
 
 MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # MI355X dense matrix peaks
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP16 cross-term kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# summarised (with their commands and copy-probe calibration) in this file
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_map3d_128_k4_hp2_t81_fp16.md")
+
+
+def measured_traffic(kernel_row):
+    """HBM read + written bytes of one launch from the committed counter summary (None if absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if kernel_row in line:
+                    cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                    return int(round((float(cells[3]) + float(cells[4])) * 2 ** 30))
+    except (OSError, ValueError, IndexError):
+        return None
+    return None
 
 
 def build_problem(args):
@@ -225,9 +241,13 @@ def main():
     kname = {"fp32": "k_coarse_gemm (coarse cross term, FP32 MFMA)",
              "bf16": "k_gemm_lowp<bf16> (coarse cross term, BF16 MFMA)",
              "fp16": "k_gemm_lowp<fp16> (coarse cross term, FP16 MFMA)"}[head]
+    traffic = measured_traffic("k_gemm_lowp<_Float16, list>") if head == "fp16" and stats.get("mode") == 0 else None
     roofline = {"kernel": kname, "bound": "mfma",
                 "achieved": round(gemm_tflops, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(gemm_tflops / peak, 4), "traffic": None,
+                "frac": round(gemm_tflops / peak, 4), "traffic": traffic,
+                "traffic_note": ("HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, rocprofv3, "
+                                 "profiles/hbm_traffic_map3d_128_k4_hp2_t81_fp16.md); algorithmic: 0.31 GiB "
+                                 "operands + the coarse lists") if traffic else None,
                 "avg_launch_ms": round(gemm_ms / max(gemm_calls, 1), 3), "dominant_stage": dom}
     cpu = None
     if not args.no_cpu_baseline:
