@@ -6,13 +6,17 @@
 //             V_s(p) = sum_t w_t/sumw * Minv CTF X_nomask(p) S_t(p) (float2 [S][Pf]) and
 //             W_s = sum_t w_t/sumw (double [S]; 0: the slice has no significant sample).
 //   count   : every (slice, pixel) with a positive weight, inside r_max -> trilinear base voxel ->
-//             brick (8x8x8 base voxels of one class grid); counts per brick.
-//   scan    : exclusive prefix sum of the counts (one block).
-//   fill    : records (slice << 32 | pixel) grouped by brick.
-//   brick   : one block per brick: the records' 8 corners accumulate in a 9x9x9 LDS box (base voxels
-//             plus the +1 corner halo), then each non-zero voxel goes to the grids with one atomic
-//             per component.  Corner weights and values are computed exactly as the direct scatter
-//             computes them; only the order of the FP32 additions differs.
+//             brick (8x8x8 base voxels of one class grid); counts per brick, first gathered per
+//             block in an LDS hash table (one global atomic per distinct brick of the block).
+//   scan    : exclusive prefix sums of the counts and of the chunks (kChunk records) per brick.
+//   fill    : records (slice << 32 | pixel) grouped by brick; ranges reserved per distinct brick
+//             of the block as in count.
+//   brick   : one block per chunk of a brick's records (bricks near the origin, crossed by nearly
+//             every slice, hold hundreds of thousands): the records' 8 corners accumulate in a
+//             9x9x9 LDS box (base voxels plus the +1 corner halo), then each non-zero voxel goes to
+//             the grids with one atomic per component.  Corner weights and values are computed
+//             exactly as the direct scatter computes them; only the order of the FP32 additions
+//             differs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -24,6 +28,23 @@ namespace rome {
 namespace {
 
 constexpr int kBrick = 8, kBox = kBrick + 1, kBox3 = kBox * kBox * kBox;
+constexpr unsigned kChunk = 2048;  // records per brick block
+constexpr int kHash = 512;         // LDS hash slots for the <= 256 bricks of one count / fill block
+
+// Slot of brick b in the block's hash table (keys b + 1, 0 = empty; linear probing).  At most 256
+// keys go into 512 slots, so a free slot always exists.
+DEV int hash_slot(unsigned* keys, unsigned b) {
+    int h = (int)((b * 2654435761u) >> 23);  // 9 bits
+    for (;;) {
+        const unsigned k = keys[h];
+        if (k == b + 1) return h;
+        if (k == 0) {
+            const unsigned old = atomicCAS(&keys[h], 0u, b + 1);
+            if (old == 0 || old == b + 1) return h;
+        }
+        h = (h + 1) & (kHash - 1);
+    }
+}
 
 struct Sample {
     Coords c;
@@ -57,56 +78,98 @@ DEV int brick_of(const BpArgs& a, const Sample& sm) {
 }
 
 __global__ __launch_bounds__(256) void k_bp_count(BpArgs a) {
+    __shared__ unsigned keys[kHash], cnt[kHash];
+    for (int h = threadIdx.x; h < kHash; h += blockDim.x) {
+        keys[h] = 0u;
+        cnt[h] = 0u;
+    }
+    __syncthreads();
     const long long s = blockIdx.x;
     const int p = blockIdx.y * blockDim.x + threadIdx.x;
-    if (p >= a.Pf) return;
     Sample sm;
-    if (!bp_sample(a, s, p, sm)) return;
-    atomicAdd(&a.count[brick_of(a, sm)], 1u);
+    if (p < a.Pf && bp_sample(a, s, p, sm)) atomicAdd(&cnt[hash_slot(keys, (unsigned)brick_of(a, sm))], 1u);
+    __syncthreads();
+    for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+        if (keys[h]) atomicAdd(&a.count[keys[h] - 1], cnt[h]);
 }
 
-// Exclusive scan of count[0..nb) into off[0..nb], cursor cleared.  One block of 1024 threads.
+// Exclusive scans of count[0..nb) into off[0..nb] and of the chunks per brick into coff[0..nb];
+// cursor cleared.  One block of 1024 threads.
 __global__ __launch_bounds__(1024) void k_bp_scan(const unsigned* __restrict__ count, int nb,
-                                                  unsigned* __restrict__ off, unsigned* __restrict__ cursor) {
-    __shared__ unsigned part[1024];
+                                                  unsigned* __restrict__ off, unsigned* __restrict__ coff,
+                                                  unsigned* __restrict__ cursor) {
+    __shared__ unsigned part[1024], cpart[1024];
     const int tid = threadIdx.x;
     const int chunk = (nb + 1023) / 1024;
     const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
-    unsigned loc = 0;
-    for (int b = b0; b < b1; ++b) loc += count[b];
+    unsigned loc = 0, cloc = 0;
+    for (int b = b0; b < b1; ++b) {
+        loc += count[b];
+        cloc += (count[b] + kChunk - 1) / kChunk;
+    }
     part[tid] = loc;
+    cpart[tid] = cloc;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan of the per-thread sums
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scans of the per-thread sums
         const unsigned v = tid >= d ? part[tid - d] : 0u;
+        const unsigned cv = tid >= d ? cpart[tid - d] : 0u;
         __syncthreads();
         part[tid] += v;
+        cpart[tid] += cv;
         __syncthreads();
     }
-    unsigned run = part[tid] - loc;
+    unsigned run = part[tid] - loc, crun = cpart[tid] - cloc;
     for (int b = b0; b < b1; ++b) {
         off[b] = run;
+        coff[b] = crun;
         cursor[b] = 0u;
         run += count[b];
+        crun += (count[b] + kChunk - 1) / kChunk;
     }
-    if (tid == 1023) off[nb] = part[1023];
+    if (tid == 1023) {
+        off[nb] = part[1023];
+        coff[nb] = cpart[1023];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bp_fill(BpArgs a) {
+    __shared__ unsigned keys[kHash], cnt[kHash], base[kHash];
+    for (int h = threadIdx.x; h < kHash; h += blockDim.x) {
+        keys[h] = 0u;
+        cnt[h] = 0u;
+    }
+    __syncthreads();
     const long long s = blockIdx.x;
     const int p = blockIdx.y * blockDim.x + threadIdx.x;
-    if (p >= a.Pf) return;
     Sample sm;
-    if (!bp_sample(a, s, p, sm)) return;
-    const int b = brick_of(a, sm);
-    const unsigned slot = a.off[b] + atomicAdd(&a.cursor[b], 1u);
-    a.rec[slot] = ((unsigned long long)s << 32) | (unsigned)p;
+    int h = -1;
+    unsigned rank = 0;
+    if (p < a.Pf && bp_sample(a, s, p, sm)) {
+        h = hash_slot(keys, (unsigned)brick_of(a, sm));
+        rank = atomicAdd(&cnt[h], 1u);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kHash; q += blockDim.x)
+        if (keys[q]) base[q] = a.off[keys[q] - 1] + atomicAdd(&a.cursor[keys[q] - 1], cnt[q]);
+    __syncthreads();
+    if (h >= 0) a.rec[base[h] + rank] = ((unsigned long long)s << 32) | (unsigned)p;
 }
 
-__global__ __launch_bounds__(256) void k_bp_brick(BpArgs a) {
+__global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
     __shared__ float box[3][kBox3];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const unsigned r0 = a.off[b], r1 = a.off[b + 1];
-    if (r0 == r1) return;
+    const unsigned ck = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (ck >= a.coff[nb]) return;  // the grid is sized for the most chunks there can be
+    // brick b with coff[b] <= k < coff[b + 1] (bricks without records hold no chunk)
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.coff[mid] <= ck) lo = mid;
+        else hi = mid - 1;
+    }
+    const int b = lo;
+    const unsigned r0 = a.off[b] + (ck - a.coff[b]) * kChunk, r1 = min(a.off[b + 1], r0 + kChunk);
+    if (r0 >= r1) return;
     const int k = b / a.nb_class;
     const int rem = b - k * a.nb_class;
     const int bx = rem % a.nbx, by = (rem / a.nbx) % a.nby, bz = rem / (a.nbx * a.nby);
@@ -160,9 +223,11 @@ hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a
     if (e != hipSuccess) return e;
     const dim3 grid((unsigned)S, (a.Pf + 255) / 256);
     k_bp_count<<<grid, 256, 0, s>>>(a);
-    k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.cursor);
+    k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
     k_bp_fill<<<grid, 256, 0, s>>>(a);
-    k_bp_brick<<<nb, 256, 0, s>>>(a);
+    // at most one chunk per brick plus one per kChunk records
+    const unsigned long long chunks = (unsigned long long)nb + ((unsigned long long)S * a.Pf + kChunk - 1) / kChunk;
+    k_bp_brick<<<(unsigned)chunks, 256, 0, s>>>(a, nb);
     return hipGetLastError();
 }
 

@@ -136,7 +136,7 @@ struct rome_ctx {
     static constexpr size_t kBinnedBytes = (size_t)24 << 30;  // slice values + records per batch
     DevBuf<float2> slice_val;
     DevBuf<double> slice_w;
-    DevBuf<unsigned> bp_count, bp_off, bp_cursor;
+    DevBuf<unsigned> bp_count, bp_off, bp_coff, bp_cursor;
     DevBuf<unsigned long long> bp_rec;
     // hipFFT plans by (type, rank, size), kept for the context's lifetime
     struct FftPlan {
@@ -673,6 +673,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             const size_t nb = (size_t)ba.nb_class * ctx->K;
             HIPCHK(ctx->bp_count.reserve(nb));
             HIPCHK(ctx->bp_off.reserve(nb + 1));
+            HIPCHK(ctx->bp_coff.reserve(nb + 1));
             HIPCHK(ctx->bp_cursor.reserve(nb));
             HIPCHK(ctx->bp_rec.reserve((size_t)S * ctx->Pf));
             ba.task_img = ctx->task_img.p;
@@ -684,6 +685,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             ba.slice_val = ctx->slice_val.p;
             ba.count = ctx->bp_count.p;
             ba.off = ctx->bp_off.p;
+            ba.coff = ctx->bp_coff.p;
             ba.cursor = ctx->bp_cursor.p;
             ba.rec = ctx->bp_rec.p;
             ba.bp_data = ctx->bp_data.p;

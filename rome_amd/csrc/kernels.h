@@ -226,6 +226,7 @@ struct BpArgs {
     const float2* slice_val;      // [S][Pf]
     unsigned* count;              // [nb]
     unsigned* off;                // [nb + 1]
+    unsigned* coff;               // [nb + 1] chunks of the brick kernel per brick, prefix sums
     unsigned* cursor;             // [nb]
     unsigned long long* rec;      // [records] slice << 32 | pixel
     float* bp_data;
