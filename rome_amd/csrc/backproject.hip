@@ -155,12 +155,20 @@ __global__ __launch_bounds__(256) void k_bp_fill(BpArgs a) {
     if (h >= 0) a.rec[base[h] + rank] = ((unsigned long long)s << 32) | (unsigned)p;
 }
 
+// One chunk of a brick's records without floating-point atomics: the records are counting-sorted by
+// base voxel in LDS, each thread sums the 8 corner contributions of the records of its base voxels in
+// registers, then the corners are added to the box in 8 passes, one corner offset per pass, in which
+// distinct base voxels reach distinct box voxels.
 __global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
+    constexpr int kVox = kBrick * kBrick * kBrick;  // base voxels of a brick
     __shared__ float box[3][kBox3];
+    __shared__ unsigned vcnt[kVox], voff[kVox + 1];
+    __shared__ unsigned short order[kChunk], vox[kChunk];
+    __shared__ float rec_f[6][kChunk];  // fx, fy, fz, value re, value im (mirror applied), weight
     const unsigned ck = blockIdx.x;
     const int tid = threadIdx.x;
     if (ck >= a.coff[nb]) return;  // the grid is sized for the most chunks there can be
-    // brick b with coff[b] <= k < coff[b + 1] (bricks without records hold no chunk)
+    // brick b with coff[b] <= ck < coff[b + 1] (bricks without records hold no chunk)
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -170,34 +178,95 @@ __global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
     const int b = lo;
     const unsigned r0 = a.off[b] + (ck - a.coff[b]) * kChunk, r1 = min(a.off[b + 1], r0 + kChunk);
     if (r0 >= r1) return;
+    const int n = (int)(r1 - r0);
     const int k = b / a.nb_class;
     const int rem = b - k * a.nb_class;
     const int bx = rem % a.nbx, by = (rem / a.nbx) % a.nby, bz = rem / (a.nbx * a.nby);
     for (int v = tid; v < 3 * kBox3; v += blockDim.x) (&box[0][0])[v] = 0.f;
+    for (int v = tid; v < kVox; v += blockDim.x) vcnt[v] = 0u;
     __syncthreads();
-    for (unsigned i = r0 + tid; i < r1; i += blockDim.x) {
-        const unsigned long long rc = a.rec[i];
+    // records -> LDS (geometry and values), base voxel histogram
+    for (int i = tid; i < n; i += blockDim.x) {
+        const unsigned long long rc = a.rec[r0 + i];
         const long long s = (long long)(rc >> 32);
         const int p = (int)(rc & 0xffffffffu);
         Sample sm;
-        if (!bp_sample(a, s, p, sm)) continue;  // cannot happen: the record was made from the same test
+        bp_sample(a, s, p, sm);  // true: the record was made from the same test
         const Coords& c = sm.c;
         const float2 v = a.slice_val[(size_t)s * a.Pf + p];
-        const float vr = v.x, vi = c.neg ? -v.y : v.y;
-        const float mfx = 1.f - c.fx, mfy = 1.f - c.fy, mfz = 1.f - c.fz;
-        const float dd[8] = {mfz * mfy * mfx, mfz * mfy * c.fx, mfz * c.fy * mfx, mfz * c.fy * c.fx,
-                             c.fz * mfy * mfx, c.fz * mfy * c.fx, c.fz * c.fy * mfx, c.fz * c.fy * c.fx};
         const int lx = c.x0 - bx * kBrick, ly = c.y0 - by * kBrick, lz = c.z0 - bz * kBrick;
-#pragma unroll
-        for (int cc = 0; cc < 8; ++cc) {
-            const int dz = cc >> 2, dy = (cc >> 1) & 1, dx = cc & 1;
-            const int idx = ((lz + dz) * kBox + (ly + dy)) * kBox + (lx + dx);
-            atomicAdd(&box[0][idx], dd[cc] * vr);
-            atomicAdd(&box[1][idx], dd[cc] * vi);
-            atomicAdd(&box[2][idx], dd[cc] * sm.wgt);
-        }
+        const int lv = (lz * kBrick + ly) * kBrick + lx;
+        vox[i] = (unsigned short)lv;
+        rec_f[0][i] = c.fx;
+        rec_f[1][i] = c.fy;
+        rec_f[2][i] = c.fz;
+        rec_f[3][i] = v.x;
+        rec_f[4][i] = c.neg ? -v.y : v.y;
+        rec_f[5][i] = sm.wgt;
+        atomicAdd(&vcnt[lv], 1u);
     }
     __syncthreads();
+    if (tid < 64) {  // exclusive scan of the 512 counts by one wave, 8 per lane
+        unsigned loc[8], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            loc[q] = vcnt[tid * 8 + q];
+            sum += loc[q];
+        }
+        unsigned incl = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned o = __shfl_up(incl, d, 64);
+            if (tid >= d) incl += o;
+        }
+        unsigned run = incl - sum;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            voff[tid * 8 + q] = run;
+            vcnt[tid * 8 + q] = run;  // becomes the scatter cursor
+            run += loc[q];
+        }
+        if (tid == 63) voff[kVox] = run;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) order[atomicAdd(&vcnt[vox[i]], 1u)] = (unsigned short)i;
+    __syncthreads();
+    // per base voxel (this thread: tid and tid + 256): sums of the 8 corner contributions
+    float acc[2][8][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int lv = tid + 256 * h;
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) acc[h][cc][0] = acc[h][cc][1] = acc[h][cc][2] = 0.f;
+        for (unsigned j = voff[lv]; j < voff[lv + 1]; ++j) {
+            const int i = order[j];
+            const float fx = rec_f[0][i], fy = rec_f[1][i], fz = rec_f[2][i];
+            const float vr = rec_f[3][i], vi = rec_f[4][i], w = rec_f[5][i];
+            const float mfx = 1.f - fx, mfy = 1.f - fy, mfz = 1.f - fz;
+            const float dd[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
+                                 fz * mfy * mfx, fz * mfy * fx, fz * fy * mfx, fz * fy * fx};
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) {
+                acc[h][cc][0] += dd[cc] * vr;
+                acc[h][cc][1] += dd[cc] * vi;
+                acc[h][cc][2] += dd[cc] * w;
+            }
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+        const int dz = cc >> 2, dy = (cc >> 1) & 1, dx = cc & 1;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int lv = tid + 256 * h;
+            const int lx = lv & 7, ly = (lv >> 3) & 7, lz = lv >> 6;
+            const int idx = ((lz + dz) * kBox + (ly + dy)) * kBox + (lx + dx);
+            box[0][idx] += acc[h][cc][0];
+            box[1][idx] += acc[h][cc][1];
+            box[2][idx] += acc[h][cc][2];
+        }
+        __syncthreads();
+    }
     float* gd = a.bp_data + a.grid_stride * 2 * k;
     float* gw = a.bp_weight + a.grid_stride * k;
     for (int v = tid; v < kBox3; v += blockDim.x) {
