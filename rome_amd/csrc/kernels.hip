@@ -545,7 +545,9 @@ __global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
 //   pdfs   : sumw_group, pdf_class, pdf_direction, sigma2_offset           src/map3d_optimizer.cpp:2141-2177
 //   bp     : slice(rot) += w * shift(Nomask*Minv*CTF), weight += w Minv CTF^2, then trilinear
 //            scatter into the class grid                                      src/map3d_optimizer.cpp:2360-2479
-template <int NOR, int NOT, int PPT, int NT>
+// BINNED: 3D, the slices go to HBM for the binned backprojection (the image's own pixel values stay in
+// registers); otherwise the slices are scattered into the grids here.
+template <int NOR, int NOT, int PPT, int NT, bool BINNED>
 __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
     constexpr int NE = NOR * NOT;
     __shared__ double wts[kMaxT * NE];
@@ -568,6 +570,15 @@ __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
     const float2* NM = a.NMf + (size_t)img * a.Pf;
     const float* MC2 = a.MC2bp + (size_t)img * a.Pf;
     const double ox = a.old_offx[img], oy = a.old_offy[img];
+    float2 Xr[BINNED ? PPT : 1], NMr[BINNED ? PPT : 1];
+    if constexpr (BINNED) {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int p = tid + j * NT;
+            Xr[j] = p < a.Pf ? X[p] : make_float2(0.f, 0.f);
+            NMr[j] = p < a.Pf ? NM[p] : make_float2(0.f, 0.f);
+        }
+    }
     for (long long q = q0; q < q1; ++q) {
         const int m = a.task_m[q], nt = a.task_nt[q];
         const long long b0 = a.task_blk0[q];
@@ -628,17 +639,17 @@ __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
                         const int p = tid + j * NT;
                         if (p < a.Pf) {
                             const float2 s = S[p];
-                            const float2 xs = cmul(X[p], s);
+                            const float2 xs = cmul(BINNED ? Xr[j] : X[p], s);
                             const float dr = CF[j].x - xs.x, di = CF[j].y - xs.y;
                             wsig[j] += wn * (double)(dr * dr + di * di);
-                            const float2 ns = cmul(NM[p], s);
+                            const float2 ns = cmul(BINNED ? NMr[j] : NM[p], s);
                             acc[j].x += wf * ns.x;
                             acc[j].y += wf * ns.y;
                         }
                     }
                 }
             }
-            if (a.slice_val) {  // binned 3D backprojection: the slice goes to HBM, the grids later
+            if constexpr (BINNED) {  // binned 3D backprojection: the slice goes to HBM, the grids later
                 const size_t sl = (size_t)q * NOR + o;
                 if (wslice > 0.) {
                     float2* V = a.slice_val + sl * a.Pf;
@@ -878,10 +889,18 @@ hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const F
 
 template <int NOR, int NOT>
 static hipError_t post_ppt(hipStream_t s, int n, int Pf, const PostArgs& a) {
-    if (Pf <= 4 * 256) k_post<NOR, NOT, 4, 256><<<n, 256, 0, s>>>(a);
-    else if (Pf <= 4 * 1024) k_post<NOR, NOT, 4, 1024><<<n, 1024, 0, s>>>(a);
-    else if (Pf <= 8 * 1024) k_post<NOR, NOT, 8, 1024><<<n, 1024, 0, s>>>(a);
-    else if (Pf <= 16 * 1024) k_post<NOR, NOT, 16, 1024><<<n, 1024, 0, s>>>(a);
+    if (a.slice_val) {  // 512 threads: room in registers for the image's own pixels
+        if (Pf <= 4 * 256) k_post<NOR, NOT, 4, 256, true><<<n, 256, 0, s>>>(a);
+        else if (Pf <= 8 * 512) k_post<NOR, NOT, 8, 512, true><<<n, 512, 0, s>>>(a);
+        else if (Pf <= 16 * 512) k_post<NOR, NOT, 16, 512, true><<<n, 512, 0, s>>>(a);
+        else if (Pf <= 16 * 1024) k_post<NOR, NOT, 16, 1024, true><<<n, 1024, 0, s>>>(a);
+        else return hipErrorInvalidValue;
+        return hipSuccess;
+    }
+    if (Pf <= 4 * 256) k_post<NOR, NOT, 4, 256, false><<<n, 256, 0, s>>>(a);
+    else if (Pf <= 4 * 1024) k_post<NOR, NOT, 4, 1024, false><<<n, 1024, 0, s>>>(a);
+    else if (Pf <= 8 * 1024) k_post<NOR, NOT, 8, 1024, false><<<n, 1024, 0, s>>>(a);
+    else if (Pf <= 16 * 1024) k_post<NOR, NOT, 16, 1024, false><<<n, 1024, 0, s>>>(a);
     else return hipErrorInvalidValue;
     return hipSuccess;
 }
