@@ -75,6 +75,47 @@ typedef struct {
     const int* group;
 } rome_device_images;
 
+/* ---- Image preparation of a pool: ParticleModel::setFourierTransforms (src/exp_model.h:397-639) ----
+ * From real-space particle images [n][ori][ori] (row y, then x): norm correction (x avg_norm_correction
+ * / NORM when do_norm_correction), translation by the rounded old offsets without wrapping, centred FFT
+ * (divided by ori^2), window to current_size (unmasked transform); soft mask of radius
+ * particle_diameter / (2 pixel_size) with a raised cosine over width_mask_edge pixels, filled with
+ * `noise` (the background image the reference draws from its random generator) or, when noise is
+ * NULL (-zero_mask), with the weighted background average; masked transform; power spectrum and
+ * highres_Xi2 (when current_size < ori_size, else zero); CTF image (CTF::getFftwImage; all ones
+ * without do_ctf_correction).  FP64 on the device with hipFFT.  Sizes and flags from rome_config. */
+typedef struct {
+    int n;
+    const double* images;       /* [n][ori][ori] */
+    const double* norm;         /* [n] NORM */
+    const double *xoff, *yoff;  /* [n] XOFF, YOFF (rounded here) */
+    const double* ctf;          /* [n][7] defocus U, defocus V (A), angle (deg), voltage (kV), Cs (mm), Q0, B-factor */
+    const double* noise;        /* [n][ori][ori] or NULL */
+} rome_raw_images;
+
+typedef struct {
+    double pixel_size, particle_diameter, width_mask_edge;
+    int ctf_phase_flipped, only_flip_phases, intact_ctf_first_peak;
+} rome_prep_params;
+
+/* Host outputs (FP64, the layout of rome_images; NULL fields are skipped). */
+typedef struct {
+    double *fmask_re, *fmask_im, *fnomask_re, *fnomask_im, *fctf; /* [n][cur*(cur/2+1)] */
+    double* highres_Xi2;                                        /* [n] */
+    double* power_img;                                          /* [n][ori/2+1] */
+    double *old_offx, *old_offy;                                /* [n] rounded */
+} rome_prepared;
+
+/* Device outputs (the layout of rome_device_images; NULL fields are skipped). */
+typedef struct {
+    void *fmask, *fnomask;      /* float2 [n][cur*(cur/2+1)] */
+    float* fctf;                /* [n][cur*(cur/2+1)] */
+    double *highres_Xi2, *power_img, *old_offx, *old_offy;
+} rome_prepared_device;
+
+int rome_prepare_images(rome_ctx* ctx, const rome_raw_images* in, const rome_prep_params* params,
+                        rome_prepared* host_out, rome_prepared_device* device_out);
+
 /* Per-image results (host, any pointer may be NULL).  meta columns follow the STAR
  * labels the reference writes (src/metadata.h:55): PMAX CLASS ROT TILT PSI XOFF YOFF
  * NORM DLL.  argmax = (class, dir, psi, over_rot, trans, over_trans) of the last pass. */

@@ -26,7 +26,7 @@ EXPORTED = (
     "rome_sampling_pose3d", "rome_sampling_pose2d", "rome_sampling_translations", "rome_sampling_shifts",
     "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
     "rome_get_batch_stats", "rome_maximization_other", "rome_update_model", "rome_fourier_map",
-    "rome_reconstruct", "rome_set_coarse_mode", "rome_get_coarse_stats",
+    "rome_reconstruct", "rome_set_coarse_mode", "rome_get_coarse_stats", "rome_prepare_images",
 )
 
 COARSE_LIST, COARSE_FULL = 0, 1
@@ -104,6 +104,26 @@ class rome_recon_params(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("max_iter_preweight", "do_map", "minres_map", "update_tau2_with_fsc",
                                         "is_whole_instead_of_half")] + \
                [(n, C.c_double) for n in ("tau2_fudge", "normalise")]
+
+
+class rome_raw_images(C.Structure):
+    _fields_ = [("n", C.c_int), ("images", _dp), ("norm", _dp), ("xoff", _dp), ("yoff", _dp), ("ctf", _dp),
+                ("noise", _dp)]
+
+
+class rome_prep_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("pixel_size", "particle_diameter", "width_mask_edge")] + \
+               [(n, C.c_int) for n in ("ctf_phase_flipped", "only_flip_phases", "intact_ctf_first_peak")]
+
+
+class rome_prepared(C.Structure):
+    _fields_ = [(n, _dp) for n in ("fmask_re", "fmask_im", "fnomask_re", "fnomask_im", "fctf", "highres_Xi2",
+                                   "power_img", "old_offx", "old_offy")]
+
+
+class rome_prepared_device(C.Structure):
+    _fields_ = [("fmask", C.c_void_p), ("fnomask", C.c_void_p), ("fctf", C.c_void_p), ("highres_Xi2", C.c_void_p),
+                ("power_img", C.c_void_p), ("old_offx", C.c_void_p), ("old_offy", C.c_void_p)]
 
 
 def lib():
@@ -358,6 +378,38 @@ class Engine:
         out = np.zeros(8, np.int64)
         self._check(lib().rome_get_batch_stats(self._h, _p(out, C.c_longlong)), "rome_get_batch_stats")
         return dict(zip(("n", "tasks", "blocks", "unique_rot", "Pc", "Pf", "M", "T"), out.tolist()))
+
+    def prepare_images(self, images, xoff, yoff, norm=None, ctf=None, noise=None, pixel_size=1.0,
+                       particle_diameter=None, width_mask_edge=5.0, ctf_phase_flipped=False, only_flip_phases=False,
+                       intact_ctf_first_peak=False, device_out=None):
+        """rome_prepare_images: the E-step inputs of a pool from real-space images [n][ori][ori] (dict of
+        FP64 host arrays in the layout `expectation` takes).  device_out: optional dict of device pointers
+        (fmask, fnomask, fctf, highres_Xi2, power_img, old_offx, old_offy) to fill as well."""
+        img = _f64(images)
+        n, ori = img.shape[0], img.shape[1]
+        cur = self.cfg["current_size"]
+        cF = cur * (cur // 2 + 1)
+        out = {k: np.zeros((n, cF)) for k in ("fmask_re", "fmask_im", "fnomask_re", "fnomask_im", "fctf")}
+        out.update(highres_Xi2=np.zeros(n), power_img=np.zeros((n, ori // 2 + 1)), old_offx=np.zeros(n),
+                   old_offy=np.zeros(n))
+        arrs = {"xoff": _f64(xoff), "yoff": _f64(yoff), "norm": None if norm is None else _f64(norm),
+                "ctf": None if ctf is None else _f64(ctf), "noise": None if noise is None else _f64(noise)}
+        raw = rome_raw_images(n, _p(img), _p(arrs["norm"]), _p(arrs["xoff"]), _p(arrs["yoff"]), _p(arrs["ctf"]),
+                              _p(arrs["noise"]))
+        if particle_diameter is None:
+            particle_diameter = ori * pixel_size
+        prm = rome_prep_params(pixel_size, particle_diameter, width_mask_edge, int(ctf_phase_flipped),
+                               int(only_flip_phases), int(intact_ctf_first_peak))
+        ho = rome_prepared(*[_p(out[k]) for k in ("fmask_re", "fmask_im", "fnomask_re", "fnomask_im", "fctf",
+                                                  "highres_Xi2", "power_img", "old_offx", "old_offy")])
+        do = None
+        if device_out is not None:
+            do = rome_prepared_device(*[C.c_void_p(device_out.get(k)) if device_out.get(k) else None
+                                        for k in ("fmask", "fnomask", "fctf", "highres_Xi2", "power_img", "old_offx",
+                                                  "old_offy")])
+        self._check(lib().rome_prepare_images(self._h, C.byref(raw), C.byref(prm), C.byref(ho),
+                                              C.byref(do) if do is not None else None), "rome_prepare_images")
+        return out
 
     def set_coarse_mode(self, mode, list_cap=0):
         """ROME_COARSE_LIST (keep only the entries within reach of the weights) or ROME_COARSE_FULL

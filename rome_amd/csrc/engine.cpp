@@ -140,9 +140,12 @@ struct rome_ctx {
     DevBuf<unsigned long long> bp_rec;
     // hipFFT plans by (type, rank, size), kept for the context's lifetime
     struct FftPlan {
-        int type, rank, n;
+        int type, rank, n, batch;
         hipfftHandle h;
     };
+    // image preparation (rome_prepare_images)
+    DevBuf<double> ip_img, ip_timg, ip_cbuf, ip_noise, ip_scal, ip_ctf, ip_fctf, ip_power;
+    DevBuf<double2> ip_F1, ip_F2, ip_fn, ip_fm;
     std::vector<FftPlan> fft_plans;
     // projector grids built on the device (rome_fourier_map)
     DevBuf<double> fm_vol, fm_Mc, fm_corr, fm_pw, fm_cnt;
@@ -1402,20 +1405,23 @@ int rome_update_model(rome_ctx* ctx, int install, rome_model* out) {
 }
 
 // cached hipFFT plan (type D2Z / Z2D, rank 2 or 3, n per axis) bound to the context's stream
-static int fft_plan(rome_ctx* ctx, hipfftType type, int rank, int n, hipfftHandle* out) {
+static int fft_plan(rome_ctx* ctx, hipfftType type, int rank, int n, hipfftHandle* out, int batch = 1) {
     for (auto& p : ctx->fft_plans)
-        if (p.type == (int)type && p.rank == rank && p.n == n) {
+        if (p.type == (int)type && p.rank == rank && p.n == n && p.batch == batch) {
             *out = p.h;
             return ROME_OK;
         }
     hipfftHandle h = 0;
-    const hipfftResult r = rank == 3 ? hipfftPlan3d(&h, n, n, n, type) : hipfftPlan2d(&h, n, n, type);
+    int dims[3] = {n, n, n};
+    const hipfftResult r = batch > 1 ? hipfftPlanMany(&h, rank, dims, nullptr, 1, 0, nullptr, 1, 0, type, batch)
+                           : rank == 3 ? hipfftPlan3d(&h, n, n, n, type)
+                                       : hipfftPlan2d(&h, n, n, type);
     if (r != HIPFFT_SUCCESS) return fail(ctx, ROME_ERR_HIP, "hipfftPlan (type %d, %d^%d) failed: %d", (int)type, n, rank, (int)r);
     if (hipfftSetStream(h, ctx->stream) != HIPFFT_SUCCESS) {
         (void)hipfftDestroy(h);
         return fail(ctx, ROME_ERR_HIP, "hipfftSetStream failed");
     }
-    ctx->fft_plans.push_back({(int)type, rank, n, h});
+    ctx->fft_plans.push_back({(int)type, rank, n, batch, h});
     *out = h;
     return ROME_OK;
 }
@@ -1632,5 +1638,131 @@ int rome_reconstruct(rome_ctx* ctx, int iclass, const rome_recon_params* prm, co
     }
     HIPCHK(hipMemcpyAsync(vol_out, ctx->rc_vol.p, nv * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    return ROME_OK;
+}
+
+int rome_prepare_images(rome_ctx* ctx, const rome_raw_images* in, const rome_prep_params* p, rome_prepared* ho,
+                        rome_prepared_device* dout) {
+    if (!ctx || !in || !p || in->n < 0 || (in->n > 0 && (!in->images || !in->xoff || !in->yoff)))
+        return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    const rome_config& c = ctx->cfg;
+    if (c.ref_dim != 2 && c.ref_dim != 3) return fail(ctx, ROME_ERR_ARG, "bad ref_dim");
+    if (c.do_norm_correction && !in->norm) return fail(ctx, ROME_ERR_ARG, "do_norm_correction needs NORM");
+    if (c.do_ctf_correction && !in->ctf) return fail(ctx, ROME_ERR_ARG, "do_ctf_correction needs CTF parameters");
+    if (!(p->pixel_size > 0.) || !(p->width_mask_edge > 0.)) return fail(ctx, ROME_ERR_ARG, "bad mask parameters");
+    if (in->n == 0) return ROME_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int ori = c.ori_size, cur = c.current_size, oriF = ori / 2 + 1, curF = cur / 2 + 1;
+    const size_t n2 = (size_t)ori * ori, nF = (size_t)ori * oriF, cF = (size_t)cur * curF;
+    const int chunk = 1024;  // images per device pass
+    for (int i0 = 0; i0 < in->n; i0 += chunk) {
+        const int n = std::min(chunk, in->n - i0);
+        HIPCHK(ctx->ip_img.upload(in->images + (size_t)i0 * n2, (size_t)n * n2, s));
+        // scalars: norm, xoff, yoff, old_offx, old_offy, hxi2 (6 x n)
+        HIPCHK(ctx->ip_scal.reserve((size_t)6 * chunk));
+        double* sc = ctx->ip_scal.p;
+        if (in->norm) HIPCHK(hipMemcpyAsync(sc, in->norm + i0, n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(sc + chunk, in->xoff + i0, n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(sc + 2 * chunk, in->yoff + i0, n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (in->noise) HIPCHK(ctx->ip_noise.upload(in->noise + (size_t)i0 * n2, (size_t)n * n2, s));
+        if (c.do_ctf_correction) HIPCHK(ctx->ip_ctf.upload(in->ctf + (size_t)7 * i0, (size_t)7 * n, s));
+        HIPCHK(ctx->ip_timg.reserve((size_t)n * n2));
+        HIPCHK(ctx->ip_cbuf.reserve((size_t)n * n2));
+        HIPCHK(ctx->ip_F1.reserve((size_t)n * nF));
+        HIPCHK(ctx->ip_F2.reserve((size_t)n * nF));
+        HIPCHK(ctx->ip_fn.reserve((size_t)n * cF));
+        HIPCHK(ctx->ip_fm.reserve((size_t)n * cF));
+        HIPCHK(ctx->ip_fctf.reserve((size_t)n * cF));
+        HIPCHK(ctx->ip_power.reserve((size_t)n * oriF));
+        ImagePrepArgs a{};
+        a.ori = ori;
+        a.cur = cur;
+        a.images = ctx->ip_img.p;
+        a.norm = sc;
+        a.avg_norm = c.avg_norm_correction;
+        a.do_norm = c.do_norm_correction ? 1 : 0;
+        a.xoff = sc + chunk;
+        a.yoff = sc + 2 * chunk;
+        a.old_offx = sc + 3 * chunk;
+        a.old_offy = sc + 4 * chunk;
+        a.timg = ctx->ip_timg.p;
+        a.cbuf = ctx->ip_cbuf.p;
+        a.noise = in->noise ? ctx->ip_noise.p : nullptr;
+        a.radius = p->particle_diameter / (2. * p->pixel_size);
+        a.cosine_width = p->width_mask_edge;
+        a.F1 = ctx->ip_F1.p;
+        a.F2 = ctx->ip_F2.p;
+        a.fnomask = ctx->ip_fn.p;
+        a.fmask = ctx->ip_fm.p;
+        a.fnomask32 = dout && dout->fnomask ? (float2*)dout->fnomask + (size_t)i0 * cF : nullptr;
+        a.fmask32 = dout && dout->fmask ? (float2*)dout->fmask + (size_t)i0 * cF : nullptr;
+        a.power = ctx->ip_power.p;
+        a.hxi2 = sc + 5 * chunk;
+        a.ctf = c.do_ctf_correction ? ctx->ip_ctf.p : nullptr;
+        a.angpix = p->pixel_size;
+        a.ctf_abs = p->ctf_phase_flipped ? 1 : 0;
+        a.ctf_flip = p->only_flip_phases ? 1 : 0;
+        a.intact_first_peak = p->intact_ctf_first_peak ? 1 : 0;
+        a.fctf = ctx->ip_fctf.p;
+        a.fctf32 = dout && dout->fctf ? dout->fctf + (size_t)i0 * cF : nullptr;
+        hipfftHandle plan;
+        int rc = fft_plan(ctx, HIPFFT_D2Z, 2, ori, &plan, n);
+        if (rc) return rc;
+        HIPCHK(launch_imageprep_pre(s, n, a));
+        hipfftResult fr = hipfftExecD2Z(plan, a.cbuf, reinterpret_cast<hipfftDoubleComplex*>(ctx->ip_F1.p));
+        if (fr != HIPFFT_SUCCESS) return fail(ctx, ROME_ERR_HIP, "hipfftExecD2Z failed: %d", (int)fr);
+        HIPCHK(launch_imageprep_mask(s, n, a));
+        fr = hipfftExecD2Z(plan, a.cbuf, reinterpret_cast<hipfftDoubleComplex*>(ctx->ip_F2.p));
+        if (fr != HIPFFT_SUCCESS) return fail(ctx, ROME_ERR_HIP, "hipfftExecD2Z failed: %d", (int)fr);
+        if (cur >= ori) {
+            HIPCHK(hipMemsetAsync(a.power, 0, (size_t)n * oriF * sizeof(double), s));
+            HIPCHK(hipMemsetAsync(a.hxi2, 0, (size_t)n * sizeof(double), s));
+        }
+        HIPCHK(launch_imageprep_post(s, n, a));
+        if (dout) {
+            auto d2d = [&](double* dst, const double* src, size_t cnt) -> hipError_t {
+                return dst ? hipMemcpyAsync(dst + (size_t)i0 * (cnt / n), src, cnt * sizeof(double),
+                                            hipMemcpyDeviceToDevice, s)
+                           : hipSuccess;
+            };
+            HIPCHK(d2d(dout->highres_Xi2, a.hxi2, n));
+            HIPCHK(d2d(dout->power_img, a.power, (size_t)n * oriF));
+            HIPCHK(d2d(dout->old_offx, a.old_offx, n));
+            HIPCHK(d2d(dout->old_offy, a.old_offy, n));
+        }
+        if (ho) {
+            std::vector<double2> fn, fm;
+            if (ho->fnomask_re || ho->fnomask_im) {
+                fn.resize((size_t)n * cF);
+                HIPCHK(hipMemcpyAsync(fn.data(), a.fnomask, fn.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+            }
+            if (ho->fmask_re || ho->fmask_im) {
+                fm.resize((size_t)n * cF);
+                HIPCHK(hipMemcpyAsync(fm.data(), a.fmask, fm.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+            }
+            auto d2h = [&](double* dst, const double* src, size_t cnt) -> hipError_t {
+                return dst ? hipMemcpyAsync(dst + (size_t)i0 * (cnt / n), src, cnt * sizeof(double),
+                                            hipMemcpyDeviceToHost, s)
+                           : hipSuccess;
+            };
+            HIPCHK(d2h(ho->fctf, a.fctf, (size_t)n * cF));
+            HIPCHK(d2h(ho->highres_Xi2, a.hxi2, n));
+            HIPCHK(d2h(ho->power_img, a.power, (size_t)n * oriF));
+            HIPCHK(d2h(ho->old_offx, a.old_offx, n));
+            HIPCHK(d2h(ho->old_offy, a.old_offy, n));
+            HIPCHK(hipStreamSynchronize(s));
+            for (size_t q = 0; q < fn.size(); ++q) {
+                if (ho->fnomask_re) ho->fnomask_re[(size_t)i0 * cF + q] = fn[q].x;
+                if (ho->fnomask_im) ho->fnomask_im[(size_t)i0 * cF + q] = fn[q].y;
+            }
+            for (size_t q = 0; q < fm.size(); ++q) {
+                if (ho->fmask_re) ho->fmask_re[(size_t)i0 * cF + q] = fm[q].x;
+                if (ho->fmask_im) ho->fmask_im[(size_t)i0 * cF + q] = fm[q].y;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(s));  // the staging buffers are reused by the next chunk
+    }
     return ROME_OK;
 }

@@ -233,6 +233,40 @@ struct BpArgs {
     float* bp_weight;
 };
 
+// Image preparation of a pool (setFourierTransforms), FP64; any output pointer may be null.
+struct ImagePrepArgs {
+    int ori, cur;
+    const double* images;            // [n][ori][ori] real space, row y then x
+    const double* norm;              // [n] NORM
+    double avg_norm;
+    int do_norm;
+    const double* xoff;              // [n] old offsets (rounded here)
+    const double* yoff;
+    double* old_offx;                // [n] rounded offsets out
+    double* old_offy;
+    double* timg;                    // [n][ori][ori] translated image
+    double* cbuf;                    // [n][ori][ori] centred FFT input
+    const double* noise;             // [n][ori][ori] background fill outside the mask, or null
+    double radius, cosine_width;     // mask: particle_diameter / (2 pixel_size), width_mask_edge
+    const double2* F1;               // [n][ori][ori/2+1] unnormalised transform of the unmasked image
+    const double2* F2;               // same, masked image
+    double2* fnomask;                // [n][cur][cur/2+1]
+    double2* fmask;
+    float2* fnomask32;
+    float2* fmask32;
+    double* power;                   // [n][ori/2+1]
+    double* hxi2;                    // [n]
+    const double* ctf;               // [n][7] defocus U, V, angle, voltage, Cs, Q0, B-factor (null: CTF 1)
+    double angpix;
+    int ctf_abs, ctf_flip, intact_first_peak;
+    double* fctf;                    // [n][cur][cur/2+1]
+    float* fctf32;
+};
+
+hipError_t launch_imageprep_pre(hipStream_t s, int n, const ImagePrepArgs& a);
+hipError_t launch_imageprep_mask(hipStream_t s, int n, const ImagePrepArgs& a);
+hipError_t launch_imageprep_post(hipStream_t s, int n, const ImagePrepArgs& a);
+
 struct StoreArgs {
     int n, curF, oriF, do_norm;
     double avg_norm;
