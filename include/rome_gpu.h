@@ -252,7 +252,8 @@ int rome_get_batch_stats(rome_ctx* ctx, long long* out8);
  * the image's minimum u (the only ones whose weights reach the sums), so the full pose term never
  * goes to HBM; an image with more than list_cap such entries sends the whole pool through the full
  * path again (same results).  ROME_COARSE_FULL: the whole pose term in HBM, needed by
- * rome_get_coarse_diff2.  list_cap: entries per image, <= 0 for the default (all of an image's
+ * rome_get_coarse_diff2; also what a single pass (adaptive_oversampling = 0) always uses, since its
+ * significant samples take their diff2 from it.  list_cap: entries per image, <= 0 for the default (all of an image's
  * coarse entries when the pool's lists fit in 1 GiB, else max(65536, a quarter of them)). */
 enum { ROME_COARSE_LIST = 0, ROME_COARSE_FULL = 1 };
 int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap);

@@ -417,7 +417,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     WeightParams wp = weight_params(ctx);
     stage_begin(ctx, 1);
     HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
-    if (ctx->coarse_mode == ROME_COARSE_LIST) {
+    if (ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass) {
         HIPCHK(ctx->reach.reserve(n));
         HIPCHK(launch_list_reach(s, n, wp, ctx->reach.p));
     }
@@ -504,7 +504,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(hipStreamSynchronize(s));
         return ROME_OK;
     };
-    const bool use_list = ctx->coarse_mode == ROME_COARSE_LIST;
+    // the single pass (no oversampling) takes its significant diff2 back from the full pose term
+    const bool use_list = ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass;
     rc = coarse(use_list);
     if (rc) return rc;
     ctx->last_list_max = 0;

@@ -82,7 +82,10 @@ def _run_both(engine, prob, batch, precision="fp32"):
         engine.set_coarse_mode("list")
     r, s = _run(engine, prob, batch, precision)
     st = engine.coarse_stats()
-    assert st["mode"] == 0 and 0 < st["list_max"], st
+    if prob["adaptive_oversampling"]:
+        assert st["mode"] == 0 and 0 < st["list_max"], st
+    else:  # the single pass takes its diff2 from the full pose term whatever the mode asked for
+        assert st["mode"] == 1, st
     _check_same(r, s, rf, sf)
     return r, s, d
 
@@ -176,6 +179,22 @@ def _check_sums(r, s, ref, prob, engine, name):
             continue
         assert np.linalg.norm(dg - od) <= 1e-5 * np.linalg.norm(od), name
         assert np.linalg.norm(wg - ow) <= 1e-5 * np.linalg.norm(ow), name
+
+
+def test_single_pass_in_a_fresh_context():
+    """A new context asked for the coarse list runs the single pass (2D, no oversampling) on the full
+    pose term, with the same results as the session's engine."""
+    import rome_amd
+    prob, batch, _ = _case("2d_single_pass")
+    e = rome_amd.Engine(0)
+    try:
+        r, s = _run(e, prob, batch)
+        assert e.coarse_stats()["mode"] == 1
+    finally:
+        e.close()
+    ref = O.estep(prob, batch)
+    np.testing.assert_allclose(r["meta"][:, 0], ref["meta"][:, 0], rtol=1e-3)
+    np.testing.assert_allclose(s["wsum_sigma2_noise"], ref["wsum_sigma2_noise"], rtol=1e-4, atol=1e-12)
 
 
 def test_list_overflow_reruns_full(engine):
