@@ -21,6 +21,28 @@
 
 namespace rome {
 
+// Pinned host memory (hipHostMalloc), grown on demand: read-backs and uploads go through it as one
+// DMA each instead of staged copies of pageable memory.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t reserve(size_t bytes) {
+        if (bytes <= n && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 64), hipHostMallocDefault);
+        if (e == hipSuccess) n = std::max<size_t>(bytes, 64);
+        return e;
+    }
+};
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -163,6 +185,9 @@ struct rome_ctx {
     double stage_ms[kStages] = {};
     long long stage_calls[kStages] = {};
     long long last_U = 0;
+    // batched read-backs / uploads (pack kernel + pinned staging)
+    DevBuf<unsigned char> rb_dev;
+    PinnedBuf rb_host, up_host;
     // last batch bookkeeping (parity hooks)
     int last_n = 0;
     long long last_blocks = 0, last_tasks = 0;
@@ -294,6 +319,43 @@ int ensure_ready(rome_ctx* ctx) {
                                       ctx->Mtot, ctx->fct_absmax.p, ctx->A16.p, K16));
         }
         ctx->fct_dirty = false;
+    }
+    return ROME_OK;
+}
+
+struct Seg {
+    const void* dev;
+    void* host;
+    size_t bytes;  // a multiple of 4
+};
+
+// Device arrays -> host arrays: one pack kernel and one copy to pinned memory per kPackSegs arrays;
+// returns once the data is on the host.  Segments with a null host pointer are skipped.
+int readback(rome_ctx* ctx, const std::vector<Seg>& segs) {
+    hipStream_t s = ctx->stream;
+    for (size_t g0 = 0; g0 < segs.size(); g0 += kPackSegs) {
+        PackArgs pa{};
+        size_t off = 0;
+        std::vector<const Seg*> used;
+        for (size_t g = g0; g < std::min(segs.size(), g0 + kPackSegs); ++g) {
+            if (!segs[g].host || !segs[g].bytes) continue;
+            pa.src[pa.nseg] = segs[g].dev;
+            pa.off[pa.nseg] = off;
+            pa.len[pa.nseg] = segs[g].bytes;
+            off += (segs[g].bytes + 15) & ~(size_t)15;
+            ++pa.nseg;
+            used.push_back(&segs[g]);
+        }
+        if (!pa.nseg) continue;
+        pa.off[pa.nseg] = off;
+        HIPCHK(ctx->rb_dev.reserve(off));
+        HIPCHK(ctx->rb_host.reserve(off));
+        pa.dst = ctx->rb_dev.p;
+        HIPCHK(launch_pack(s, pa));
+        HIPCHK(hipMemcpyAsync(ctx->rb_host.p, ctx->rb_dev.p, off, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int q = 0; q < pa.nseg; ++q)
+            std::memcpy(used[q]->host, static_cast<const char*>(ctx->rb_host.p) + pa.off[q], used[q]->bytes);
     }
     return ROME_OK;
 }
@@ -494,15 +556,11 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(launch_task_build(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p,
                                  ctx->blk_off.p));
         stage_end(ctx, 4);
-        HIPCHK(hipMemcpyAsync(&tot[0], ctx->task_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(&tot[1], ctx->blk_off.p + n, sizeof(long long), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(cstatus.data(), ctx->c_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
-        if (list) {
-            lcount.resize(n);
-            HIPCHK(hipMemcpyAsync(lcount.data(), ctx->list_cnt.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
-        }
-        HIPCHK(hipStreamSynchronize(s));
-        return ROME_OK;
+        if (list) lcount.resize(n);
+        return readback(ctx, {{ctx->task_off.p + n, &tot[0], sizeof(long long)},
+                              {ctx->blk_off.p + n, &tot[1], sizeof(long long)},
+                              {ctx->c_status.p, cstatus.data(), n * sizeof(int)},
+                              {ctx->list_cnt.p, list ? lcount.data() : nullptr, n * sizeof(int)}});
     };
     // the single pass (no oversampling) takes its significant diff2 back from the full pose term
     const bool use_list = ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass;
@@ -542,8 +600,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
                             ctx->task_m.p, ctx->task_blk0.p, ctx->task_nt.p, ctx->blk_task.p, ctx->blk_t.p, ctx->used.p,
                             ctx->ulist.p, ctx->m2u.p, ctx->ucount.p));
     int U = 0;
-    HIPCHK(hipMemcpyAsync(&U, ctx->ucount.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    rc = readback(ctx, {{ctx->ucount.p, &U, sizeof(int)}});
+    if (rc) return rc;
     HIPCHK(ctx->D2.reserve((size_t)nblocks * NE));
     HIPCHK(ctx->Pfine.reserve((size_t)U * NOR * ctx->Pf));
     stage_begin(ctx, 5);
@@ -705,23 +763,20 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     std::vector<long long> amax(n);
     std::vector<double> maxw(n), sumw(n);
     const long long* amax_d = ctx->two_pass ? ctx->f_amax.p : ctx->c_amax.p;
-    HIPCHK(hipMemcpyAsync(amax.data(), amax_d, n * sizeof(long long), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(maxw.data(), ctx->two_pass ? ctx->f_maxw.p : ctx->c_maxw.p, n * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(sumw.data(), sum_p, n * sizeof(double), hipMemcpyDeviceToHost, s));
     std::vector<double> offx(n), offy(n);
-    HIPCHK(hipMemcpyAsync(offx.data(), ctx->in_offx.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(offy.data(), ctx->in_offy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    std::vector<int> fstatus(n, 0);
     std::vector<int2> mt(n);
     if (ctx->two_pass) {
         HIPCHK(ctx->amax_mt.reserve(n));
         HIPCHK(launch_decode_argmax(s, ctx->f_amax.p, ctx->blk_off.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, n, NE,
                                     ctx->amax_mt.p));
-        HIPCHK(hipMemcpyAsync(mt.data(), ctx->amax_mt.p, n * sizeof(int2), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(fstatus.data(), ctx->f_status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
     }
-    HIPCHK(hipStreamSynchronize(s));
+    rc = readback(ctx, {{amax_d, amax.data(), n * sizeof(long long)},
+                        {ctx->two_pass ? ctx->f_maxw.p : ctx->c_maxw.p, maxw.data(), n * sizeof(double)},
+                        {sum_p, sumw.data(), n * sizeof(double)},
+                        {ctx->in_offx.p, offx.data(), n * sizeof(double)},
+                        {ctx->in_offy.p, offy.data(), n * sizeof(double)},
+                        {ctx->amax_mt.p, ctx->two_pass ? mt.data() : nullptr, n * sizeof(int2)}});
+    if (rc) return rc;
     std::vector<int> argmax(6 * (size_t)n, 0);
     std::vector<double> meta(9 * (size_t)n, 0.);
     for (int i = 0; i < n; ++i) {
@@ -758,7 +813,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         md[5] = offx[i] + sx[(size_t)t * nots + ot];
         md[6] = offy[i] + sy[(size_t)t * nots + ot];
     }
-    HIPCHK(hipMemcpyAsync(ctx->meta.p, meta.data(), meta.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx->up_host.reserve(meta.size() * sizeof(double)));
+    std::memcpy(ctx->up_host.p, meta.data(), meta.size() * sizeof(double));
+    HIPCHK(hipMemcpyAsync(ctx->meta.p, ctx->up_host.p, meta.size() * sizeof(double), hipMemcpyHostToDevice, s));
     // ---- storeWeightedSums
     StoreArgs sa;
     sa.n = n;
@@ -780,24 +837,22 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(launch_store_sums(s, sa));
     stage_end(ctx, 9);
     if (res) {
-        auto d2h = [&](void* h, const void* d, size_t bytes) -> hipError_t {
-            return h ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
-        };
-        HIPCHK(d2h(res->meta, ctx->meta.p, 9 * (size_t)n * sizeof(double)));
-        HIPCHK(d2h(res->nr_sign, ctx->c_nsig.p, n * sizeof(int)));
-        HIPCHK(d2h(res->min_diff2, dmin_p, n * sizeof(double)));
-        HIPCHK(d2h(res->sum_weight, sum_p, n * sizeof(double)));
-        HIPCHK(d2h(res->sig_weight, thr_p, n * sizeof(double)));
-        HIPCHK(d2h(res->min_diff2_coarse, ctx->c_dmin.p, n * sizeof(double)));
-        HIPCHK(d2h(res->sum_weight_coarse, ctx->c_sumw.p, n * sizeof(double)));
-        HIPCHK(d2h(res->sig_weight_coarse, ctx->c_thr.p, n * sizeof(double)));
-        HIPCHK(d2h(res->wsum_norm_correction, ctx->wnorm.p, n * sizeof(double)));
+        const size_t nd = n * sizeof(double);
+        rc = readback(ctx, {{ctx->meta.p, res->meta, 9 * nd},
+                            {ctx->c_nsig.p, res->nr_sign, n * sizeof(int)},
+                            {dmin_p, res->min_diff2, nd},
+                            {sum_p, res->sum_weight, nd},
+                            {thr_p, res->sig_weight, nd},
+                            {ctx->c_dmin.p, res->min_diff2_coarse, nd},
+                            {ctx->c_sumw.p, res->sum_weight_coarse, nd},
+                            {ctx->c_thr.p, res->sig_weight_coarse, nd},
+                            {ctx->wnorm.p, res->wsum_norm_correction, nd}});
+        if (rc) return rc;
         if (res->argmax) std::memcpy(res->argmax, argmax.data(), argmax.size() * sizeof(int));
     }
     HIPCHK(hipStreamSynchronize(s));
     stage_collect(ctx);
     ctx->last_U = U;
-    (void)fstatus;
     return ROME_OK;
 }
 

@@ -328,6 +328,18 @@ hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const F
 hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a);
 hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
                           long long nblocks, const float* E, const double* C2c, int M, int T, double* D2);
+// Gathers up to kPackSegs device arrays into one contiguous staging buffer (one launch), so that a
+// group of small read-backs is one copy to pinned host memory.  Sizes and offsets are multiples of 4.
+constexpr int kPackSegs = 16;
+struct PackArgs {
+    const void* src[kPackSegs];
+    unsigned long long off[kPackSegs + 1];  // byte offsets in dst (16-aligned); off[nseg] = total
+    unsigned long long len[kPackSegs];      // bytes of each source array
+    int nseg;
+    void* dst;
+};
+hipError_t launch_pack(hipStream_t s, const PackArgs& a);
+
 hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
                                 const int* blk_t, const int* task_m, int n, int NE, int2* out);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);

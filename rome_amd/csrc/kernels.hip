@@ -11,6 +11,8 @@
 //   D2      [blocks][NOR*NOT] double         fine squared differences, block = (task, significant t)
 // diff2 = E + 1/2 (sum Minv |X|^2 + highres_Xi2): the per-image constant is added in FP64.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "device_util.h"
@@ -838,6 +840,28 @@ hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a) {
     k_prep_images<<<n, 256, 0, s>>>(a);
     LAUNCH_CHECK();
     return hipSuccess;
+}
+
+__global__ void k_pack(PackArgs a) {
+    const unsigned long long words = a.off[a.nseg] / 4;
+    for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long b = 4 * w;
+        int g = 0;
+        while (g + 1 < a.nseg && a.off[g + 1] <= b) ++g;
+        const unsigned long long rel = b - a.off[g];
+        if (rel < a.len[g])  // the alignment gap after each array is left alone
+            reinterpret_cast<unsigned*>(a.dst)[w] = reinterpret_cast<const unsigned*>(a.src[g])[rel / 4];
+    }
+}
+
+hipError_t launch_pack(hipStream_t s, const PackArgs& a) {
+    if (a.nseg <= 0) return hipSuccess;
+    if (a.nseg > kPackSegs) return hipErrorInvalidValue;
+    const unsigned long long words = a.off[a.nseg] / 4;
+    const unsigned blocks = (unsigned)std::min<unsigned long long>((words + 255) / 256, 1024);
+    k_pack<<<blocks, 256, 0, s>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
