@@ -63,7 +63,9 @@ typedef struct {
     const int* group;           /* [n] */
 } rome_images;
 
-/* Device-resident variant: interleaved FP32 complex / FP32 CTF, FP64 scalars. */
+/* Device-resident variant: interleaved FP32 complex / FP32 CTF, FP64 scalars.  The arrays are read in
+ * place during the call (no copy): on the context's device, each aligned to its element size, and
+ * not written by the caller until the call returns. */
 typedef struct {
     int n;
     const void* fmask;   /* float2 [n][cur*(cur/2+1)] */
@@ -260,6 +262,16 @@ int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap);
 /* Last pool: out3 = {0 list / 1 full / 2 list overflowed and the pool ran full, largest list,
  * total list entries}. */
 int rome_get_coarse_stats(rome_ctx* ctx, long long* out3);
+
+/* Form of the coarse significance the fine pass is built from (no counterpart in the reference, whose
+ * Exp_Mweight stores it per sample): by default the keys of each image's significant samples, up to
+ * `cap` (<= 4096) per image; a pool where some image has more goes through a dense per-sample mask
+ * instead (and so do the next 8 pools).  cap = 0: the mask always; < 0: the default (4096).
+ * Results are the same either way. */
+int rome_set_significance_cap(rome_ctx* ctx, int cap);
+/* Last pool: out2 = {0 keys / 1 mask / 2 keys overflowed and the pool used the mask, largest number of
+ * significant samples of an image (keys form; -1 otherwise)}. */
+int rome_get_significance_stats(rome_ctx* ctx, long long* out2);
 
 /* ---- M-step: model parameters other than the maps ----
  * maximizationOtherParameters (src/map3d_optimizer.cpp:2620-2798; 2D src/map2d_optimizer.cpp:1985-2045)

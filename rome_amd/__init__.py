@@ -27,6 +27,7 @@ EXPORTED = (
     "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
     "rome_get_batch_stats", "rome_maximization_other", "rome_update_model", "rome_fourier_map",
     "rome_reconstruct", "rome_set_coarse_mode", "rome_get_coarse_stats", "rome_prepare_images",
+    "rome_set_significance_cap", "rome_get_significance_stats",
 )
 
 COARSE_LIST, COARSE_FULL = 0, 1
@@ -421,6 +422,16 @@ class Engine:
         out = np.zeros(3, np.int64)
         self._check(lib().rome_get_coarse_stats(self._h, _p(out, C.c_longlong)), "rome_get_coarse_stats")
         return dict(zip(("mode", "list_max", "list_total"), out.tolist()))
+
+    def set_significance_cap(self, cap=-1):
+        """Significant samples per image kept as keys (<= 4096); 0: the dense mask always; < 0: default."""
+        self._check(lib().rome_set_significance_cap(self._h, int(cap)), "rome_set_significance_cap")
+
+    def significance_stats(self):
+        out = np.zeros(2, np.int64)
+        self._check(lib().rome_get_significance_stats(self._h, _p(out, C.c_longlong)),
+                    "rome_get_significance_stats")
+        return dict(zip(("sig_form", "sig_max"), out.tolist()))
 
     def project(self, iclass, size, A):
         A = _f64(A).reshape(-1, 9)

@@ -94,41 +94,64 @@ __global__ __launch_bounds__(256) void k_bp_count(BpArgs a) {
 }
 
 // Exclusive scans of count[0..nb) into off[0..nb] and of the chunks per brick into coff[0..nb];
-// cursor cleared.  One block of 1024 threads.
+// cursor cleared.  One block of 1024 threads over tiles of 4096 bricks: four consecutive bricks per
+// thread, wave scans by shuffles, the 16 wave totals through LDS, a running carry between tiles.
 __global__ __launch_bounds__(1024) void k_bp_scan(const unsigned* __restrict__ count, int nb,
                                                   unsigned* __restrict__ off, unsigned* __restrict__ coff,
                                                   unsigned* __restrict__ cursor) {
-    __shared__ unsigned part[1024], cpart[1024];
-    const int tid = threadIdx.x;
-    const int chunk = (nb + 1023) / 1024;
-    const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
-    unsigned loc = 0, cloc = 0;
-    for (int b = b0; b < b1; ++b) {
-        loc += count[b];
-        cloc += (count[b] + kChunk - 1) / kChunk;
-    }
-    part[tid] = loc;
-    cpart[tid] = cloc;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scans of the per-thread sums
-        const unsigned v = tid >= d ? part[tid - d] : 0u;
-        const unsigned cv = tid >= d ? cpart[tid - d] : 0u;
+    __shared__ unsigned wa[16], wc[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    unsigned carry = 0u, ccarry = 0u;
+    for (int t0 = 0; t0 < nb; t0 += 4096) {
+        const int b = t0 + 4 * tid;
+        unsigned c[4], cc[4], loc = 0u, cloc = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] = b + j < nb ? count[b + j] : 0u;
+            cc[j] = (c[j] + kChunk - 1) / kChunk;
+            loc += c[j];
+            cloc += cc[j];
+        }
+        unsigned in = loc, cin = cloc;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned v = __shfl_up(in, o, 64), cv = __shfl_up(cin, o, 64);
+            if (lane >= o) {
+                in += v;
+                cin += cv;
+            }
+        }
+        if (lane == 63) {
+            wa[wid] = in;
+            wc[wid] = cin;
+        }
         __syncthreads();
-        part[tid] += v;
-        cpart[tid] += cv;
-        __syncthreads();
+        unsigned pre = 0u, cpre = 0u, tot = 0u, ctot = 0u;
+        for (int w = 0; w < 16; ++w) {
+            if (w < wid) {
+                pre += wa[w];
+                cpre += wc[w];
+            }
+            tot += wa[w];
+            ctot += wc[w];
+        }
+        unsigned run = carry + pre + in - loc, crun = ccarry + cpre + cin - cloc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (b + j < nb) {
+                off[b + j] = run;
+                coff[b + j] = crun;
+                cursor[b + j] = 0u;
+            }
+            run += c[j];
+            crun += cc[j];
+        }
+        carry += tot;
+        ccarry += ctot;
+        __syncthreads();  // the wave totals are rewritten by the next tile
     }
-    unsigned run = part[tid] - loc, crun = cpart[tid] - cloc;
-    for (int b = b0; b < b1; ++b) {
-        off[b] = run;
-        coff[b] = crun;
-        cursor[b] = 0u;
-        run += count[b];
-        crun += (count[b] + kChunk - 1) / kChunk;
-    }
-    if (tid == 1023) {
-        off[nb] = part[1023];
-        coff[nb] = cpart[1023];
+    if (tid == 0) {
+        off[nb] = carry;
+        coff[nb] = ccarry;
     }
 }
 

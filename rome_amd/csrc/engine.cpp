@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -56,13 +57,21 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // A buffer that has to grow again (its size follows the data, e.g. the number of significant
+    // poses) gets 25 % headroom: hipFree waits for the device, so every regrowth stalls the stream.
     hipError_t reserve(size_t count) {
         if (count <= n && p) return hipSuccess;
+        if (p) count += count / 4;
         release();
         size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
         hipError_t e = hipMalloc(&p, bytes);
         if (e == hipSuccess) n = std::max<size_t>(count, 1);
         return e;
+    }
+    // for sizes that follow the data: 25 % headroom from the first allocation on
+    hipError_t grow(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        return reserve(count + count / 4);
     }
     hipError_t upload(const T* h, size_t count, hipStream_t s) {
         hipError_t e = reserve(count);
@@ -105,7 +114,7 @@ struct rome_ctx {
     // device persistent
     DevBuf<float2> grids;
     DevBuf<float4> cube;  // 3D: 2x2x2 neighbourhood records of grids, rebuilt when a grid changes
-    DevBuf<double> Ai_c, Ai_f, sintab, costab, d_sxc, d_syc, d_sxf, d_syf, d_tx, d_ty;
+    DevBuf<double> Ai_c, Ai_f, sintab, costab, d_sxc, d_syc, d_sxf, d_syf, d_tx, d_ty, d_pose_c, d_pose_f;
     DevBuf<int> c_nfine, c_ires, c_xy, f_nfine, f_ires, f_xy;
     DevBuf<float2> Sc, Sf;
     DevBuf<double> sigma2, pdf_dir, lpdf_dir, prior_x, prior_y, logsigma2, pdf_m, lpdf_m;
@@ -139,6 +148,14 @@ struct rome_ctx {
     DevBuf<float> in_fctf;
     DevBuf<double> in_hxi2, in_power, in_offx, in_offy, in_norm;
     DevBuf<int> in_group;
+    // the batch's inputs as the E-step reads them: the buffers above (host entry) or the caller's own
+    // device arrays (device entry, read in place)
+    struct {
+        const float2 *fmask, *fnomask;
+        const float* fctf;
+        const double *hxi2, *power, *offx, *offy, *norm;
+        const int* group;
+    } in{};
     DevBuf<float> WcT, MC2c, At, E, Minvf, Ctff, MC2bp;
     DevBuf<double> C2c, C2f;
     DevBuf<float2> Xf, NMf;
@@ -146,6 +163,10 @@ struct rome_ctx {
     DevBuf<long long> c_amax, f_amax;
     DevBuf<int> c_nsig, c_status, f_status;
     DevBuf<uint32_t> sigmask;
+    DevBuf<int> sig_keys, sig_cnt;
+    int sig_cap = kSigCap;   // significant samples per image kept as keys (0: the mask always)
+    int sig_dense_left = 0;  // batches that go straight to the significance mask after a key overflow
+    long long last_sig_form = 0, last_sig_max = -1;
     DevBuf<int> ntask, nblk;
     DevBuf<long long> task_off, blk_off;
     DevBuf<int> task_img, task_m, task_nt, blk_task, blk_t, ulist, m2u, ucount;
@@ -153,7 +174,7 @@ struct rome_ctx {
     DevBuf<unsigned char> used;
     DevBuf<float2> Pfine;
     DevBuf<double> D2;
-    DevBuf<int2> amax_mt;
+    DevBuf<int> argmax;
     // binned backprojection
     static constexpr size_t kBinnedBytes = (size_t)24 << 30;  // slice values + records per batch
     DevBuf<float2> slice_val;
@@ -376,8 +397,8 @@ WeightParams weight_params(rome_ctx* ctx) {
     wp.lpdf_max = ctx->lpdf_max;
     wp.trans_x = ctx->d_tx.p;
     wp.trans_y = ctx->d_ty.p;
-    wp.old_offx = ctx->in_offx.p;
-    wp.old_offy = ctx->in_offy.p;
+    wp.old_offx = ctx->in.offx;
+    wp.old_offy = ctx->in.offy;
     wp.prior_x = ctx->prior_x.p;
     wp.prior_y = ctx->prior_y.p;
     wp.sigma2_offset = ctx->cfg.sigma2_offset;
@@ -385,7 +406,7 @@ WeightParams weight_params(rome_ctx* ctx) {
     return wp;
 }
 
-// The E-step on inputs already in ctx->in_* (device).
+// The E-step on the device inputs ctx->in points to.
 int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     int rc = ensure_ready(ctx);
     if (rc) return rc;
@@ -417,11 +438,11 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->NMf.reserve(nPf));
     HIPCHK(ctx->MC2bp.reserve(nPf));
     PrepArgs pa;
-    pa.fmask = ctx->in_fmask.p;
-    pa.fnomask = ctx->in_fnomask.p;
-    pa.fctf = ctx->in_fctf.p;
-    pa.group = ctx->in_group.p;
-    pa.hxi2 = ctx->in_hxi2.p;
+    pa.fmask = ctx->in.fmask;
+    pa.fnomask = ctx->in.fnomask;
+    pa.fctf = ctx->in.fctf;
+    pa.group = ctx->in.group;
+    pa.hxi2 = ctx->in.hxi2;
     pa.sigma2 = ctx->sigma2.p;
     pa.ori_Fsize = ctx->oriF;
     pa.F2f = ctx->F2f;
@@ -453,9 +474,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(launch_prep_images(s, n, pa));
     if (prec != ROME_PREC_FP32) {
         PrepLowpArgs pl;
-        pl.fmask = ctx->in_fmask.p;
-        pl.fctf = ctx->in_fctf.p;
-        pl.group = ctx->in_group.p;
+        pl.fmask = ctx->in.fmask;
+        pl.fctf = ctx->in.fctf;
+        pl.group = ctx->in.group;
         pl.sigma2 = ctx->sigma2.p;
         pl.ori_Fsize = ctx->oriF;
         pl.F2f = ctx->F2f;
@@ -493,7 +514,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->c_nsig.reserve(n));
     HIPCHK(ctx->c_status.reserve(n));
     HIPCHK(ctx->f_status.reserve(n));
-    HIPCHK(ctx->sigmask.reserve((size_t)n * M * ctx->TW));
+    HIPCHK(ctx->used.reserve(M));
+    HIPCHK(ctx->ulist.reserve(M));
+    HIPCHK(ctx->m2u.reserve(M));
+    HIPCHK(ctx->ucount.reserve(1));
     HIPCHK(ctx->emin_key.reserve(n));
     HIPCHK(ctx->cand_w.reserve((size_t)n * rome_ctx::kCandCap));
     HIPCHK(ctx->cand_du.reserve((size_t)n * rome_ctx::kCandCap));
@@ -506,7 +530,44 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
                            ctx->c_status.p};
     const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
     long long tot[2];
-    std::vector<int> cstatus(n), lcount;
+    int U = 0;
+    std::vector<int> cstatus(n), lcount, scount(n);
+    bool sig_dense = false;
+    // weights and significance -> fine-pass task counts and the rotations they need; returns after
+    // one host synchronisation.  Sparse: per-image keys of the significant samples; dense: the mask
+    // (when some image has more than kSigCap significant samples).
+    auto significance = [&](const ListArgs* L, bool dense) -> int {
+        SigArgs sg{nullptr, ctx->TW, nullptr, nullptr, 0};
+        if (dense) {
+            HIPCHK(ctx->sigmask.reserve((size_t)n * M * ctx->TW));
+            sg.mask = ctx->sigmask.p;
+        } else {
+            HIPCHK(ctx->sig_keys.reserve((size_t)n * ctx->sig_cap));
+            HIPCHK(ctx->sig_cnt.reserve(n));
+            sg.keys = ctx->sig_keys.p;
+            sg.cnt = ctx->sig_cnt.p;
+            sg.cap = ctx->sig_cap;
+        }
+        HIPCHK(hipMemsetAsync(ctx->used.p, 0, M, s));
+        stage_begin(ctx, 3);
+        if (L)
+            HIPCHK(launch_weights_list(s, n, wp, *L, ctx->C2c.p, ctx->emin_key.p, wc, sg, cand));
+        else
+            HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->C2c.p, ctx->emin_key.p, wc, sg, cand));
+        stage_end(ctx, 3);
+        stage_begin(ctx, 4);
+        HIPCHK(launch_task_build(s, n, sg, M, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p, ctx->blk_off.p,
+                                 ctx->used.p, ctx->ulist.p, ctx->m2u.p, ctx->ucount.p));
+        stage_end(ctx, 4);
+        if (L) lcount.resize(n);
+        sig_dense = dense;
+        return readback(ctx, {{ctx->task_off.p + n, &tot[0], sizeof(long long)},
+                              {ctx->blk_off.p + n, &tot[1], sizeof(long long)},
+                              {ctx->ucount.p, &U, sizeof(int)},
+                              {ctx->c_status.p, cstatus.data(), n * sizeof(int)},
+                              {ctx->list_cnt.p, L ? lcount.data() : nullptr, n * sizeof(int)},
+                              {ctx->sig_cnt.p, dense ? nullptr : scount.data(), n * sizeof(int)}});
+    };
     // cross term -> list of the entries within reach of the weights (or the whole pose term E) ->
     // weights and significance -> fine-pass tasks; returns after one host synchronisation
     auto coarse = [&](bool list) -> int {
@@ -545,27 +606,23 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             HIPCHK(launch_gemm_lowp(s, prec, la));
         }
         stage_end(ctx, 2);
-        stage_begin(ctx, 3);
-        if (list)
-            HIPCHK(launch_weights_list(s, n, wp, L, ctx->C2c.p, ctx->emin_key.p, wc, ctx->sigmask.p, ctx->TW, cand));
-        else
-            HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->C2c.p, ctx->emin_key.p, wc, ctx->sigmask.p, ctx->TW,
-                                         cand));
-        stage_end(ctx, 3);
-        stage_begin(ctx, 4);
-        HIPCHK(launch_task_build(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p,
-                                 ctx->blk_off.p));
-        stage_end(ctx, 4);
-        if (list) lcount.resize(n);
-        return readback(ctx, {{ctx->task_off.p + n, &tot[0], sizeof(long long)},
-                              {ctx->blk_off.p + n, &tot[1], sizeof(long long)},
-                              {ctx->c_status.p, cstatus.data(), n * sizeof(int)},
-                              {ctx->list_cnt.p, list ? lcount.data() : nullptr, n * sizeof(int)}});
+        const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
+        int r = significance(list ? &L : nullptr, dense);
+        ctx->last_sig_form = dense ? 1 : 0;
+        ctx->last_sig_max = dense ? -1 : *std::max_element(scount.begin(), scount.end());
+        if (r || dense) return r;
+        if (ctx->last_sig_max > ctx->sig_cap) {
+            ctx->sig_dense_left = 8;  // more significant samples than the keys hold: the mask, for a while
+            ctx->last_sig_form = 2;
+            return significance(list ? &L : nullptr, true);
+        }
+        return 0;
     };
     // the single pass (no oversampling) takes its significant diff2 back from the full pose term
     const bool use_list = ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass;
     rc = coarse(use_list);
     if (rc) return rc;
+    if (ctx->sig_dense_left > 0) --ctx->sig_dense_left;
     ctx->last_list_max = 0;
     ctx->last_list_total = 0;
     for (int v : lcount) {
@@ -585,25 +642,18 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     ctx->last_tasks = ntasks;
     ctx->last_blocks = nblocks;
     const int NOR = ctx->NORf, NOT = ctx->NOTf, NE = NOR * NOT;
-    HIPCHK(ctx->task_img.reserve(ntasks));
-    HIPCHK(ctx->task_m.reserve(ntasks));
-    HIPCHK(ctx->task_nt.reserve(ntasks));
-    HIPCHK(ctx->task_blk0.reserve(ntasks));
-    HIPCHK(ctx->blk_task.reserve(nblocks));
-    HIPCHK(ctx->blk_t.reserve(nblocks));
-    HIPCHK(ctx->used.reserve(M));
-    HIPCHK(hipMemsetAsync(ctx->used.p, 0, M, s));
-    HIPCHK(ctx->ulist.reserve(M));
-    HIPCHK(ctx->m2u.reserve(M));
-    HIPCHK(ctx->ucount.reserve(1));
-    HIPCHK(launch_task_emit(s, n, ctx->sigmask.p, M, ctx->TW, T, ctx->task_off.p, ctx->blk_off.p, ctx->task_img.p,
-                            ctx->task_m.p, ctx->task_blk0.p, ctx->task_nt.p, ctx->blk_task.p, ctx->blk_t.p, ctx->used.p,
-                            ctx->ulist.p, ctx->m2u.p, ctx->ucount.p));
-    int U = 0;
-    rc = readback(ctx, {{ctx->ucount.p, &U, sizeof(int)}});
-    if (rc) return rc;
-    HIPCHK(ctx->D2.reserve((size_t)nblocks * NE));
-    HIPCHK(ctx->Pfine.reserve((size_t)U * NOR * ctx->Pf));
+    HIPCHK(ctx->task_img.grow(ntasks));
+    HIPCHK(ctx->task_m.grow(ntasks));
+    HIPCHK(ctx->task_nt.grow(ntasks));
+    HIPCHK(ctx->task_blk0.grow(ntasks));
+    HIPCHK(ctx->blk_task.grow(nblocks));
+    HIPCHK(ctx->blk_t.grow(nblocks));
+    const SigArgs sg = sig_dense ? SigArgs{ctx->sigmask.p, ctx->TW, nullptr, nullptr, 0}
+                                 : SigArgs{nullptr, ctx->TW, ctx->sig_keys.p, ctx->sig_cnt.p, ctx->sig_cap};
+    HIPCHK(launch_task_emit(s, n, sg, M, T, ctx->task_off.p, ctx->blk_off.p, ctx->task_img.p, ctx->task_m.p,
+                            ctx->task_blk0.p, ctx->task_nt.p, ctx->blk_task.p, ctx->blk_t.p));
+    HIPCHK(ctx->D2.grow((size_t)nblocks * NE));
+    HIPCHK(ctx->Pfine.grow((size_t)U * NOR * ctx->Pf));
     stage_begin(ctx, 5);
     HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->is3d ? ctx->cube.p : nullptr, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d, ctx->Ai_f.p,
                                ctx->R, NOR, ctx->ulist.p, U, ctx->f_xy.p, ctx->Pf, ctx->r_max, ctx->Pfine.p));
@@ -662,7 +712,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.c = ctx->cgrid;
         pp.r_max = ctx->r_max;
         pp.grid_stride = ctx->grid_n;
-        pp.group = ctx->in_group.p;
+        pp.group = ctx->in.group;
         pp.sumw = sum_p;
         pp.thr = thr_p;
         pp.dmin = dmin_p;
@@ -678,8 +728,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.trans_y = ctx->d_ty.p;
         pp.shift_x = ctx->two_pass ? ctx->d_sxf.p : ctx->d_sxc.p;
         pp.shift_y = ctx->two_pass ? ctx->d_syf.p : ctx->d_syc.p;
-        pp.old_offx = ctx->in_offx.p;
-        pp.old_offy = ctx->in_offy.p;
+        pp.old_offx = ctx->in.offx;
+        pp.old_offy = ctx->in.offy;
         pp.prior_x = ctx->prior_x.p;
         pp.prior_y = ctx->prior_y.p;
         pp.sigma2_offset = c.sigma2_offset;
@@ -709,8 +759,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.slice_val = nullptr;
         pp.slice_w = nullptr;
         if (binned) {
-            HIPCHK(ctx->slice_val.reserve((size_t)S * ctx->Pf));
-            HIPCHK(ctx->slice_w.reserve((size_t)S));
+            HIPCHK(ctx->slice_val.grow((size_t)S * ctx->Pf));
+            HIPCHK(ctx->slice_w.grow((size_t)S));
             pp.slice_val = ctx->slice_val.p;
             pp.slice_w = ctx->slice_w.p;
         }
@@ -737,7 +787,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             HIPCHK(ctx->bp_off.reserve(nb + 1));
             HIPCHK(ctx->bp_coff.reserve(nb + 1));
             HIPCHK(ctx->bp_cursor.reserve(nb));
-            HIPCHK(ctx->bp_rec.reserve((size_t)S * ctx->Pf));
+            HIPCHK(ctx->bp_rec.grow((size_t)S * ctx->Pf));
             ba.task_img = ctx->task_img.p;
             ba.task_m = ctx->task_m.p;
             ba.Ai_fine = ctx->Ai_f.p;
@@ -759,63 +809,33 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     } else {
         HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
     }
-    // ---- metadata of the last pass
-    std::vector<long long> amax(n);
-    std::vector<double> maxw(n), sumw(n);
-    const long long* amax_d = ctx->two_pass ? ctx->f_amax.p : ctx->c_amax.p;
-    std::vector<double> offx(n), offy(n);
-    std::vector<int2> mt(n);
-    if (ctx->two_pass) {
-        HIPCHK(ctx->amax_mt.reserve(n));
-        HIPCHK(launch_decode_argmax(s, ctx->f_amax.p, ctx->blk_off.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, n, NE,
-                                    ctx->amax_mt.p));
-    }
-    rc = readback(ctx, {{amax_d, amax.data(), n * sizeof(long long)},
-                        {ctx->two_pass ? ctx->f_maxw.p : ctx->c_maxw.p, maxw.data(), n * sizeof(double)},
-                        {sum_p, sumw.data(), n * sizeof(double)},
-                        {ctx->in_offx.p, offx.data(), n * sizeof(double)},
-                        {ctx->in_offy.p, offy.data(), n * sizeof(double)},
-                        {ctx->amax_mt.p, ctx->two_pass ? mt.data() : nullptr, n * sizeof(int2)}});
-    if (rc) return rc;
-    std::vector<int> argmax(6 * (size_t)n, 0);
-    std::vector<double> meta(9 * (size_t)n, 0.);
-    for (int i = 0; i < n; ++i) {
-        if (!(sumw[i] > 0)) return fail(ctx, ROME_ERR_NUMERIC, "image %d: exp_sum_weight == 0", i);
-        int m, t, o = 0, ot = 0;
-        if (ctx->two_pass) {
-            m = mt[i].x;
-            t = mt[i].y;
-            o = (int)((amax[i] % NE) / NOT);
-            ot = (int)(amax[i] % NOT);
-        } else {
-            m = (int)(amax[i] / T);
-            t = (int)(amax[i] % T);
-        }
-        const int k = m / ctx->R, r = m % ctx->R, dir = r / ctx->PSI, psi = r % ctx->PSI;
-        int* am = &argmax[6 * i];
-        am[0] = k, am[1] = dir, am[2] = psi, am[3] = o, am[4] = t, am[5] = ot;
-        double* md = &meta[9 * i];
-        md[0] = maxw[i] / sumw[i];
-        md[1] = k + 1;
-        const bool fine = ctx->two_pass;
-        if (ctx->is3d) {
-            const double* pose = fine ? &ctx->pose_f[3 * ((size_t)r * NOR + o)] : &ctx->pose_c[3 * (size_t)r];
-            md[2] = pose[0], md[3] = pose[1], md[4] = pose[2];
-        } else {
-            double ps = fine ? ctx->pose_f[(size_t)r * NOR + o] : ctx->pose_c[r];
-            if (ps > 180.) ps -= 360.;
-            if (ps < -180.) ps += 360.;
-            md[2] = 0, md[3] = 0, md[4] = ps;
-        }
-        const std::vector<double>& sx = fine ? ctx->sxf : ctx->sxc;
-        const std::vector<double>& sy = fine ? ctx->syf : ctx->syc;
-        const int nots = fine ? NOT : 1;
-        md[5] = offx[i] + sx[(size_t)t * nots + ot];
-        md[6] = offy[i] + sy[(size_t)t * nots + ot];
-    }
-    HIPCHK(ctx->up_host.reserve(meta.size() * sizeof(double)));
-    std::memcpy(ctx->up_host.p, meta.data(), meta.size() * sizeof(double));
-    HIPCHK(hipMemcpyAsync(ctx->meta.p, ctx->up_host.p, meta.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    // ---- metadata of the last pass (on the device; read back with the results)
+    MetaArgs ma;
+    ma.n = n;
+    ma.two_pass = ctx->two_pass;
+    ma.is3d = ctx->is3d;
+    ma.R = ctx->R;
+    ma.PSI = ctx->PSI;
+    ma.NOR = NOR;
+    ma.NOT = NOT;
+    ma.T = T;
+    ma.Mtot = M;
+    ma.amax = ctx->two_pass ? ctx->f_amax.p : ctx->c_amax.p;
+    ma.img_blk = ctx->blk_off.p;
+    ma.blk_task = ctx->blk_task.p;
+    ma.blk_t = ctx->blk_t.p;
+    ma.task_m = ctx->task_m.p;
+    ma.maxw = ctx->two_pass ? ctx->f_maxw.p : ctx->c_maxw.p;
+    ma.sumw = sum_p;
+    ma.offx = ctx->in.offx;
+    ma.offy = ctx->in.offy;
+    ma.pose = ctx->two_pass ? ctx->d_pose_f.p : ctx->d_pose_c.p;
+    ma.sx = ctx->two_pass ? ctx->d_sxf.p : ctx->d_sxc.p;
+    ma.sy = ctx->two_pass ? ctx->d_syf.p : ctx->d_syc.p;
+    ma.meta = ctx->meta.p;
+    HIPCHK(ctx->argmax.reserve((size_t)6 * n));
+    ma.argmax = ctx->argmax.p;
+    HIPCHK(launch_meta(s, ma));
     // ---- storeWeightedSums
     StoreArgs sa;
     sa.n = n;
@@ -823,9 +843,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     sa.oriF = ctx->oriF;
     sa.do_norm = c.do_norm_correction;
     sa.avg_norm = c.avg_norm_correction;
-    sa.group = ctx->in_group.p;
-    sa.power = ctx->in_power.p;
-    sa.norm = ctx->in_norm.p;
+    sa.group = ctx->in.group;
+    sa.power = ctx->in.power;
+    sa.norm = ctx->in.norm;
     sa.sumw = sum_p;
     sa.dmin = dmin_p;
     sa.logsigma2 = ctx->logsigma2.p;
@@ -836,20 +856,24 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     stage_begin(ctx, 9);
     HIPCHK(launch_store_sums(s, sa));
     stage_end(ctx, 9);
-    if (res) {
+    std::vector<double> sumw(n);
+    {
         const size_t nd = n * sizeof(double);
-        rc = readback(ctx, {{ctx->meta.p, res->meta, 9 * nd},
-                            {ctx->c_nsig.p, res->nr_sign, n * sizeof(int)},
-                            {dmin_p, res->min_diff2, nd},
-                            {sum_p, res->sum_weight, nd},
-                            {thr_p, res->sig_weight, nd},
-                            {ctx->c_dmin.p, res->min_diff2_coarse, nd},
-                            {ctx->c_sumw.p, res->sum_weight_coarse, nd},
-                            {ctx->c_thr.p, res->sig_weight_coarse, nd},
-                            {ctx->wnorm.p, res->wsum_norm_correction, nd}});
+        rc = readback(ctx, {{sum_p, sumw.data(), nd},
+                            {ctx->meta.p, res ? res->meta : nullptr, 9 * nd},
+                            {ctx->argmax.p, res ? res->argmax : nullptr, 6 * n * sizeof(int)},
+                            {ctx->c_nsig.p, res ? res->nr_sign : nullptr, n * sizeof(int)},
+                            {dmin_p, res ? res->min_diff2 : nullptr, nd},
+                            {sum_p, res ? res->sum_weight : nullptr, nd},
+                            {thr_p, res ? res->sig_weight : nullptr, nd},
+                            {ctx->c_dmin.p, res ? res->min_diff2_coarse : nullptr, nd},
+                            {ctx->c_sumw.p, res ? res->sum_weight_coarse : nullptr, nd},
+                            {ctx->c_thr.p, res ? res->sig_weight_coarse : nullptr, nd},
+                            {ctx->wnorm.p, res ? res->wsum_norm_correction : nullptr, nd}});
         if (rc) return rc;
-        if (res->argmax) std::memcpy(res->argmax, argmax.data(), argmax.size() * sizeof(int));
     }
+    for (int i = 0; i < n; ++i)
+        if (!(sumw[i] > 0)) return fail(ctx, ROME_ERR_NUMERIC, "image %d: exp_sum_weight == 0", i);
     HIPCHK(hipStreamSynchronize(s));
     stage_collect(ctx);
     ctx->last_U = U;
@@ -1033,6 +1057,8 @@ int rome_set_sampling(rome_ctx* ctx, const double* pose_coarse, const double* po
     hipStream_t s = ctx->stream;
     HIPCHK(ctx->d_sxc.upload(ctx->sxc.data(), ctx->T, s));
     HIPCHK(ctx->d_syc.upload(ctx->syc.data(), ctx->T, s));
+    HIPCHK(ctx->d_pose_c.upload(ctx->pose_c.data(), ctx->pose_c.size(), s));
+    if (ctx->two_pass) HIPCHK(ctx->d_pose_f.upload(ctx->pose_f.data(), ctx->pose_f.size(), s));
     if (ctx->two_pass) {
         HIPCHK(ctx->d_sxf.upload(ctx->sxf.data(), ctx->sxf.size(), s));
         HIPCHK(ctx->d_syf.upload(ctx->syf.data(), ctx->syf.size(), s));
@@ -1184,6 +1210,8 @@ int rome_expectation(rome_ctx* ctx, const rome_images* im, rome_results* res) {
     HIPCHK(ctx->in_offy.upload(im->old_offy, n, s));
     HIPCHK(ctx->in_norm.upload(im->norm, n, s));
     HIPCHK(ctx->in_group.upload(im->group, n, s));
+    ctx->in = {ctx->in_fmask.p, ctx->in_fnomask.p, ctx->in_fctf.p, ctx->in_hxi2.p, ctx->in_power.p,
+               ctx->in_offx.p, ctx->in_offy.p, ctx->in_norm.p, ctx->in_group.p};
     return run_estep(ctx, n, res);
 }
 
@@ -1193,26 +1221,15 @@ int rome_expectation_device(rome_ctx* ctx, const rome_device_images* im, rome_re
     const int n = im->n;
     if (n <= 0) return fail(ctx, ROME_ERR_ARG, "empty image pool");
     HIPCHK(hipSetDevice(ctx->device));
-    hipStream_t s = ctx->stream;
-    const size_t F2 = ctx->F2f;
-    auto d2d = [&](void* dst, const void* src, size_t bytes) {
-        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
-    };
-    HIPCHK(ctx->in_fmask.reserve((size_t)n * F2));
-    HIPCHK(ctx->in_fnomask.reserve((size_t)n * F2));
-    HIPCHK(ctx->in_fctf.reserve((size_t)n * F2));
-    for (auto* b : {&ctx->in_hxi2, &ctx->in_offx, &ctx->in_offy, &ctx->in_norm}) HIPCHK(b->reserve(n));
-    HIPCHK(ctx->in_power.reserve((size_t)n * ctx->oriF));
-    HIPCHK(ctx->in_group.reserve(n));
-    HIPCHK(d2d(ctx->in_fmask.p, im->fmask, (size_t)n * F2 * sizeof(float2)));
-    HIPCHK(d2d(ctx->in_fnomask.p, im->fnomask, (size_t)n * F2 * sizeof(float2)));
-    HIPCHK(d2d(ctx->in_fctf.p, im->fctf, (size_t)n * F2 * sizeof(float)));
-    HIPCHK(d2d(ctx->in_hxi2.p, im->highres_Xi2, n * sizeof(double)));
-    HIPCHK(d2d(ctx->in_power.p, im->power_img, (size_t)n * ctx->oriF * sizeof(double)));
-    HIPCHK(d2d(ctx->in_offx.p, im->old_offx, n * sizeof(double)));
-    HIPCHK(d2d(ctx->in_offy.p, im->old_offy, n * sizeof(double)));
-    HIPCHK(d2d(ctx->in_norm.p, im->norm, n * sizeof(double)));
-    HIPCHK(d2d(ctx->in_group.p, im->group, n * sizeof(int)));
+    // read in place: no copy of the caller's arrays (each aligned to its element size)
+    const void* ptrs[] = {im->fmask, im->fnomask, im->fctf, im->highres_Xi2, im->power_img,
+                          im->old_offx, im->old_offy, im->norm, im->group};
+    const size_t align[] = {8, 8, 4, 8, 8, 8, 8, 8, 4};
+    for (int i = 0; i < 9; ++i)
+        if (!ptrs[i] || (reinterpret_cast<uintptr_t>(ptrs[i]) % align[i]) != 0)
+            return fail(ctx, ROME_ERR_ARG, "device image array %d is null or misaligned", i);
+    ctx->in = {reinterpret_cast<const float2*>(im->fmask), reinterpret_cast<const float2*>(im->fnomask), im->fctf,
+               im->highres_Xi2, im->power_img, im->old_offx, im->old_offy, im->norm, im->group};
     return run_estep(ctx, n, res);
 }
 
@@ -1363,6 +1380,21 @@ int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap) {
     if (mode != ROME_COARSE_LIST && mode != ROME_COARSE_FULL) return fail(ctx, ROME_ERR_ARG, "unknown coarse mode %d", mode);
     ctx->coarse_mode = mode;
     ctx->list_cap = list_cap > 0 ? list_cap : 0;
+    return ROME_OK;
+}
+
+int rome_set_significance_cap(rome_ctx* ctx, int cap) {
+    if (!ctx) return ROME_ERR_ARG;
+    if (cap > kSigCap) return fail(ctx, ROME_ERR_ARG, "significance cap %d above %d", cap, kSigCap);
+    ctx->sig_cap = cap < 0 ? kSigCap : cap;
+    ctx->sig_dense_left = 0;
+    return ROME_OK;
+}
+
+int rome_get_significance_stats(rome_ctx* ctx, long long* out) {
+    if (!ctx || !out) return ROME_ERR_ARG;
+    out[0] = ctx->last_sig_form;
+    out[1] = ctx->last_sig_max;
     return ROME_OK;
 }
 

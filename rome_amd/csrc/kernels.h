@@ -305,13 +305,26 @@ struct CandArgs {  // per-image candidate list of the weight pass: [n][cap]
     int cap;
 };
 
+// Significant coarse samples (weight >= the image's threshold), one of two forms:
+//   keys != nullptr: per image the keys m*T + t of its significant samples, unordered, in
+//                    keys[img][0 .. cap); cnt[img] = how many there are (may exceed cap: the caller
+//                    then redoes the pass with the mask);
+//   otherwise      : mask[img][m][TW] bit t of rotation m.
+constexpr int kSigCap = 4096;  // most keys the sparse task build sorts per image (in LDS)
+struct SigArgs {
+    uint32_t* mask;
+    int TW;
+    int* keys;
+    int* cnt;
+    int cap;
+};
 hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const double* C2c,
-                                 const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                                 const int* emin_key, const WeightOutArgs& wo, const SigArgs& sig,
                                  const CandArgs& cand);
 // The same weights and significance from the coarse list (ListArgs); an image whose list
 // overflowed gets status 8 and nothing else.
 hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, const ListArgs& list, const double* C2c,
-                               const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                               const int* emin_key, const WeightOutArgs& wo, const SigArgs& sig,
                                const CandArgs& cand);
 // reach[img] = kWeightWindow + (max - min of ln pdf_offset over (class, translation)) + slack,
 // +inf when a pdf_offset is zero
@@ -319,11 +332,19 @@ hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
                                const WeightOutArgs& wo, const CandArgs& cand);
-hipError_t launch_task_build(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, int* ntask, int* nblk,
-                             long long* task_off, long long* blk_off);
-hipError_t launch_task_emit(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, const long long* task_off,
+// Task lists of the fine pass from the significant samples: per image the number of tasks
+// (rotations with any significant translation) and blocks (significant samples), their exclusive
+// scans (task_off, blk_off [n + 1]), used[m] = 1 for every rotation some image needs (zero before)
+// and the compaction of those: ulist[u] = m, m2u[m] = u, *ucount = U.  The sparse form also sorts
+// each image's keys in place.
+hipError_t launch_task_build(hipStream_t s, int n, const SigArgs& sig, int M, int T, int* ntask, int* nblk,
+                             long long* task_off, long long* blk_off, unsigned char* used, int* ulist, int* m2u,
+                             int* ucount);
+// Ordered emission (rotations ascending per image, translations ascending per rotation):
+// task_img/task_m/task_blk0/task_nt per task, blk_task/blk_t per block.
+hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int T, const long long* task_off,
                             const long long* blk_off, int* task_img, int* task_m, long long* task_blk0, int* task_nt,
-                            int* blk_task, int* blk_t, unsigned char* used, int* ulist, int* m2u, int* ucount);
+                            int* blk_task, int* blk_t);
 hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a);
 hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a);
 hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
@@ -340,8 +361,23 @@ struct PackArgs {
 };
 hipError_t launch_pack(hipStream_t s, const PackArgs& a);
 
-hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
-                                const int* blk_t, const int* task_m, int n, int NE, int2* out);
+// Per-image metadata of the last pass (the max-posterior sample): meta columns 0-6 (PMAX, CLASS,
+// ROT, TILT, PSI, XOFF, YOFF; 7 and 8 zero, filled by the sums kernel) and argmax = (class, dir, psi,
+// over_rot, trans, over_trans).  src/map3d_optimizer.cpp:1786-1805, src/map2d_optimizer.cpp:1200-1218.
+struct MetaArgs {
+    int n, two_pass, is3d, R, PSI, NOR, NOT, T, Mtot;
+    const long long* amax;     // [n] argmax key: fine entry (b - img_blk[i]) * NE + o * NOT + ot, or coarse m * T + t
+    const long long* img_blk;  // [n + 1] fine blocks per image (two pass)
+    const int* blk_task;
+    const int* blk_t;
+    const int* task_m;
+    const double *maxw, *sumw, *offx, *offy;
+    const double* pose;        // fine [R][NOR][3 or 1] (two pass) or coarse [R][3 or 1]
+    const double *sx, *sy;     // shifts of the pass: fine [T][NOT] or coarse [T]
+    double* meta;              // [n][9]
+    int* argmax;               // [n][6]
+};
+hipError_t launch_meta(hipStream_t s, const MetaArgs& a);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
 hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a);
 // M-step reconstruction (reconstruct.hip)

@@ -92,12 +92,22 @@ __global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad
 // neighbourhood grid (cube, same values and arithmetic as the flat gather); 2D the flat grid.
 __global__ void k_project_fine(const float2* __restrict__ grids, const float4* __restrict__ cube, size_t grid_stride,
                                int pad, int padF, int c, int is3d, const double* __restrict__ Ai_fine, int R, int NOR,
-                               const int* __restrict__ ulist, const int* __restrict__ xy, int Pf, int r_max,
+                               const int* __restrict__ ulist, int U, const int* __restrict__ xy, int Pf, int r_max,
                                float2* __restrict__ Pfine) {
-    const int p = blockIdx.y * blockDim.x + threadIdx.x;
-    const int uo = blockIdx.x;  // U*NOR can exceed the 65535 limit of gridDim.y
+    // Workgroups go to the 8 XCDs round-robin by launch index; each XCD takes one contiguous range of
+    // work items w = (u * nPB + pixel block) * NOR + o, so the NOR oversampled orientations of a
+    // rotation, which read neighbouring records, share one L2.
+    const int nPB = (Pf + 255) / 256;
+    const long long G = (long long)U * NOR * nPB, per = (G + 7) / 8;
+    const long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x;  // gridDim.x is a multiple of 8
+    if ((L >> 3) >= per) return;  // the last grid row runs past 8 * per
+    const long long w = (L & 7) * per + (L >> 3);
+    if (w >= G) return;
+    const int o = (int)(w % NOR);
+    const long long upb = w / NOR;
+    const int u = (int)(upb / nPB);
+    const int p = (int)(upb % nPB) * 256 + threadIdx.x;
     if (p >= Pf) return;
-    const int u = uo / NOR, o = uo % NOR;
     const int m = ulist[u];
     const int k = m / R, r = m % R;
     const int x = px_x(xy[p]), y = px_y(xy[p]);
@@ -315,9 +325,9 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
 
 // ------------------------------------------------------------------ task building
 // Per image: count rotations (m) with any significant translation and the number of
-// significant (m, t) pairs.  One block per image.
+// significant (m, t) pairs; flag the rotations in used[].  One block per image.
 __global__ void k_task_count(const uint32_t* __restrict__ sigmask, int M, int TW, int* __restrict__ ntask,
-                             int* __restrict__ nblk) {
+                             int* __restrict__ nblk, unsigned char* __restrict__ used) {
     __shared__ long long red[16];
     const int img = blockIdx.x;
     const uint32_t* msk = sigmask + (size_t)img * M * TW;
@@ -327,6 +337,7 @@ __global__ void k_task_count(const uint32_t* __restrict__ sigmask, int M, int TW
         for (int w = 0; w < TW; ++w) c += __popc(msk[(size_t)m * TW + w]);
         nt += c > 0;
         nb += c;
+        if (c > 0) used[m] = 1;
     }
     nt = block_reduce(nt, [](long long a, long long b) { return a + b; }, red);
     nb = block_reduce(nb, [](long long a, long long b) { return a + b; }, red);
@@ -336,19 +347,137 @@ __global__ void k_task_count(const uint32_t* __restrict__ sigmask, int M, int TW
     }
 }
 
-// Exclusive scans of the per-image counts (single block).
-__global__ void k_scan_images(const int* __restrict__ ntask, const int* __restrict__ nblk, int n,
-                              long long* __restrict__ task_off, long long* __restrict__ blk_off) {
-    if (threadIdx.x != 0) return;
+// Sparse form: sort the image's keys m*T + t ascending (bitonic, in LDS, written back in place),
+// count the distinct rotations and flag them in used[].  An image with more keys than the buffer
+// holds counts nothing (the caller redoes the pass with the mask).
+__global__ __launch_bounds__(1024) void k_task_count_keys(int* __restrict__ keys, const int* __restrict__ cnt, int cap,
+                                                          int T, int* __restrict__ ntask, int* __restrict__ nblk,
+                                                          unsigned char* __restrict__ used) {
+    __shared__ int sk[kSigCap];
+    __shared__ int red[16];
+    const int img = blockIdx.x, tid = threadIdx.x;
+    const int c = cnt[img];
+    if (c > cap) {
+        if (tid == 0) ntask[img] = nblk[img] = 0;
+        return;
+    }
+    int* k = keys + (size_t)img * cap;
+    int np2 = 1;
+    while (np2 < c) np2 <<= 1;
+    for (int q = tid; q < np2; q += 1024) sk[q] = q < c ? k[q] : 0x7fffffff;
+    __syncthreads();
+    for (int ksz = 2; ksz <= np2; ksz <<= 1)
+        for (int j = ksz >> 1; j > 0; j >>= 1) {
+            for (int q = tid; q < np2; q += 1024) {
+                const int ixj = q ^ j;
+                if (ixj > q) {
+                    const int a1 = sk[q], a2 = sk[ixj];
+                    if (((q & ksz) == 0) ? (a1 > a2) : (a1 < a2)) {
+                        sk[q] = a2;
+                        sk[ixj] = a1;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    int nt = 0;
+    for (int q = tid; q < c; q += 1024) {
+        k[q] = sk[q];
+        const int m = sk[q] / T;
+        if (q == 0 || sk[q - 1] / T != m) {
+            ++nt;
+            used[m] = 1;
+        }
+    }
+    nt = block_reduce(nt, [](int a, int b) { return a + b; }, red);
+    if (tid == 0) {
+        ntask[img] = nt;
+        nblk[img] = c;
+    }
+}
+
+// Emission from the sorted keys: block q of the image is key q; a task starts at each new rotation.
+__global__ __launch_bounds__(1024) void k_task_emit_keys(const int* __restrict__ keys, const int* __restrict__ cnt,
+                                                         int cap, int T, const long long* __restrict__ task_off,
+                                                         const long long* __restrict__ blk_off,
+                                                         int* __restrict__ task_img, int* __restrict__ task_m,
+                                                         long long* __restrict__ task_blk0, int* __restrict__ task_nt,
+                                                         int* __restrict__ blk_task, int* __restrict__ blk_t) {
+    __shared__ int sc[1024];
+    __shared__ int base;
+    const int img = blockIdx.x, tid = threadIdx.x;
+    const int c = cnt[img];
+    const int* k = keys + (size_t)img * cap;
+    const long long bt = task_off[img], bb = blk_off[img];
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int q0 = 0; q0 < c; q0 += 1024) {
+        const int q = q0 + tid;
+        const int key = q < c ? k[q] : 0;
+        const int m = key / T;
+        const int first = (q < c && (q == 0 || k[q - 1] / T != m)) ? 1 : 0;
+        sc[tid] = first;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+            const int v = tid >= o ? sc[tid - o] : 0;
+            __syncthreads();
+            sc[tid] += v;
+            __syncthreads();
+        }
+        if (q < c) {
+            const long long tq = bt + base + sc[tid] - 1;
+            blk_task[bb + q] = (int)tq;
+            blk_t[bb + q] = key - m * T;
+            if (first) {
+                int e = q + 1;
+                while (e < c && k[e] / T == m) ++e;
+                task_img[tq] = img;
+                task_m[tq] = m;
+                task_blk0[tq] = bb + q;
+                task_nt[tq] = e - q;
+            }
+        }
+        __syncthreads();
+        if (tid == 1023) base += sc[1023];
+        __syncthreads();
+    }
+}
+
+// Exclusive scans of the per-image counts: one 1024-thread block, each thread a contiguous run of
+// images, the run totals scanned in LDS.
+__global__ __launch_bounds__(1024) void k_scan_images(const int* __restrict__ ntask, const int* __restrict__ nblk,
+                                                      int n, long long* __restrict__ task_off,
+                                                      long long* __restrict__ blk_off) {
+    __shared__ long long sa[1024], sb[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024, i0 = min(n, tid * per), i1 = min(n, i0 + per);
     long long a = 0, b = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = i0; i < i1; ++i) {
+        a += ntask[i];
+        b += nblk[i];
+    }
+    sa[tid] = a;
+    sb[tid] = b;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan of the run totals
+        const long long va = tid >= o ? sa[tid - o] : 0, vb = tid >= o ? sb[tid - o] : 0;
+        __syncthreads();
+        sa[tid] += va;
+        sb[tid] += vb;
+        __syncthreads();
+    }
+    a = sa[tid] - a;
+    b = sb[tid] - b;
+    for (int i = i0; i < i1; ++i) {
         task_off[i] = a;
         blk_off[i] = b;
         a += ntask[i];
         b += nblk[i];
     }
-    task_off[n] = a;
-    blk_off[n] = b;
+    if (tid == 1023) {
+        task_off[n] = sa[1023];
+        blk_off[n] = sb[1023];
+    }
 }
 
 // Ordered emission of tasks (m ascending) and blocks (t ascending) for one image per block.
@@ -357,7 +486,7 @@ __global__ __launch_bounds__(1024) void k_task_emit(const uint32_t* __restrict__
                                                     const long long* __restrict__ blk_off, int* __restrict__ task_img,
                                                     int* __restrict__ task_m, long long* __restrict__ task_blk0,
                                                     int* __restrict__ task_nt, int* __restrict__ blk_task,
-                                                    int* __restrict__ blk_t, unsigned char* __restrict__ used) {
+                                                    int* __restrict__ blk_t) {
     __shared__ int scan_t[1024], scan_b[1024];
     __shared__ long long base_t, base_b;
     const int img = blockIdx.x, tid = threadIdx.x;
@@ -390,7 +519,6 @@ __global__ __launch_bounds__(1024) void k_task_emit(const uint32_t* __restrict__
             task_m[q] = m;
             task_blk0[q] = b;
             task_nt[q] = c;
-            used[m] = 1;
             int j = 0;
             for (int w = 0; w < TW; ++w) {
                 uint32_t bits = msk[(size_t)m * TW + w];
@@ -412,36 +540,46 @@ __global__ __launch_bounds__(1024) void k_task_emit(const uint32_t* __restrict__
     }
 }
 
-// Compaction of the rotations used by any image: ulist[u] = m, m2u[m] = u (single block).
+// Compaction of the rotations used by any image: ulist[u] = m, m2u[m] = u (single block, tiles of
+// 4096 flags: four per thread, wave scans by shuffles, wave totals through LDS).
 __global__ __launch_bounds__(1024) void k_unique_rot(const unsigned char* __restrict__ used, int M,
                                                      int* __restrict__ ulist, int* __restrict__ m2u,
                                                      int* __restrict__ ucount) {
-    __shared__ int sc[1024];
-    __shared__ int base;
-    const int tid = threadIdx.x;
-    if (tid == 0) base = 0;
-    __syncthreads();
-    for (int m0 = 0; m0 < M; m0 += 1024) {
-        const int m = m0 + tid;
-        const int f = (m < M) ? used[m] : 0;
-        sc[tid] = f;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            int v = tid >= o ? sc[tid - o] : 0;
-            __syncthreads();
-            sc[tid] += v;
-            __syncthreads();
+    __shared__ int wt[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int carry = 0;
+    for (int t0 = 0; t0 < M; t0 += 4096) {
+        const int m = t0 + 4 * tid;
+        int f[4], loc = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            f[j] = (m + j < M && used[m + j]) ? 1 : 0;
+            loc += f[j];
         }
-        if (f) {
-            const int u = base + sc[tid] - 1;
-            ulist[u] = m;
-            m2u[m] = u;
+        int in = loc;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(in, o, 64);
+            if (lane >= o) in += v;
         }
+        if (lane == 63) wt[wid] = in;
         __syncthreads();
-        if (tid == 1023) base += sc[1023];
+        int pre = 0, tot = 0;
+        for (int w = 0; w < 16; ++w) {
+            if (w < wid) pre += wt[w];
+            tot += wt[w];
+        }
+        int u = carry + pre + in - loc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (f[j]) {
+                ulist[u] = m + j;
+                m2u[m + j] = u;
+                ++u;
+            }
+        carry += tot;
         __syncthreads();
     }
-    if (tid == 0) *ucount = base;
+    if (tid == 0) *ucount = carry;
 }
 
 // ------------------------------------------------------------------ fine squared differences
@@ -770,14 +908,54 @@ __global__ void k_fill_d2(const int* __restrict__ blk_task, const int* __restric
     D2[b] = (double)E[((size_t)img * T + blk_t[b]) * M + task_m[q]] + C2c[img];
 }
 
-// argmax entry of the fine list -> (rotation m, translation t)
-__global__ void k_decode_argmax(const long long* __restrict__ amax, const long long* __restrict__ img_blk,
-                                const int* __restrict__ blk_task, const int* __restrict__ blk_t,
-                                const int* __restrict__ task_m, int n, int NE, int2* __restrict__ out) {
+// Max-posterior sample of each image -> metadata row and argmax tuple (one thread per image).  An image
+// without one (no weight; the caller reports it) gets zeros.
+__global__ void k_meta(MetaArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long long b = img_blk[i] + amax[i] / NE;
-    out[i] = make_int2(task_m[blk_task[b]], blk_t[b]);
+    if (i >= a.n) return;
+    double* md = a.meta + (size_t)9 * i;
+    int* am = a.argmax + (size_t)6 * i;
+    for (int c = 0; c < 9; ++c) md[c] = 0.;
+    for (int c = 0; c < 6; ++c) am[c] = 0;
+    const long long key = a.amax[i];
+    if (!(a.sumw[i] > 0.) || key < 0) return;
+    int m, t, o = 0, ot = 0;
+    if (a.two_pass) {
+        const int NE = a.NOR * a.NOT;
+        const long long b = a.img_blk[i] + key / NE;
+        if (b >= a.img_blk[i + 1]) return;
+        m = a.task_m[a.blk_task[b]];
+        t = a.blk_t[b];
+        o = (int)((key % NE) / a.NOT);
+        ot = (int)(key % a.NOT);
+    } else {
+        if (key >= (long long)a.Mtot * a.T) return;
+        m = (int)(key / a.T);
+        t = (int)(key % a.T);
+    }
+    const int k = m / a.R, r = m % a.R;
+    am[0] = k;
+    am[1] = r / a.PSI;
+    am[2] = r % a.PSI;
+    am[3] = o;
+    am[4] = t;
+    am[5] = ot;
+    md[0] = a.maxw[i] / a.sumw[i];
+    md[1] = k + 1;
+    const size_t pr = a.two_pass ? (size_t)r * a.NOR + o : (size_t)r;
+    if (a.is3d) {
+        md[2] = a.pose[3 * pr];
+        md[3] = a.pose[3 * pr + 1];
+        md[4] = a.pose[3 * pr + 2];
+    } else {
+        double ps = a.pose[pr];
+        if (ps > 180.) ps -= 360.;
+        if (ps < -180.) ps += 360.;
+        md[4] = ps;
+    }
+    const int nots = a.two_pass ? a.NOT : 1;
+    md[5] = a.offx[i] + a.sx[(size_t)t * nots + ot];
+    md[6] = a.offy[i] + a.sy[(size_t)t * nots + ot];
 }
 
 // ------------------------------------------------------------------ launch wrappers
@@ -819,9 +997,12 @@ hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4*
                                const int* xy, int Pf, int r_max, float2* Pfine) {
     if (U == 0) return hipSuccess;
     if (is3d && !cube) return hipErrorInvalidValue;  // 3D projections read the neighbourhood grid
-    dim3 grid(U * NOR, (Pf + 255) / 256);
-    k_project_fine<<<grid, 256, 0, s>>>(grids, is3d ? cube : nullptr, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR,
-                                        ulist, xy, Pf, r_max, Pfine);
+    const long long G = (long long)U * NOR * ((Pf + 255) / 256);
+    const long long blocks = 8 * ((G + 7) / 8);
+    const unsigned gx = (unsigned)std::min<long long>(blocks, 32768), gy = (unsigned)((blocks + gx - 1) / gx);
+    if (gy > 65535) return hipErrorInvalidValue;
+    k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, is3d ? cube : nullptr, grid_stride, pad, padF, c, is3d, Ai_fine, R,
+                                          NOR, ulist, U, xy, Pf, r_max, Pfine);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -881,22 +1062,34 @@ hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a) {
     return hipSuccess;
 }
 
-hipError_t launch_task_build(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, int* ntask, int* nblk,
-                             long long* task_off, long long* blk_off) {
-    k_task_count<<<n, 256, 0, s>>>(sigmask, M, TW, ntask, nblk);
+hipError_t launch_task_build(hipStream_t s, int n, const SigArgs& sig, int M, int T, int* ntask, int* nblk,
+                             long long* task_off, long long* blk_off, unsigned char* used, int* ulist, int* m2u,
+                             int* ucount) {
+    if (n <= 0) return hipSuccess;
+    if (sig.keys) {
+        if (sig.cap > kSigCap) return hipErrorInvalidValue;
+        k_task_count_keys<<<n, 1024, 0, s>>>(sig.keys, sig.cnt, sig.cap, T, ntask, nblk, used);
+    } else {
+        k_task_count<<<n, 256, 0, s>>>(sig.mask, M, sig.TW, ntask, nblk, used);
+    }
     LAUNCH_CHECK();
-    k_scan_images<<<1, 64, 0, s>>>(ntask, nblk, n, task_off, blk_off);
+    k_scan_images<<<1, 1024, 0, s>>>(ntask, nblk, n, task_off, blk_off);
+    LAUNCH_CHECK();
+    k_unique_rot<<<1, 1024, 0, s>>>(used, M, ulist, m2u, ucount);
     LAUNCH_CHECK();
     return hipSuccess;
 }
 
-hipError_t launch_task_emit(hipStream_t s, int n, const uint32_t* sigmask, int M, int TW, int T, const long long* task_off,
+hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int T, const long long* task_off,
                             const long long* blk_off, int* task_img, int* task_m, long long* task_blk0, int* task_nt,
-                            int* blk_task, int* blk_t, unsigned char* used, int* ulist, int* m2u, int* ucount) {
-    k_task_emit<<<n, 1024, 0, s>>>(sigmask, M, TW, T, task_off, blk_off, task_img, task_m, task_blk0, task_nt, blk_task,
-                                    blk_t, used);
-    LAUNCH_CHECK();
-    k_unique_rot<<<1, 1024, 0, s>>>(used, M, ulist, m2u, ucount);
+                            int* blk_task, int* blk_t) {
+    if (n <= 0) return hipSuccess;
+    if (sig.keys)
+        k_task_emit_keys<<<n, 1024, 0, s>>>(sig.keys, sig.cnt, sig.cap, T, task_off, blk_off, task_img, task_m,
+                                            task_blk0, task_nt, blk_task, blk_t);
+    else
+        k_task_emit<<<n, 1024, 0, s>>>(sig.mask, M, sig.TW, T, task_off, blk_off, task_img, task_m, task_blk0, task_nt,
+                                        blk_task, blk_t);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -948,9 +1141,9 @@ hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, 
     return hipSuccess;
 }
 
-hipError_t launch_decode_argmax(hipStream_t s, const long long* amax, const long long* img_blk, const int* blk_task,
-                                const int* blk_t, const int* task_m, int n, int NE, int2* out) {
-    k_decode_argmax<<<(n + 127) / 128, 128, 0, s>>>(amax, img_blk, blk_task, blk_t, task_m, n, NE, out);
+hipError_t launch_meta(hipStream_t s, const MetaArgs& a) {
+    if (a.n <= 0) return hipSuccess;
+    k_meta<<<(a.n + 127) / 128, 128, 0, s>>>(a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -15,7 +15,8 @@
 //            refined while it holds more candidates than the LDS sort; exact weights of that
 //            bucket sorted and accumulated from the top as the reference's sorted loop does
 //            -> threshold and NR_SIGN.
-//   mark   : (coarse) significance bits for candidates with w >= threshold.
+//   mark   : (coarse) the samples with w >= threshold: their keys m*T + t per image, or bits of
+//            the significance mask.
 // When an image has more candidates than the list holds, select / mark rescan the entries.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -59,6 +60,7 @@ struct WShared {
     long long red_l[16];
     int ncand_sort;
     int ncand;
+    int nmark;
     int done;
     double lo, width, mbefore;
     long long cbefore;
@@ -73,7 +75,7 @@ struct WShared {
 // first maximum in that order (ties between equal weights).
 template <typename Get, typename Pre, typename Key>
 __device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, Pre pre, Key key,
-                            WeightOut wo, WShared& sh, uint32_t* sigmask, int TW, Cand cand, const int* emin_key) {
+                            WeightOut wo, WShared& sh, SigArgs sig, Cand cand, const int* emin_key) {
     constexpr int NT = 1024;
     const int tid = threadIdx.x;
     const int T = wp.T;
@@ -344,9 +346,26 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         if (wo.nsig) wo.nsig[img] = status == 1 ? 0 : (int)sh.nsig;
         wo.status[img] = status | (listed ? 0 : 4);
     }
-    if (!sigmask) return;
     // ---- mark: w >= threshold, with the weight the reference compares
-    uint32_t* msk = sigmask + (size_t)img * wp.Mtot * TW;
+    if (sig.keys) {
+        if (tid == 0) sh.nmark = 0;
+        __syncthreads();
+        if (status != 1)
+            visit(0.0, kWin, [&](double w, double, long long e) {
+                if (!(w >= thr)) return;
+                double d;
+                int m, t;
+                get((size_t)e, d, m, t);
+                const int slot = atomicAdd(&sh.nmark, 1);
+                if (slot < sig.cap) sig.keys[(size_t)img * sig.cap + slot] = m * T + t;
+            });
+        __syncthreads();
+        if (tid == 0) sig.cnt[img] = sh.nmark;
+        return;
+    }
+    if (!sig.mask) return;
+    const int TW = sig.TW;
+    uint32_t* msk = sig.mask + (size_t)img * wp.Mtot * TW;
     for (size_t q = tid; q < (size_t)wp.Mtot * TW; q += NT) msk[q] = 0u;
     __syncthreads();
     if (status == 1) return;
@@ -414,7 +433,7 @@ struct AllPre {
 __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const float* __restrict__ E,
                                                          const double* __restrict__ C2c,
                                                          const int* __restrict__ emin_key, WeightOut wo,
-                                                         uint32_t* __restrict__ sigmask, int TW, Cand cand) {
+                                                         SigArgs sig, Cand cand) {
     __shared__ WShared sh;
     const int img = blockIdx.x;
     const int T = wp.T;
@@ -429,19 +448,22 @@ __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const 
     };
     const CoarsePre pre{Ei, (size_t)Mtot * T, magic, Mtot, wp.mrow, (Mtot & 3) == 0};
     auto key = [&](size_t, int m, int t) { return (long long)m * T + t; };  // (class, dir, psi, t) order
-    weight_pass(img, (size_t)Mtot * T, C2c[img], wp, get, pre, key, wo, sh, sigmask, TW, cand, emin_key);
+    weight_pass(img, (size_t)Mtot * T, C2c[img], wp, get, pre, key, wo, sh, sig, cand, emin_key);
 }
 
 // Coarse weights from the list the cross-term GEMM kept (every entry within kWin of the minimum
 // u, and more): entries i = 0 .. count-1 of the image's list, argmax keys in the reference's order.
 __global__ __launch_bounds__(1024) void k_weights_list(WeightParams wp, ListArgs L, const double* __restrict__ C2c,
                                                        const int* __restrict__ emin_key, WeightOut wo,
-                                                       uint32_t* __restrict__ sigmask, int TW, Cand cand) {
+                                                       SigArgs sig, Cand cand) {
     __shared__ WShared sh;
     const int img = blockIdx.x;
     const int cnt = L.count[img];
     if (cnt > L.cap) {  // the list overflowed: the caller reruns the batch with the full pose term
-        if (threadIdx.x == 0) wo.status[img] = 8;
+        if (threadIdx.x == 0) {
+            wo.status[img] = 8;
+            if (sig.keys) sig.cnt[img] = 0;
+        }
         return;
     }
     const int T = wp.T;
@@ -457,7 +479,7 @@ __global__ __launch_bounds__(1024) void k_weights_list(WeightParams wp, ListArgs
     };
     const AllPre pre{(size_t)cnt};
     auto key = [&](size_t, int m, int t) { return (long long)m * T + t; };
-    weight_pass(img, (size_t)cnt, C2c[img], wp, get, pre, key, wo, sh, sigmask, TW, cand, emin_key);
+    weight_pass(img, (size_t)cnt, C2c[img], wp, get, pre, key, wo, sh, sig, cand, emin_key);
 }
 
 // reach[img] = kWin + (max - min of ln pdf_offset over the image's (class, translation) pairs) + 1
@@ -498,25 +520,27 @@ __global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const do
     };
     const AllPre pre{count};
     auto key = [](size_t e, int, int) { return (long long)e; };
-    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, nullptr, 1, cand, nullptr);
+    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, SigArgs{nullptr, 1, nullptr, nullptr, 0}, cand, nullptr);
 }
 
 }  // namespace
 
 hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, const float* E, const double* C2c,
-                                 const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                                 const int* emin_key, const WeightOutArgs& wo, const SigArgs& sig,
                                  const CandArgs& c) {
+    if (sig.keys && ((long long)wp.Mtot * wp.T > 0x7fffffffLL || sig.cap > kSigCap)) return hipErrorInvalidValue;
     WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
-    k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, C2c, emin_key, o, sigmask, TW, Cand{c.w, c.du, c.e, c.cap});
+    k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, C2c, emin_key, o, sig, Cand{c.w, c.du, c.e, c.cap});
     return hipGetLastError();
 }
 
 hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, const ListArgs& list, const double* C2c,
-                               const int* emin_key, const WeightOutArgs& wo, uint32_t* sigmask, int TW,
+                               const int* emin_key, const WeightOutArgs& wo, const SigArgs& sig,
                                const CandArgs& c) {
     if (wp.K * wp.T > kMaxKT) return hipErrorInvalidValue;
+    if (sig.keys && ((long long)wp.Mtot * wp.T > 0x7fffffffLL || sig.cap > kSigCap)) return hipErrorInvalidValue;
     WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
-    k_weights_list<<<n, 1024, 0, s>>>(wp, list, C2c, emin_key, o, sigmask, TW, Cand{c.w, c.du, c.e, c.cap});
+    k_weights_list<<<n, 1024, 0, s>>>(wp, list, C2c, emin_key, o, sig, Cand{c.w, c.du, c.e, c.cap});
     return hipGetLastError();
 }
 

@@ -218,6 +218,35 @@ def test_list_overflow_reruns_full(engine):
         np.testing.assert_allclose(s[k], sf[k], rtol=1e-12, atol=1e-15, err_msg=k)
 
 
+@pytest.mark.parametrize("name", ["3d_window_groups_offsets", "2d_single_pass"])
+def test_significance_keys_and_mask_agree(engine, name):
+    """The fine pass built from the per-image keys of the significant samples, from the dense mask
+    (cap 0), and from keys that overflow their cap (the pool then takes the mask): the same tasks in
+    the same order, so the same E-step."""
+    prob, batch, _ = _case(name)
+    out = {}
+    try:
+        for cap in (-1, 0, 1):
+            engine.set_significance_cap(cap)
+            r, s = _run(engine, prob, batch)
+            keys, d2 = engine.get_fine_list()
+            out[cap] = (r, s, keys, d2, engine.significance_stats())
+    finally:
+        engine.set_significance_cap(-1)
+    assert out[-1][4]["sig_form"] == 0 and out[-1][4]["sig_max"] > 1, out[-1][4]
+    assert out[0][4]["sig_form"] == 1, out[0][4]
+    assert out[1][4]["sig_form"] == 2, out[1][4]
+    r0, s0, k0, d0, _ = out[-1]
+    for cap in (0, 1):
+        r, s, k, d, _ = out[cap]
+        assert np.array_equal(r["argmax"], r0["argmax"]) and np.array_equal(r["nr_sign"], r0["nr_sign"])
+        np.testing.assert_array_equal(r["meta"], r0["meta"])
+        np.testing.assert_array_equal(k, k0)
+        np.testing.assert_array_equal(d, d0)
+        for key in s0:  # FP64 atomic accumulation: order-dependent last bits only
+            np.testing.assert_allclose(s[key], s0[key], rtol=1e-12, atol=1e-15, err_msg=key)
+
+
 def test_update_model_matches_oracle(engine, case):
     """M-step model parameters from the engine's own sums vs the oracle's update of the oracle's sums."""
     import oracle.mstep as OM
