@@ -35,12 +35,6 @@ DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
 template <int TN>
 constexpr int epi_slots() { return TN + 2; }
 
-// List test of one E value e with ln pdf_direction l against the image's limit (the slack
-// 1e-6 (|e| + |l|) covers the FP32 rounding of e - l); one expression for counting and storing.
-DEV bool list_keeps(float e, float l, float lim) {
-    return __builtin_fmaf(-1e-6f, fabsf(e) + fabsf(l), e - l) < lim;
-}
-
 // One E value from an accumulator: At - C (FP32 arm) or At - C * inv_scale as one fused
 // multiply-add (scaled arms), the same rounding in every epilogue.
 template <bool SCALED>
@@ -190,22 +184,24 @@ DEV void gemm_epilogue_staged(const f16v (&acc)[NI][NJ], int r0, int c0, int n0,
 }
 
 // List epilogue (ListArgs): the per-image minimum of E as gemm_epilogue, and instead of E the
-// entries whose v = E - ln pdf_direction can lie within reach[img] of the image's minimum v.
-// v is evaluated in FP32 as vf with the slack tol = 1e-6 (|E| + |ln pdf_direction|), far above its
-// rounding.  Per 32-row accumulator block: the E values go to this wave's LDS region (column-major,
-// as gemm_epilogue_staged) while min(vf + tol) per image lowers the image's bound B >= min v (LDS
-// slot, then one global atomic per image); then eight lanes per column test the staged values
-// against B + reach and append the entries that pass: offsets from LDS counters per image, one
-// global atomic per image and block to reserve the range, then the stores.  B only falls, so every
-// entry the weights need passes, whatever the order in which tiles run.  All threads of the block
-// call it (barriers).  slot_* arrays hold epi_slots<TN>() entries.
+// entries whose v = E - ln pdf_direction can lie within reach[img] of the image's minimum v.  The
+// E values never leave the registers:
+//   1. E in place of the accumulators; per lane and column the minimum of E over rows with a
+//      prior (finite ln pdf_direction) and the minimum vf of v in FP32;
+//   2. per image of the tile (LDS slots, then one global atomic each) the bound B >= min v, where a
+//      lane offers vf_min + 1e-6 |vf_min| + 1e-4 >= its entry's exact v = (double)E - ln pdf (FP32
+//      rounding of E - l: < 6e-8 |vf|; of l = (float)ln pdf: < 6e-8 * 746);
+//   3. the test vf < lim = B + reach + 1e-6 (|B| + reach) + 1e-3 keeps every entry with exact
+//      v < B + reach (the margin exceeds both roundings and that of lim itself); counts per
+//      accumulator block -> LDS offsets per image -> one global atomic per image to reserve the
+//      range -> each lane stores its own entries.
+// B only falls, so every entry the weights need passes, whatever the order in which tiles run.  All
+// threads of the block call it (barriers).  slot_* arrays hold epi_slots<TN>() entries.
 template <bool SCALED, int NI, int NJ, int TN>
-DEV void gemm_epilogue_list(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
+DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
                             const float* __restrict__ At, const float* __restrict__ inv_scale,
-                            const unsigned char* __restrict__ valid, int* __restrict__ emin_key, int Mtot, int Mpad,
-                            int Ncols, int T, const ListArgs& L, int* slot_min, int* slot_vb, float* slot_lim,
-                            int* slot_cnt, float* stage) {
-    constexpr int LD = 36;  // column stride in floats (16-byte aligned)
+                            int* __restrict__ emin_key, int Mtot, int Mpad, int Ncols, int T,
+                            const ListArgs& L, int* slot_min, int* slot_vb, float* slot_lim, int* slot_cnt) {
     const int img0 = n0 / T;
     const int last = min(n0 + TN, Ncols) - 1;
     const int nslot = last >= n0 ? last / T - img0 + 1 : 0;
@@ -216,113 +212,117 @@ DEV void gemm_epilogue_list(const f16v (&acc)[NI][NJ], int r0, int c0, int n0, i
     }
     __syncthreads();
     const int h = lane >> 5;
-    int img_j[NJ];
-    float s_j[NJ], emin_j[NJ];
+    int sl_j[NJ];
+    bool ok_j[NJ];
+    float s_j[NJ], emin_j[NJ], vmin_j[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        const int col = min(c0 + j * 32 + (lane & 31), Ncols - 1);  // clamped: padding columns are dropped
-        img_j[j] = col / T;
-        s_j[j] = SCALED ? inv_scale[img_j[j]] : 1.f;
+        const int col = c0 + j * 32 + (lane & 31);
+        ok_j[j] = col < Ncols;
+        const int img = min(col, Ncols - 1) / T;  // clamped: padding columns are dropped
+        sl_j[j] = img - img0;
+        s_j[j] = SCALED ? inv_scale[img] : 1.f;
         emin_j[j] = INFINITY;
+        vmin_j[j] = INFINITY;
     }
+    // 1. E in place, minima
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
-        // registers -> LDS, minimum of E over valid rows, min(vf + tol) of this lane's column
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            const float* Ar = At + (size_t)img_j[j] * Mpad;
-            const bool colok = c0 + j * 32 + (lane & 31) < Ncols;
-            float vb = INFINITY;
+        for (int q = 0; q < 4; ++q) {
+            const int base = r0 + i * 32 + 8 * q + 4 * h;
+            const float4 lp = *reinterpret_cast<const float4*>(L.lpd + base);
+            const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int base = r0 + i * 32 + 8 * q + 4 * h;
-                const float4 at = *reinterpret_cast<const float4*>(Ar + base);
-                const float4 lp = *reinterpret_cast<const float4*>(L.lpd + base);
-                const uint32_t v4 = *reinterpret_cast<const uint32_t*>(valid + base);
+            for (int j = 0; j < NJ; ++j) {
+                const float4 at = *reinterpret_cast<const float4*>(At + (size_t)(img0 + sl_j[j]) * Mpad + base);
                 const float atv[4] = {at.x, at.y, at.z, at.w};
-                const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
-                float ev[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    ev[u] = e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j]);
-                    if (colok && base + u < Mtot && ((v4 >> (8 * u)) & 0xff)) emin_j[j] = fminf(emin_j[j], ev[u]);
-                    const float vf = ev[u] - lpd[u];
-                    vb = fminf(vb, vf + 1e-6f * (fabsf(ev[u]) + fabsf(lpd[u])));  // +inf without a prior
+                    const float e = e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j]);
+                    acc[i][j][4 * q + u] = e;
+                    emin_j[j] = fminf(emin_j[j], lpd[u] > -INFINITY ? e : INFINITY);  // rows with a prior
+                    vmin_j[j] = fminf(vmin_j[j], e - lpd[u]);  // +inf without a prior
                 }
-                *reinterpret_cast<float4*>(&stage[(j * 32 + (lane & 31)) * LD + 8 * q + 4 * h]) =
-                    make_float4(ev[0], ev[1], ev[2], ev[3]);
             }
-            if (colok && vb < INFINITY) atomicMin(&slot_vb[img_j[j] - img0], float_order_key(vb));
         }
-        __syncthreads();
-        for (int q = tid; q < nslot; q += blockDim.x) {
-            const int img = img0 + q;
-            const int mine = slot_vb[q];
-            const int glob = __hip_atomic_load(&L.vbound[img], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (mine < glob) atomicMin(&L.vbound[img], mine);
-            const int b = min(mine, glob);  // a stale global value is larger: still a valid bound
-            slot_vb[q] = b;
-            const float bv = float_from_order_key(b);
-            slot_lim[q] = b == INT_MAX ? -INFINITY : bv + L.reach[img] + 1e-6f * fabsf(bv);  // covers the sum's rounding
-        }
-        __syncthreads();
-        // LDS -> list: lane group of 8 = one column's 32 rows, four per lane.  Count, reserve, store.
-        const int row0 = r0 + i * 32 + 4 * (lane & 7);
-        const float4 lp = *reinterpret_cast<const float4*>(L.lpd + row0);
-        const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
-        int off[NJ * 4];
+    // 2. bounds per image
 #pragma unroll
-        for (int p = 0; p < NJ * 4; ++p) {
-            off[p] = -1;
-            const int c = p * 8 + (lane >> 3);
-            const int col = c0 + c;
-            if (col >= Ncols) continue;
-            const int sl = col / T - img0;
-            const float lim = slot_lim[sl];
-            const float4 v = *reinterpret_cast<const float4*>(&stage[c * LD + 4 * (lane & 7)]);
-            const float vv[4] = {v.x, v.y, v.z, v.w};
+    for (int j = 0; j < NJ; ++j) {
+        if (!ok_j[j]) continue;
+        if (emin_j[j] < INFINITY) atomicMin(&slot_min[sl_j[j]], float_order_key(emin_j[j]));
+        if (vmin_j[j] < INFINITY)
+            atomicMin(&slot_vb[sl_j[j]], float_order_key(vmin_j[j] + 1e-6f * fabsf(vmin_j[j]) + 1e-4f));
+    }
+    __syncthreads();
+    for (int q = tid; q < nslot; q += blockDim.x) {
+        const int img = img0 + q;
+        if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img], slot_min[q]);
+        const int mine = slot_vb[q];
+        const int glob = __hip_atomic_load(&L.vbound[img], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (mine < glob) atomicMin(&L.vbound[img], mine);
+        const int b = min(mine, glob);  // a stale global value is larger: still a valid bound
+        const float bv = float_from_order_key(b), r = L.reach[img];
+        slot_lim[q] = b == INT_MAX ? -INFINITY : bv + r + 1e-6f * (fabsf(bv) + r) + 1e-3f;
+    }
+    __syncthreads();
+    // 3. count, reserve, store
+    float lim_j[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) lim_j[j] = ok_j[j] ? slot_lim[sl_j[j]] : -INFINITY;
+    int off[NI][NJ];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        float lpd[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 lp = *reinterpret_cast<const float4*>(L.lpd + r0 + i * 32 + 8 * q + 4 * h);
+            lpd[4 * q] = lp.x, lpd[4 * q + 1] = lp.y, lpd[4 * q + 2] = lp.z, lpd[4 * q + 3] = lp.w;
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
             int cnt = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) cnt += list_keeps(vv[u], lpd[u], lim);
-            if (cnt) off[p] = atomicAdd(&slot_cnt[sl], cnt);
+            for (int r = 0; r < 16; ++r) cnt += acc[i][j][r] - lpd[r] < lim_j[j];
+            off[i][j] = cnt ? atomicAdd(&slot_cnt[sl_j[j]], cnt) : -1;
         }
-        __syncthreads();
-        for (int q = tid; q < nslot; q += blockDim.x) {
-            const int c = slot_cnt[q];
-            slot_cnt[q] = c ? atomicAdd(&L.count[img0 + q], c) : 0;  // now the base of this block's range
+    }
+    __syncthreads();
+    for (int q = tid; q < nslot; q += blockDim.x) {
+        const int c = slot_cnt[q];
+        slot_cnt[q] = c ? atomicAdd(&L.count[img0 + q], c) : 0;  // now the base of this block's range
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) any |= off[i][j] >= 0;
+        if (!any) continue;
+        float lpd[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 lp = *reinterpret_cast<const float4*>(L.lpd + r0 + i * 32 + 8 * q + 4 * h);
+            lpd[4 * q] = lp.x, lpd[4 * q + 1] = lp.y, lpd[4 * q + 2] = lp.z, lpd[4 * q + 3] = lp.w;
         }
-        __syncthreads();
 #pragma unroll
-        for (int p = 0; p < NJ * 4; ++p) {
-            if (off[p] < 0) continue;
-            const int c = p * 8 + (lane >> 3);
-            const int col = c0 + c;
-            const int img = col / T;
-            const float lim = slot_lim[img - img0];
-            const float4 v = *reinterpret_cast<const float4*>(&stage[c * LD + 4 * (lane & 7)]);
-            const float vv[4] = {v.x, v.y, v.z, v.w};
-            int pos = slot_cnt[img - img0] + off[p];
+        for (int j = 0; j < NJ; ++j) {
+            if (off[i][j] < 0) continue;
+            const int img = img0 + sl_j[j];
+            const unsigned tm = (unsigned)(c0 + j * 32 + (lane & 31) - img * T) * (unsigned)Mtot;
+            int pos = slot_cnt[sl_j[j]] + off[i][j];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (!list_keeps(vv[u], lpd[u], lim)) continue;
+            for (int r = 0; r < 16; ++r) {
+                if (!(acc[i][j][r] - lpd[r] < lim_j[j])) continue;
                 if (pos < L.cap) {
                     const size_t o = (size_t)img * L.cap + pos;
-                    L.ent[o] = (unsigned)(col - img * T) * (unsigned)Mtot + (unsigned)(row0 + u);
-                    L.val[o] = vv[u];
+                    L.ent[o] = tm + (unsigned)(r0 + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3));
+                    L.val[o] = acc[i][j][r];
                 }
                 ++pos;
             }
         }
-        __syncthreads();
-        for (int q = tid; q < nslot; q += blockDim.x) slot_cnt[q] = 0;  // next row block counts afresh
     }
-#pragma unroll
-    for (int j = 0; j < NJ; ++j)
-        if (c0 + j * 32 + (lane & 31) < Ncols && emin_j[j] < INFINITY)
-            atomicMin(&slot_min[img_j[j] - img0], float_order_key(emin_j[j]));
-    __syncthreads();
-    for (int q = tid; q < nslot; q += blockDim.x)
-        if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img0 + q], slot_min[q]);
 }
 
 }  // namespace rome

@@ -124,14 +124,14 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
 #undef LOWP_LOAD
 #undef LOWP_STORE
 #undef LOWP_COMPUTE
-    // the operand stages are free now: 8 waves x 128 columns x 36 floats of staging
-    float* stage = reinterpret_cast<float*>(&ABs[0][0][0][0]) + wave * (TJ * 32 * 36);
-    static_assert(sizeof(ABs) >= 8 * TJ * 32 * 36 * sizeof(float), "epilogue staging does not fit");
     if constexpr (LIST) {
         gemm_epilogue_list<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale,
-                                             a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min,
-                                             slot_vb, slot_lim, slot_cnt, stage);
+                                             a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb,
+                                             slot_lim, slot_cnt);
     } else {
+        // the operand stages are free now: 8 waves x 128 columns x 36 floats of staging
+        float* stage = reinterpret_cast<float*>(&ABs[0][0][0][0]) + wave * (TJ * 32 * 36);
+        static_assert(sizeof(ABs) >= 8 * TJ * 32 * 36 * sizeof(float), "epilogue staging does not fit");
         gemm_epilogue_staged<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale,
                                                a.E, a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min, stage);
     }

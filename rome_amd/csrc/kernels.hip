@@ -246,9 +246,8 @@ __global__ __launch_bounds__(256) void k_coarse_aterm(const float* __restrict__ 
 template <bool LIST>
 __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
     constexpr int BM = 128, BN = 128, BK = 16;
-    // operand stages; with a list output the epilogue stages 4 waves x 64 columns x 36 floats here
-    constexpr int kOps = 2 * 2 * BK * BM, kStage = 4 * 64 * 36;
-    __shared__ __attribute__((aligned(16))) float smem[LIST && kStage > kOps ? kStage : kOps];
+    constexpr int kOps = 2 * 2 * BK * BM;  // operand stages
+    __shared__ __attribute__((aligned(16))) float smem[kOps];
     typedef float Stage[BK][BM];
     Stage* As = reinterpret_cast<Stage*>(smem);
     Stage* Bs = reinterpret_cast<Stage*>(smem + 2 * BK * BM);
@@ -315,9 +314,9 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
         __syncthreads();
     }
     if constexpr (LIST)
-        gemm_epilogue_list<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr, a.valid,
+        gemm_epilogue_list<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr,
                                             a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb,
-                                            slot_lim, slot_cnt, smem + wave * (64 * 36));
+                                            slot_lim, slot_cnt);
     else
         gemm_epilogue<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr, a.E, a.valid,
                                        a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
