@@ -35,14 +35,14 @@ struct Lowp<_Float16> {
     static DEV f16v mfma(v8 a, v8 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
 };
 
-// 256x256 output tile per block of 8 waves (4 along rotations x 2 along columns); each wave owns
+// Full pose term E: 256x256 output tile per block of 8 waves (4 along rotations x 2 along columns); each wave owns
 // 64x128 = 2x4 accumulator blocks (128 accumulator registers, two waves per SIMD).  BK = 64: per
 // 16-k step a wave reads 6 fragments from LDS for 8 MFMAs; the global loads of chunk c+1 (held in
 // eight named 16-byte registers: an array here would live in scratch) are in flight while chunk c is
 // multiplied by both waves of the SIMD.  Two LDS stages of 72 KB, one block per CU; after the main
 // loop the same 144 KB stage the E rows.  (4x2 waves of 64x128 beat 2x4 waves of 128x64 by 15 % in
 // the main loop at K = 512.)
-template <typename T, bool LIST>
+template <typename T>
 __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
     constexpr int BM = 256, BN = 256, BK = 64, LDW = BK + 8;  // 144-byte LDS rows: conflict-free b128 reads
     typedef typename Lowp<T>::v8 v8;
@@ -50,9 +50,6 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
     // operand stages [A|B][stage][row][k]; the epilogue reuses the whole array
     __shared__ __attribute__((aligned(16))) T ABs[2][2][BM][LDW];
     __shared__ int slot_min[epi_slots<BN>()];
-    __shared__ int slot_vb[LIST ? epi_slots<BN>() : 1];
-    __shared__ float slot_lim[LIST ? epi_slots<BN>() : 1];
-    __shared__ int slot_cnt[LIST ? epi_slots<BN>() : 1];
     auto& As = ABs[0];
     auto& Bs = ABs[1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -124,17 +121,108 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
 #undef LOWP_LOAD
 #undef LOWP_STORE
 #undef LOWP_COMPUTE
-    if constexpr (LIST) {
-        gemm_epilogue_list<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale,
-                                             a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb,
-                                             slot_lim, slot_cnt);
-    } else {
-        // the operand stages are free now: 8 waves x 128 columns x 36 floats of staging
-        float* stage = reinterpret_cast<float*>(&ABs[0][0][0][0]) + wave * (TJ * 32 * 36);
-        static_assert(sizeof(ABs) >= 8 * TJ * 32 * 36 * sizeof(float), "epilogue staging does not fit");
-        gemm_epilogue_staged<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale,
-                                               a.E, a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min, stage);
+    // the operand stages are free now: 8 waves x 128 columns x 36 floats of staging
+    float* stage = reinterpret_cast<float*>(&ABs[0][0][0][0]) + wave * (TJ * 32 * 36);
+    static_assert(sizeof(ABs) >= 8 * TJ * 32 * 36 * sizeof(float), "epilogue staging does not fit");
+    gemm_epilogue_staged<true, TI, TJ, BN>(acc, m0 + wm * 64, n0 + wn * 128, n0, lane, tid, a.At, a.inv_scale, a.E,
+                                           a.valid, a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, slot_min, stage);
+}
+
+// List output: 256x128 tiles per block of 4 waves (one per SIMD, each 64x128 = 2x4 accumulator
+// blocks), one 55 KB LDS stage (the next chunk waits in registers), two blocks per CU: one block's
+// epilogue runs beside the other's MFMA loop.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_gemm_lowp_list(LowpGemmArgs a) {
+    constexpr int BM = 256, BN = 128, BK = 64, LDW = BK + 8;
+    typedef typename Lowp<T>::v8 v8;
+    constexpr int TI = 2, TJ = 4;
+    __shared__ __attribute__((aligned(16))) T As[BM][LDW];
+    __shared__ __attribute__((aligned(16))) T Bs[BN][LDW];
+    __shared__ int slot_min[epi_slots<BN>()];
+    __shared__ int slot_vb[epi_slots<BN>()];
+    __shared__ float slot_lim[epi_slots<BN>()];
+    __shared__ int slot_cnt[epi_slots<BN>()];
+    const int tid = threadIdx.x, lane = tid & 63, wm = tid >> 6;
+    int bm, bn;
+    gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
+    const int m0 = bm * BM, n0 = bn * BN;
+    const T* A = reinterpret_cast<const T*>(a.A);
+    const T* B = reinterpret_cast<const T*>(a.B);
+    f16v acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    const int nk = a.K16 / BK;
+    // chunk: A 256 x 64 (8 pieces of 16 bytes per thread), B 128 x 64 (4 per thread); rows tid / 8 + 32 h
+    const int prow = tid >> 3, pcol = (tid & 7) * 8;
+    const T* Ag = A + (size_t)(m0 + prow) * a.K16 + pcol;
+    const T* Bg = B + (size_t)(n0 + prow) * a.K16 + pcol;
+    const size_t q32 = (size_t)32 * a.K16;
+    uint4 ra0, ra1, ra2, ra3, ra4, ra5, ra6, ra7, rb0, rb1, rb2, rb3;
+#define L2_LOAD(kc)                                                          \
+    do {                                                                     \
+        const T* ag = Ag + (kc) * BK;                                        \
+        const T* bg = Bg + (kc) * BK;                                        \
+        ra0 = *reinterpret_cast<const uint4*>(ag);                           \
+        ra1 = *reinterpret_cast<const uint4*>(ag + q32);                     \
+        ra2 = *reinterpret_cast<const uint4*>(ag + 2 * q32);                 \
+        ra3 = *reinterpret_cast<const uint4*>(ag + 3 * q32);                 \
+        ra4 = *reinterpret_cast<const uint4*>(ag + 4 * q32);                 \
+        ra5 = *reinterpret_cast<const uint4*>(ag + 5 * q32);                 \
+        ra6 = *reinterpret_cast<const uint4*>(ag + 6 * q32);                 \
+        ra7 = *reinterpret_cast<const uint4*>(ag + 7 * q32);                 \
+        rb0 = *reinterpret_cast<const uint4*>(bg);                           \
+        rb1 = *reinterpret_cast<const uint4*>(bg + q32);                     \
+        rb2 = *reinterpret_cast<const uint4*>(bg + 2 * q32);                 \
+        rb3 = *reinterpret_cast<const uint4*>(bg + 3 * q32);                 \
+    } while (0)
+#define L2_STORE()                                                           \
+    do {                                                                     \
+        *reinterpret_cast<uint4*>(&As[prow][pcol]) = ra0;                    \
+        *reinterpret_cast<uint4*>(&As[prow + 32][pcol]) = ra1;               \
+        *reinterpret_cast<uint4*>(&As[prow + 64][pcol]) = ra2;               \
+        *reinterpret_cast<uint4*>(&As[prow + 96][pcol]) = ra3;               \
+        *reinterpret_cast<uint4*>(&As[prow + 128][pcol]) = ra4;              \
+        *reinterpret_cast<uint4*>(&As[prow + 160][pcol]) = ra5;              \
+        *reinterpret_cast<uint4*>(&As[prow + 192][pcol]) = ra6;              \
+        *reinterpret_cast<uint4*>(&As[prow + 224][pcol]) = ra7;              \
+        *reinterpret_cast<uint4*>(&Bs[prow][pcol]) = rb0;                    \
+        *reinterpret_cast<uint4*>(&Bs[prow + 32][pcol]) = rb1;               \
+        *reinterpret_cast<uint4*>(&Bs[prow + 64][pcol]) = rb2;               \
+        *reinterpret_cast<uint4*>(&Bs[prow + 96][pcol]) = rb3;               \
+    } while (0)
+    L2_LOAD(0);
+    L2_STORE();
+    __syncthreads();
+    for (int kc = 0; kc < nk; ++kc) {
+        if (kc + 1 < nk) L2_LOAD(kc + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            v8 av[TI], bv[TJ];
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+                av[i] = *reinterpret_cast<const v8*>(&As[wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int j = 0; j < TJ; ++j)
+                bv[j] = *reinterpret_cast<const v8*>(&Bs[j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) acc[i][j] = Lowp<T>::mfma(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+        if (kc + 1 < nk) {
+            L2_STORE();
+            __syncthreads();
+        }
     }
+#undef L2_LOAD
+#undef L2_STORE
+    gemm_epilogue_list<true, TI, TJ, BN>(acc, m0 + wm * 64, n0, n0, lane, tid, a.At, a.inv_scale, a.emin_key, a.Mtot,
+                                         a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb, slot_lim, slot_cnt);
 }
 
 // sF: power of two that brings max |FcT| into [0.5, 1) (exact rescaling)
@@ -214,11 +302,11 @@ hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a)
     const bool list = a.list.count != nullptr;
     if (!list && !a.E) return hipErrorInvalidValue;
     if (precision == 1) {
-        if (list) k_gemm_lowp<__bf16, true><<<tiles, 512, 0, s>>>(a);
-        else k_gemm_lowp<__bf16, false><<<tiles, 512, 0, s>>>(a);
+        if (list) k_gemm_lowp_list<__bf16><<<2 * tiles, 256, 0, s>>>(a);
+        else k_gemm_lowp<__bf16><<<tiles, 512, 0, s>>>(a);
     } else if (precision == 2) {
-        if (list) k_gemm_lowp<_Float16, true><<<tiles, 512, 0, s>>>(a);
-        else k_gemm_lowp<_Float16, false><<<tiles, 512, 0, s>>>(a);
+        if (list) k_gemm_lowp_list<_Float16><<<2 * tiles, 256, 0, s>>>(a);
+        else k_gemm_lowp<_Float16><<<tiles, 512, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
     }
