@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--perturbation", type=float, default=0.25)
     ap.add_argument("--cpu-sample", type=int, default=0, help="images timed on the CPU oracle (0: 2 per thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=["fp32", "bf16", "fp16"], default="bf16",
+    ap.add_argument("--precision", choices=["fp32", "bf16", "fp16"], default="fp16",
                     help="mixed arm timed beside the FP32 arm (fp32: FP32 arm only)")
     ap.add_argument("--only-mixed", action="store_true",
                     help="time the mixed arm alone (profiling runs: no pose-agreement gate, no FP32 arm)")
@@ -242,10 +242,11 @@ def main():
     gemm_flops = 2.0 * M * (B * T) * (2 * Pc)  # algorithmic: Re<CF, XS> = 2 real MACs per complex pixel
     gemm_tflops = gemm_flops * gemm_calls / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     peak = MFMA_PEAK_TFLOPS[head]
+    listed = stats.get("mode") == 0  # coarse list output (the product path) rather than the full pose term
     kname = {"fp32": "k_coarse_gemm (coarse cross term, FP32 MFMA)",
-             "bf16": "k_gemm_lowp<bf16> (coarse cross term, BF16 MFMA)",
-             "fp16": "k_gemm_lowp<fp16> (coarse cross term, FP16 MFMA)"}[head]
-    traffic = measured_traffic("k_gemm_lowp<_Float16, list>") if head == "fp16" and stats.get("mode") == 0 else None
+             "bf16": "k_gemm_lowp%s<bf16> (coarse cross term, BF16 MFMA)" % ("_list" if listed else ""),
+             "fp16": "k_gemm_lowp%s<fp16> (coarse cross term, FP16 MFMA)" % ("_list" if listed else "")}[head]
+    traffic = measured_traffic("k_gemm_lowp_list<_Float16>") if head == "fp16" and listed else None
     roofline = {"kernel": kname, "bound": "mfma",
                 "achieved": round(gemm_tflops, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(gemm_tflops / peak, 4), "traffic": traffic,
