@@ -135,6 +135,8 @@ struct rome_ctx {
     static constexpr int kCandCap = 8192;
     DevBuf<unsigned char> valid;
     DevBuf<float> FcT;
+    DevBuf<float> Qc;  // [Kq][Mpad] |FcT|^2, the A-term operand
+    int Kq = 16;
     // mixed-precision arm: scaled BF16/FP16 copies of the coarse table and of the image operand
     DevBuf<int> fct_absmax;
     DevBuf<uint16_t> A16, W16;
@@ -325,6 +327,9 @@ int ensure_ready(rome_ctx* ctx) {
         HIPCHK(launch_project_coarse(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d,
                                      ctx->Ai_c.p, ctx->R, ctx->Mtot, ctx->c_xy.p, ctx->Pc, ctx->Pc_pad, my_r_max,
                                      ctx->Mpad, ctx->FcT.p));
+        ctx->Kq = (ctx->Pc + 15) / 16 * 16;
+        HIPCHK(ctx->Qc.reserve((size_t)ctx->Kq * ctx->Mpad));
+        HIPCHK(launch_fct_norm(ctx->stream, ctx->FcT.p, ctx->Pc, ctx->Pc_pad, ctx->Mpad, ctx->Kq, ctx->Qc.p));
         if (ctx->is3d) {
             HIPCHK(ctx->cube.reserve(4 * ctx->grid_n * ctx->K));
             HIPCHK(launch_build_cube(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->K, ctx->pad, ctx->padF, ctx->cube.p));
@@ -428,7 +433,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         if (Npad > Ncols)  // rows past the last column only feed discarded outputs; keep them finite
             HIPCHK(hipMemsetAsync(ctx->W16.p + (size_t)Ncols * K16, 0, (size_t)(Npad - Ncols) * K16 * 2, s));
     }
-    HIPCHK(ctx->MC2c.reserve((size_t)n * ctx->Pc));
+    const int ldn = (n + 127) / 128 * 128;  // images of the A-term operand, padded with zero columns
+    HIPCHK(ctx->MC2c.reserve((size_t)ctx->Kq * ldn));
+    HIPCHK(hipMemsetAsync(ctx->MC2c.p, 0, (size_t)ctx->Kq * ldn * sizeof(float), s));
     HIPCHK(ctx->C2c.reserve(n));
     HIPCHK(ctx->C2f.reserve(n));
     const size_t nPf = (size_t)n * ctx->Pf;
@@ -463,6 +470,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     pa.f_ires = ctx->f_ires.p;
     pa.WcT = prec == ROME_PREC_FP32 ? ctx->WcT.p : nullptr;
     pa.MC2c = ctx->MC2c.p;
+    pa.mc2_ld = ldn;
     pa.C2c = ctx->C2c.p;
     pa.Xf = ctx->Xf.p;
     pa.Minvf = ctx->Minvf.p;
@@ -499,7 +507,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->At.reserve((size_t)n * ctx->Mpad));
     WeightParams wp = weight_params(ctx);
     stage_begin(ctx, 1);
-    HIPCHK(launch_coarse_aterm(s, n, ctx->FcT.p, M, ctx->Mpad, ctx->Pc, ctx->Pc_pad, ctx->MC2c.p, ctx->At.p));
+    HIPCHK(launch_coarse_aterm(s, n, ctx->Qc.p, ctx->Kq, M, ctx->Mpad, ctx->MC2c.p, ldn, ctx->At.p));
     if (ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass) {
         HIPCHK(ctx->reach.reserve(n));
         HIPCHK(launch_list_reach(s, n, wp, ctx->reach.p));
@@ -1001,6 +1009,7 @@ int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
     }
     ctx->Pc = (int)cn.size();
     ctx->Pc_pad = (ctx->Pc + 7) / 8 * 8;
+    ctx->Kq = (ctx->Pc + 15) / 16 * 16;
     ctx->Pf = (int)fn.size();
     if (ctx->Pf > 16 * 1024) return fail(ctx, ROME_ERR_UNSUPPORTED, "current_size too large for the post kernel");
     ctx->f_nfine_h = fn;

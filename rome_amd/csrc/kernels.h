@@ -34,7 +34,8 @@ struct PrepArgs {
     const int* f_ires;
     // outputs
     float* WcT;             // [2*Pc_pad][Npad]
-    float* MC2c;            // [n][Pc]
+    float* MC2c;            // [Kq][mc2_ld] (transposed: A-term operand; rows >= Pc, columns >= n zero)
+    int mc2_ld;
     double* C2c;            // [n]  1/2 (sum Minv|X|^2 + hxi2), coarse
     float2* Xf;             // [n][Pf]
     float* Minvf;           // [n][Pf]
@@ -295,8 +296,11 @@ hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4*
 hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
                                 int nrot, int size, int my_r_max, float2* out);
 hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a);
-hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
-                               const float* MC2c, float* At);
+// Qc[p][m] = |FcT[p][m]|^2 (Re and Im rows), zero for p in [Pc, Kq); Kq = Pc rounded up to 16
+hipError_t launch_fct_norm(hipStream_t s, const float* FcT, int Pc, int Pc_pad, int Mpad, int Kq, float* Qc);
+// At[img][m] = 0.5 * sum_p Qc[p][m] * MC2cT[p][img] (FP32 MFMA); Mpad and ldn multiples of 128
+hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* Qc, int Kq, int Mtot, int Mpad, const float* MC2cT,
+                               int ldn, float* At);
 hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a);
 struct CandArgs {  // per-image candidate list of the weight pass: [n][cap]
     double* w;

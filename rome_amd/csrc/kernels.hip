@@ -162,7 +162,7 @@ __global__ void k_prep_images(PrepArgs a) {
         const double minv = 1.0 / (a.sigma2_fudge * s2[a.c_ires[p]]);
         const float ctf = a.ctf_in_diff ? fc[nf] : 1.f;
         const float mc = (float)minv * ctf;
-        a.MC2c[(size_t)img * a.Pc + p] = mc * ctf;
+        a.MC2c[(size_t)p * a.mc2_ld + img] = mc * ctf;
         x2 += minv * ((double)X.x * X.x + (double)X.y * X.y);
         if (!a.WcT) continue;  // the mixed-precision arm writes its own operand
         for (int t = 0; t < a.T; ++t) {
@@ -202,41 +202,99 @@ __global__ void k_prep_images(PrepArgs a) {
     if (threadIdx.x == 0) a.C2f[img] = 0.5 * (s + a.hxi2[img]);
 }
 
-// A-term: At[img][m] = 0.5 * sum_p MC2c[img][p] |F_m(p)|^2, rows padded to Mpad (zero beyond Mtot).
-// Block = 256 rotations x kATermImgs images: |F_m(p)|^2 is formed once per pixel and reused for every
-// image of the block; the images' weights pass through LDS in chunks of 256 pixels.
-constexpr int kATermImgs = 16;
-__global__ __launch_bounds__(256) void k_coarse_aterm(const float* __restrict__ FcT, int Mtot, int Mpad, int Pc,
-                                                      int Pc_pad, int n, const float* __restrict__ MC2c,
-                                                      float* __restrict__ At) {
-    __shared__ float w[kATermImgs][256];
-    const int m = blockIdx.x * 256 + threadIdx.x;
-    const int i0 = blockIdx.y * kATermImgs;
-    const int ni = min(kATermImgs, n - i0);
-    float acc[kATermImgs];
-#pragma unroll
-    for (int i = 0; i < kATermImgs; ++i) acc[i] = 0.f;
-    for (int p0 = 0; p0 < Pc; p0 += 256) {
-        const int np = min(256, Pc - p0);
-        __syncthreads();
-        for (int q = threadIdx.x; q < kATermImgs * 256; q += 256) {
-            const int i = q >> 8, p = q & 255;
-            w[i][p] = (i < ni && p < np) ? MC2c[(size_t)(i0 + i) * Pc + p0 + p] : 0.f;
-        }
-        __syncthreads();
-        if (m < Mtot) {
-            for (int p = 0; p < np; ++p) {
-                const float fr = FcT[(size_t)(p0 + p) * Mpad + m], fi = FcT[(size_t)(Pc_pad + p0 + p) * Mpad + m];
-                const float q = fr * fr + fi * fi;
-#pragma unroll
-                for (int i = 0; i < kATermImgs; ++i) acc[i] += w[i][p] * q;
-            }
-        }
+// Squared magnitudes of the coarse projections, the A-term's K-major operand: Qc[p][m] = Re^2 + Im^2 of
+// FcT for p < Pc (zero for p in [Pc, Kq)), once per reference change.
+__global__ void k_fct_norm(const float* __restrict__ FcT, int Pc, int Pc_pad, int Mpad, int Kq, float* __restrict__ Qc) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)Kq * Mpad) return;
+    const int p = (int)(i / Mpad);
+    const size_t m = i - (size_t)p * Mpad;
+    float q = 0.f;
+    if (p < Pc) {
+        const float fr = FcT[(size_t)p * Mpad + m], fi = FcT[(size_t)(Pc_pad + p) * Mpad + m];
+        q = fr * fr + fi * fi;
     }
-    if (m < Mpad) {
+    Qc[i] = q;
+}
+
+// A-term as an FP32 MFMA GEMM: At[img][m] = 0.5 * sum_p Qc[p][m] * MC2cT[p][img] (rows padded to Mpad,
+// zero beyond Mtot).  128 rotations x 128 images per block of 4 waves (2 x 2, each 2 x 2 blocks of
+// v_mfma_f32_32x32x2f32), BK = 16, two LDS stages: the rotation table is read once per 128 images.
+__global__ __launch_bounds__(256) void k_coarse_aterm(const float* __restrict__ Qc, int Kq, int Mtot, int Mpad,
+                                                      const float* __restrict__ MC2cT, int ldn, int n,
+                                                      float* __restrict__ At) {
+    constexpr int BM = 128, BN = 128, BK = 16;
+    __shared__ __attribute__((aligned(16))) float As[2][BK][BM];
+    __shared__ __attribute__((aligned(16))) float Bs[2][BK][BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int lr = tid >> 5, lc = (tid & 31) * 4;  // rows lr and lr + 8 of a 16 x 128 chunk
+    f16v acc[2][2];
 #pragma unroll
-        for (int i = 0; i < kATermImgs; ++i)
-            if (i < ni) At[(size_t)(i0 + i) * Mpad + m] = m < Mtot ? 0.5f * acc[i] : 0.f;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    const float* Ag = Qc + (size_t)lr * Mpad + m0 + lc;
+    const float* Bg = MC2cT + (size_t)lr * ldn + n0 + lc;
+    const size_t a8 = (size_t)8 * Mpad, b8 = (size_t)8 * ldn;
+    float4 ra0 = *reinterpret_cast<const float4*>(Ag), ra1 = *reinterpret_cast<const float4*>(Ag + a8);
+    float4 rb0 = *reinterpret_cast<const float4*>(Bg), rb1 = *reinterpret_cast<const float4*>(Bg + b8);
+    *reinterpret_cast<float4*>(&As[0][lr][lc]) = ra0;
+    *reinterpret_cast<float4*>(&As[0][lr + 8][lc]) = ra1;
+    *reinterpret_cast<float4*>(&Bs[0][lr][lc]) = rb0;
+    *reinterpret_cast<float4*>(&Bs[0][lr + 8][lc]) = rb1;
+    __syncthreads();
+    const int nk = Kq / BK;
+    for (int kc = 0; kc < nk; ++kc) {
+        const int cur = kc & 1;
+        if (kc + 1 < nk) {
+            const float* an = Ag + (size_t)(kc + 1) * BK * Mpad;
+            const float* bn = Bg + (size_t)(kc + 1) * BK * ldn;
+            ra0 = *reinterpret_cast<const float4*>(an);
+            ra1 = *reinterpret_cast<const float4*>(an + a8);
+            rb0 = *reinterpret_cast<const float4*>(bn);
+            rb1 = *reinterpret_cast<const float4*>(bn + b8);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 2) {
+            float av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = As[cur][kk + (lane >> 5)][wm * 64 + i * 32 + (lane & 31)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[cur][kk + (lane >> 5)][wn * 64 + j * 32 + (lane & 31)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kc + 1 < nk) {
+            *reinterpret_cast<float4*>(&As[cur ^ 1][lr][lc]) = ra0;
+            *reinterpret_cast<float4*>(&As[cur ^ 1][lr + 8][lc]) = ra1;
+            *reinterpret_cast<float4*>(&Bs[cur ^ 1][lr][lc]) = rb0;
+            *reinterpret_cast<float4*>(&Bs[cur ^ 1][lr + 8][lc]) = rb1;
+        }
+        __syncthreads();
+    }
+    // lane: column img = n0 + wn*64 + j*32 + lane%32, rows m0 + wm*64 + i*32 + 8q + 4(lane/32) + u
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int img = n0 + wn * 64 + j * 32 + (lane & 31);
+        if (img >= n) continue;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * q + 4 * (lane >> 5);
+                float4 v;
+                v.x = m < Mtot ? 0.5f * acc[i][j][4 * q] : 0.f;
+                v.y = m + 1 < Mtot ? 0.5f * acc[i][j][4 * q + 1] : 0.f;
+                v.z = m + 2 < Mtot ? 0.5f * acc[i][j][4 * q + 2] : 0.f;
+                v.w = m + 3 < Mtot ? 0.5f * acc[i][j][4 * q + 3] : 0.f;
+                *reinterpret_cast<float4*>(&At[(size_t)img * Mpad + m]) = v;
+            }
     }
 }
 
@@ -1044,10 +1102,19 @@ hipError_t launch_pack(hipStream_t s, const PackArgs& a) {
     return hipGetLastError();
 }
 
-hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* FcT, int Mtot, int Mpad, int Pc, int Pc_pad,
-                               const float* MC2c, float* At) {
-    dim3 grid((Mpad + 255) / 256, (n + kATermImgs - 1) / kATermImgs);
-    k_coarse_aterm<<<grid, 256, 0, s>>>(FcT, Mtot, Mpad, Pc, Pc_pad, n, MC2c, At);
+hipError_t launch_fct_norm(hipStream_t s, const float* FcT, int Pc, int Pc_pad, int Mpad, int Kq, float* Qc) {
+    const size_t tot = (size_t)Kq * Mpad;
+    k_fct_norm<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(FcT, Pc, Pc_pad, Mpad, Kq, Qc);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_coarse_aterm(hipStream_t s, int n, const float* Qc, int Kq, int Mtot, int Mpad, const float* MC2cT,
+                               int ldn, float* At) {
+    if (n <= 0) return hipSuccess;
+    if (Kq % 16 || Mpad % 128 || ldn % 128 || ldn < n) return hipErrorInvalidValue;
+    dim3 grid(Mpad / 128, ldn / 128);
+    k_coarse_aterm<<<grid, 256, 0, s>>>(Qc, Kq, Mtot, Mpad, MC2cT, ldn, n, At);
     LAUNCH_CHECK();
     return hipSuccess;
 }
