@@ -126,7 +126,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--perturbation", type=float, default=0.25)
     ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="images timed on the CPU oracle (0: 10 per thread, about 10 s of CPU work)")
+                    help="images timed on the CPU oracle (0: 25 per thread, about 15 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "bf16", "fp16"], default="fp16",
                     help="mixed arm timed beside the FP32 arm (fp32: FP32 arm only)")
@@ -258,7 +258,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
-        ns = args.cpu_sample or 10 * threads
+        ns = args.cpu_sample or 25 * threads
         ips, cdt = cpu_baseline(prob, F, ctf, np.arange(ns), threads)
         cpu = {"value": round(ips, 3), "unit": "images/s", "cores": threads, "kind": "port",
                "sample": f"{ns} particles of the same workload through the FP64 oracle E-step ({cdt:.1f}s)"}
