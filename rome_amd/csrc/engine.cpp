@@ -725,6 +725,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.thr = thr_p;
         pp.dmin = dmin_p;
         pp.task_off = ctx->task_off.p;
+        pp.task_img = ctx->task_img.p;
         pp.task_m = ctx->task_m.p;
         pp.task_nt = ctx->task_nt.p;
         pp.task_blk0 = ctx->task_blk0.p;
@@ -773,7 +774,13 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             pp.slice_w = ctx->slice_w.p;
         }
         stage_begin(ctx, 8);
-        HIPCHK(launch_post(s, n, NOR, NOT, pp));
+        hipError_t pe = hipErrorNotSupported;
+        if (binned) {
+            HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
+            pe = launch_post_slices(s, ntasks, NOR, NOT, pp);
+        }
+        if (pe == hipErrorNotSupported) pe = launch_post(s, n, NOR, NOT, pp);
+        HIPCHK(pe);
         stage_end(ctx, 8);
         if (binned) {
             BpArgs ba;

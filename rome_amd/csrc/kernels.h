@@ -172,6 +172,7 @@ struct PostArgs {
     const double* thr;
     const double* dmin;
     const long long* task_off;  // [n+1]
+    const int* task_img;        // [tasks]
     const int* task_m;
     const int* task_nt;
     const long long* task_blk0;
@@ -351,6 +352,9 @@ hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int
                             int* blk_task, int* blk_t);
 hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a);
 hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a);
+// The binned 3D form with one block per slice (hipErrorNotSupported outside NOR = 8, NOT = 4,
+// Pf <= 8192): wnorm must be zero before the launch.
+hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a);
 hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
                           long long nblocks, const float* E, const double* C2c, int M, int T, double* D2);
 // Gathers up to kPackSegs device arrays into one contiguous staging buffer (one launch), so that a

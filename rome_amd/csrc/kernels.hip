@@ -927,6 +927,121 @@ __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
     }
 }
 
+// 3D (binned backprojection) form of k_post with one block per (task, over_rot) slice instead of one
+// per image: eight times the blocks and short dependency chains.  The slice's 8 blocks go to one XCD
+// back to back (workgroups are dispatched to the 8 XCDs round-robin), so the image's pixels and the
+// phase rows they share come from that XCD's L2.  Per block: the weights of its entries (the same
+// expression as k_post), the slice value and weight in the same sample order, sigma2 shells of its
+// pixels added to the group's spectrum, and its share of the per-image / per-class sums (atomics; the
+// caller zeroes wnorm first).
+template <int NOR, int NOT, int PPT, int NT>
+__global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) {
+    __shared__ double wts[kMaxT * NOT];
+    __shared__ double red[NT / 64];
+    __shared__ double shell[kMaxShell];
+    const int tid = threadIdx.x;
+    const long long G = ntask * NOR, per = (G + 7) / 8;
+    const long long L = blockIdx.x;
+    if ((L >> 3) >= per) return;
+    const long long w = (L & 7) * per + (L >> 3);
+    if (w >= G) return;
+    const long long q = w / NOR;
+    const int o = (int)(w - q * NOR);
+    const int img = a.task_img[q];
+    const int g = a.group[img];
+    const double sumw = a.sumw[img], thr = a.thr[img], dmin = a.dmin[img];
+    const int m = a.task_m[q], nt = a.task_nt[q];
+    const long long b0 = a.task_blk0[q];
+    const int k = m / a.R, r = m % a.R, dir = r / a.PSI;
+    const int u = a.m2u[m];
+    const double ox = a.old_offx[img], oy = a.old_offy[img];
+    for (int s = tid; s < a.oriF; s += NT) shell[s] = 0.;
+    for (int e = tid; e < nt * NOT; e += NT) {
+        const int tb = e / NOT, ot = e - tb * NOT;
+        const int t = a.blk_t[b0 + tb];
+        const double d2 = a.D2[(size_t)(b0 + tb) * (NOR * NOT) + o * NOT + ot];
+        const double dd = d2 - dmin;
+        double wv = a.pdf_dir[(size_t)k * a.D + dir] *
+                    pdf_offset(a.trans_x[t], a.trans_y[t], ox, oy, a.prior_x[k], a.prior_y[k], a.sigma2_offset);
+        wv = dd > 700. ? 0. : wv * exp(-dd);
+        wts[e] = wv;
+    }
+    __syncthreads();
+    const float2* X = a.Xf + (size_t)img * a.Pf;
+    const float* Ct = a.Ctff + (size_t)img * a.Pf;
+    const float2* NM = a.NMf + (size_t)img * a.Pf;
+    const float2* Pr = a.Pfine + ((size_t)u * NOR + o) * a.Pf;
+    float2 Xr[PPT], NMr[PPT], CF[PPT], acc[PPT];
+    double wsig[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int p = tid + j * NT;
+        const bool in = p < a.Pf;
+        Xr[j] = in ? X[p] : make_float2(0.f, 0.f);
+        NMr[j] = in ? NM[p] : make_float2(0.f, 0.f);
+        const float2 f = in ? Pr[p] : make_float2(0.f, 0.f);
+        const float c = in ? Ct[p] : 0.f;
+        CF[j] = make_float2(c * f.x, c * f.y);
+        acc[j] = make_float2(0.f, 0.f);
+        wsig[j] = 0.;
+    }
+    double wslice = 0., t_off = 0.;
+    for (int tb = 0; tb < nt; ++tb) {
+        const int t = a.blk_t[b0 + tb];
+        for (int ot = 0; ot < NOT; ++ot) {
+            const double wv = wts[tb * NOT + ot];
+            if (!(wv >= thr)) continue;
+            const double wn = wv / sumw;
+            wslice += wn;
+            if (tid == 0) {
+                const double sx = a.shift_x[t * NOT + ot], sy = a.shift_y[t * NOT + ot];
+                const double offx = a.prior_x[k] - ox - sx, offy = a.prior_y[k] - oy - sy;
+                t_off += wn * (offx * offx + offy * offy);
+            }
+            const float wf = (float)wn;
+            const float2* S = a.Sf + ((size_t)t * NOT + ot) * a.Pf;
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                const int p = tid + j * NT;
+                if (p < a.Pf) {
+                    const float2 s = S[p];
+                    const float2 xs = cmul(Xr[j], s);
+                    const float dr = CF[j].x - xs.x, di = CF[j].y - xs.y;
+                    wsig[j] += wn * (double)(dr * dr + di * di);
+                    const float2 ns = cmul(NMr[j], s);
+                    acc[j].x += wf * ns.x;
+                    acc[j].y += wf * ns.y;
+                }
+            }
+        }
+    }
+    const size_t sl = (size_t)q * NOR + o;
+    if (tid == 0) a.slice_w[sl] = wslice;
+    if (!(wslice > 0.)) return;  // the same value in every thread: the whole block leaves
+    float2* V = a.slice_val + sl * a.Pf;
+    double tot = 0.;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int p = tid + j * NT;
+        if (p < a.Pf) {
+            V[p] = acc[j];
+            atomicAdd(&shell[a.f_ires[p]], wsig[j]);
+            tot += wsig[j];
+        }
+    }
+    tot = block_reduce(tot, [](double x, double y) { return x + y; }, red);
+    __syncthreads();
+    for (int s = tid; s < a.oriF; s += NT)
+        if (shell[s] != 0.) atomicAdd(&a.wsum_sigma2[(size_t)g * a.oriF + s], shell[s]);
+    if (tid == 0) {
+        atomicAdd(&a.wnorm[img], tot);
+        atomicAdd(&a.wsum_sumw_group[g], wslice);
+        atomicAdd(&a.wsum_pdf_class[k], wslice);
+        atomicAdd(&a.wsum_pdf_dir[(size_t)k * a.D + dir], wslice);
+        atomicAdd(&a.wsum_scalars[0], t_off);
+    }
+}
+
 // ------------------------------------------------------------------ storeWeightedSums (src/map3d_optimizer.cpp:1933-2057)
 __global__ void k_store_sums(StoreArgs a) {
     const int img = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1195,6 +1310,16 @@ hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a
     else if (NOR == 1 && NOT == 1) e = post_ppt<1, 1>(s, n, a.Pf, a);
     else return hipErrorInvalidValue;
     if (e != hipSuccess) return e;
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a) {
+    if (ntask <= 0) return hipSuccess;
+    if (NOR != 8 || NOT != 4 || !a.slice_val || a.Pf > 8 * 1024 || !a.is3d) return hipErrorNotSupported;
+    const long long blocks = 8 * ((ntask * NOR + 7) / 8);
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    k_post_slice<8, 4, 8, 1024><<<(unsigned)blocks, 1024, 0, s>>>(a, ntask);
     LAUNCH_CHECK();
     return hipSuccess;
 }
