@@ -139,7 +139,8 @@ struct rome_ctx {
     int Kq = 16;
     // mixed-precision arm: scaled BF16/FP16 copies of the coarse table and of the image operand
     DevBuf<int> fct_absmax;
-    DevBuf<uint16_t> A16, W16;
+    DevBuf<uint16_t> A16, W16;  // A16: every rotation, or the half table of the paired cross term
+    bool pq = false;            // the rotation table pairs psi with psi + 180 deg (conjugate projections)
     DevBuf<float> inv_scale;
     int K16() const { return (2 * Pc_pad + 63) / 64 * 64; }
     // sums
@@ -295,6 +296,17 @@ int upload_poses(rome_ctx* ctx) {
     } else {
         b = a;  // single pass: the "fine" stage reuses the pass's own rotations
     }
+    // Paired cross term: rotation r + PSI/2 of a direction maps pixel (x, y) to minus the coordinates of
+    // rotation r (psi + 180 deg), so its projection is the conjugate of r's (Friedel symmetry of the
+    // real maps' transforms); used when every pair has that form and the half table tiles evenly.
+    ctx->pq = ctx->PSI % 8 == 0 && (ctx->Mtot / 2) % 128 == 0 && ctx->T >= 8;  // T >= 8: <= 18 images per tile
+    for (int d = 0; ctx->pq && d < ctx->R / ctx->PSI; ++d)
+        for (int p = 0; ctx->pq && p < ctx->PSI / 2; ++p) {
+            const double* u = &a[6 * ((size_t)d * ctx->PSI + p)];
+            const double* v = &a[6 * ((size_t)d * ctx->PSI + p + ctx->PSI / 2)];
+            for (int e = 0; e < 6; ++e)
+                if (!(std::fabs(u[e] + v[e]) <= 1e-12)) ctx->pq = false;
+        }
     HIPCHK(ctx->Ai_c.upload(a.data(), a.size(), ctx->stream));
     HIPCHK(ctx->Ai_f.upload(b.data(), b.size(), ctx->stream));
     return ROME_OK;
@@ -340,9 +352,10 @@ int ensure_ready(rome_ctx* ctx) {
             HIPCHK(ctx->fct_absmax.reserve(1));
             HIPCHK(hipMemsetAsync(ctx->fct_absmax.p, 0, sizeof(int), ctx->stream));
             HIPCHK(launch_absmax(ctx->stream, ctx->FcT.p, nf, ctx->fct_absmax.p));
-            HIPCHK(ctx->A16.reserve((size_t)ctx->Mpad * K16));
-            HIPCHK(launch_fct_to_lowp(ctx->stream, ctx->cfg.precision, ctx->FcT.p, 2 * ctx->Pc_pad, ctx->Mpad,
-                                      ctx->Mtot, ctx->fct_absmax.p, ctx->A16.p, K16));
+            const int rows = ctx->pq ? ctx->Mtot / 2 : ctx->Mpad;
+            HIPCHK(ctx->A16.reserve((size_t)rows * K16));
+            HIPCHK(launch_fct_to_lowp(ctx->stream, ctx->cfg.precision, ctx->FcT.p, ctx->Pc_pad, ctx->Mpad, ctx->Mtot,
+                                      rows, ctx->pq ? ctx->PSI / 2 : 0, ctx->fct_absmax.p, ctx->A16.p, K16));
         }
         ctx->fct_dirty = false;
     }
@@ -610,7 +623,13 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         } else {
             LowpGemmArgs la{ctx->A16.p, ctx->W16.p, ctx->At.p, ctx->inv_scale.p, Ep, ctx->valid.p,
                             ctx->emin_key.p, K16, ctx->Mpad, Npad, M, Ncols, T,
-                            gemm_sub_band(ctx->Mpad / 256, (size_t)256 * K16 * 2, (size_t)5 << 19), L};
+                            gemm_sub_band(ctx->Mpad / 256, (size_t)256 * K16 * 2, (size_t)5 << 19), L, 0, 0};
+            if (ctx->pq) {  // the paired cross term over the half table (list or E output)
+                la.Mh = M / 2;
+                la.psih = ctx->PSI / 2;
+                la.sband = gemm_sub_band(la.Mh / 128, (size_t)128 * K16 * 2, (size_t)5 << 19);
+                if (!list) la.list.lpd = ctx->lpd_row.p;
+            }
             HIPCHK(launch_gemm_lowp(s, prec, la));
         }
         stage_end(ctx, 2);

@@ -197,8 +197,17 @@ DEV void gemm_epilogue_staged(const f16v (&acc)[NI][NJ], int r0, int c0, int n0,
 //      range -> each lane stores its own entries.
 // B only falls, so every entry the weights need passes, whatever the order in which tiles run.  All
 // threads of the block call it (barriers).  slot_* arrays hold epi_slots<TN>() entries.
-template <bool SCALED, int NI, int NJ, int TN>
-DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lane, int tid,
+// Rotation rows of a wave's accumulator blocks: rows(i, q) = the rotation of the four rows
+// 8q + 4(lane/32) .. +3 of block i (consecutive rotations), or -1 for padding rows.
+struct ContigRows {  // block i = rotations r0 + 32 i ..
+    static constexpr bool kPadding = false;  // never -1
+    static constexpr bool kPaired = false;   // the accumulators hold the cross terms themselves
+    int r0, h;
+    DEV int operator()(int i, int q) const { return r0 + i * 32 + 8 * q + 4 * h; }
+};
+
+template <bool SCALED, int NI, int NJ, int TN, typename Rows>
+DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n0, int lane, int tid,
                             const float* __restrict__ At, const float* __restrict__ inv_scale,
                             int* __restrict__ emin_key, int Mtot, int Mpad, int Ncols, int T,
                             const ListArgs& L, int* slot_min, int* slot_vb, float* slot_lim, int* slot_cnt) {
@@ -226,26 +235,63 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lan
         vmin_j[j] = INFINITY;
     }
     // 1. E in place, minima
+    if constexpr (Rows::kPaired) {
+        // paired cross term: block 0 holds P, block 1 holds Q; rotation rows(0, q) has cross term P + Q,
+        // rows(1, q) has P - Q (both formed here, in place)
+        static_assert(NI == 2, "paired blocks");
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // the tile's rows of At and ln pdf_direction were staged in LDS (rows.at / rows.lp)
+            const int c0l = rows.local(0, q), c1l = rows.local(1, q);
+            const float4 l0 = *reinterpret_cast<const float4*>(rows.lp + c0l);
+            const float4 l1 = *reinterpret_cast<const float4*>(rows.lp + c1l);
+            const float lp0[4] = {l0.x, l0.y, l0.z, l0.w}, lp1[4] = {l1.x, l1.y, l1.z, l1.w};
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const float* Ar = rows.at + sl_j[j] * rows.kTileRows;
+                const float4 a0 = *reinterpret_cast<const float4*>(Ar + c0l);
+                const float4 a1 = *reinterpret_cast<const float4*>(Ar + c1l);
+                const float at0[4] = {a0.x, a0.y, a0.z, a0.w}, at1[4] = {a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float P = acc[0][j][4 * q + u], Q = acc[1][j][4 * q + u];
+                    const float e0 = e_value<SCALED>(at0[u], P + Q, s_j[j]);
+                    const float e1 = e_value<SCALED>(at1[u], P - Q, s_j[j]);
+                    acc[0][j][4 * q + u] = e0;
+                    acc[1][j][4 * q + u] = e1;
+                    emin_j[j] = fminf(emin_j[j], lp0[u] > -INFINITY ? e0 : INFINITY);
+                    emin_j[j] = fminf(emin_j[j], lp1[u] > -INFINITY ? e1 : INFINITY);
+                    vmin_j[j] = fminf(vmin_j[j], e0 - lp0[u]);
+                    vmin_j[j] = fminf(vmin_j[j], e1 - lp1[u]);
+                }
+            }
+        }
+    } else {
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int base = r0 + i * 32 + 8 * q + 4 * h;
-            const float4 lp = *reinterpret_cast<const float4*>(L.lpd + base);
+            const int base = rows(i, q);  // padding rows (-1): E = +inf, no prior, never kept
+            float4 lp = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+            if (!Rows::kPadding || base >= 0) lp = *reinterpret_cast<const float4*>(L.lpd + base);
             const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                const float4 at = *reinterpret_cast<const float4*>(At + (size_t)(img0 + sl_j[j]) * Mpad + base);
+                float4 at = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+                if (!Rows::kPadding || base >= 0)
+                    at = *reinterpret_cast<const float4*>(At + (size_t)(img0 + sl_j[j]) * Mpad + base);
                 const float atv[4] = {at.x, at.y, at.z, at.w};
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const float e = e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j]);
+                    const float e = (!Rows::kPadding || base >= 0) ? e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j])
+                                                                    : INFINITY;
                     acc[i][j][4 * q + u] = e;
                     emin_j[j] = fminf(emin_j[j], lpd[u] > -INFINITY ? e : INFINITY);  // rows with a prior
                     vmin_j[j] = fminf(vmin_j[j], e - lpd[u]);  // +inf without a prior
                 }
             }
         }
+    }
     // 2. bounds per image
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
@@ -276,7 +322,13 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lan
         float lpd[16];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float4 lp = *reinterpret_cast<const float4*>(L.lpd + r0 + i * 32 + 8 * q + 4 * h);
+            float4 lp = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+            if constexpr (Rows::kPaired) {
+                lp = *reinterpret_cast<const float4*>(rows.lp + rows.local(i, q));
+            } else {
+                const int base = rows(i, q);
+                if (!Rows::kPadding || base >= 0) lp = *reinterpret_cast<const float4*>(L.lpd + base);
+            }
             lpd[4 * q] = lp.x, lpd[4 * q + 1] = lp.y, lpd[4 * q + 2] = lp.z, lpd[4 * q + 3] = lp.w;
         }
 #pragma unroll
@@ -302,7 +354,13 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lan
         float lpd[16];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float4 lp = *reinterpret_cast<const float4*>(L.lpd + r0 + i * 32 + 8 * q + 4 * h);
+            float4 lp = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+            if constexpr (Rows::kPaired) {
+                lp = *reinterpret_cast<const float4*>(rows.lp + rows.local(i, q));
+            } else {
+                const int base = rows(i, q);
+                if (!Rows::kPadding || base >= 0) lp = *reinterpret_cast<const float4*>(L.lpd + base);
+            }
             lpd[4 * q] = lp.x, lpd[4 * q + 1] = lp.y, lpd[4 * q + 2] = lp.z, lpd[4 * q + 3] = lp.w;
         }
 #pragma unroll
@@ -316,7 +374,7 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], int r0, int c0, int n0, int lan
                 if (!(acc[i][j][r] - lpd[r] < lim_j[j])) continue;
                 if (pos < L.cap) {
                     const size_t o = (size_t)img * L.cap + pos;
-                    L.ent[o] = tm + (unsigned)(r0 + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3));
+                    L.ent[o] = tm + (unsigned)(rows(i, r >> 2) + (r & 3));
                     L.val[o] = acc[i][j][r];
                 }
                 ++pos;

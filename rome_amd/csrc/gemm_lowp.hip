@@ -221,8 +221,185 @@ __global__ __launch_bounds__(256, 2) void k_gemm_lowp_list(LowpGemmArgs a) {
     }
 #undef L2_LOAD
 #undef L2_STORE
-    gemm_epilogue_list<true, TI, TJ, BN>(acc, m0 + wm * 64, n0, n0, lane, tid, a.At, a.inv_scale, a.emin_key, a.Mtot,
+    gemm_epilogue_list<true, TI, TJ, BN>(acc, ContigRows{m0 + wm * 64, lane >> 5}, n0, n0, lane, tid, a.At, a.inv_scale, a.emin_key, a.Mtot,
                                          a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb, slot_lim, slot_cnt);
+}
+
+// Rows of the paired cross term: block 0 = rotations m of the half-table rows, block 1 = m + psih.
+// The tile's At rows and ln pdf_direction sit in LDS (at: [image slot][kTileRows], lp: [kTileRows],
+// tile-local row = 128 i + half-table row - m0), so the epilogue needs no per-row global addresses.
+struct PairRows {
+    static constexpr bool kPadding = false;  // Mh is a multiple of the tile
+    static constexpr bool kPaired = true;    // block 0 holds P, block 1 holds Q
+    static constexpr int kTileRows = 256;
+    int m[4], psih, l0;                       // rotation of the lane's quad q of block 0; its local row
+    const float *at, *lp;
+    DEV PairRows(int hbase, int lrow, int ph, const float* at_, const float* lp_) : psih(ph), l0(lrow), at(at_),
+                                                                                    lp(lp_) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int hh = hbase + lrow + 8 * q;
+            m[q] = (hh / ph) * 2 * ph + hh % ph;
+        }
+    }
+    DEV int operator()(int i, int q) const { return m[q] + i * psih; }
+    DEV int local(int i, int q) const { return 128 * i + l0 + 8 * q; }
+};
+
+// E output of the paired cross term (block 0 of acc holds P, block 1 Q): E[img][t][m] for both
+// rotations of each pair (16-byte stores of four consecutive rotations), and the per-image minimum of
+// E over rotations with a prior.
+template <int NI, int NJ, int TN, typename Rows>
+DEV void gemm_epilogue_e(const f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n0, int lane, int tid,
+                         const float* __restrict__ At, const float* __restrict__ inv_scale,
+                         const float* __restrict__ lpd_row, float* __restrict__ E, int* __restrict__ emin_key,
+                         int Mtot, int Mpad, int Ncols, int T, int* slot_min) {
+    const int img0 = n0 / T;
+    const int last = min(n0 + TN, Ncols) - 1;
+    const int nslot = last >= n0 ? last / T - img0 + 1 : 0;
+    for (int q = tid; q < nslot; q += blockDim.x) slot_min[q] = INT_MAX;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int col = c0 + j * 32 + (lane & 31);
+        if (col >= Ncols) continue;
+        const int img = col / T, t = col - img * T;
+        const float s = inv_scale[img];
+        float emin = INFINITY;
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int base = rows(i, q);
+                const float4 lp = *reinterpret_cast<const float4*>(lpd_row + base);
+                const float4 at = *reinterpret_cast<const float4*>(At + (size_t)img * Mpad + base);
+                const float lpv[4] = {lp.x, lp.y, lp.z, lp.w}, atv[4] = {at.x, at.y, at.z, at.w};
+                float ev[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float P = acc[0][j][4 * q + u], Q = acc[1][j][4 * q + u];
+                    ev[u] = e_value<true>(atv[u], i == 0 ? P + Q : P - Q, s);
+                    emin = fminf(emin, lpv[u] > -INFINITY ? ev[u] : INFINITY);
+                }
+                *reinterpret_cast<float4*>(E + ((size_t)img * T + t) * Mtot + base) =
+                    make_float4(ev[0], ev[1], ev[2], ev[3]);
+            }
+        if (emin < INFINITY) atomicMin(&slot_min[img - img0], float_order_key(emin));
+    }
+    __syncthreads();
+    for (int q = tid; q < nslot; q += blockDim.x)
+        if (slot_min[q] != INT_MAX) atomicMin(&emin_key[img0 + q], slot_min[q]);
+}
+
+// Paired cross term: 128 half-table rows x 128 columns per block of 4 waves (one per SIMD), each wave
+// 32 half-table rows x 128 columns with P and Q accumulators (2 x 4 blocks of
+// v_mfma_f32_32x32x16_{f16,bf16}); 32-k chunks of both halves (Re and Im) of both operands in one
+// 40 KB LDS stage, the next chunk waiting in registers; two blocks per CU.  The epilogue turns P, Q
+// into the two rotations' cross terms P + Q, P - Q and writes the list (LIST) or E.
+template <typename T, bool LIST>
+__global__ __launch_bounds__(256, 2) void k_gemm_lowp_pq(LowpGemmArgs a) {
+    constexpr int BM = 128, BN = 128, BK = 32, LDW = BK + 8;  // 80-byte LDS rows: conflict-free b128 reads
+    typedef typename Lowp<T>::v8 v8;
+    constexpr int TJ = 4;
+    __shared__ __attribute__((aligned(16))) T As[2][BM][LDW];  // [Re | Im][half-table row][k]
+    __shared__ __attribute__((aligned(16))) T Bs[2][BN][LDW];  // [Re | Im][column][k]
+    __shared__ int slot_min[epi_slots<BN>()];
+    __shared__ int slot_vb[LIST ? epi_slots<BN>() : 1];
+    __shared__ float slot_lim[LIST ? epi_slots<BN>() : 1];
+    __shared__ int slot_cnt[LIST ? epi_slots<BN>() : 1];
+    const int tid = threadIdx.x, lane = tid & 63, wm = tid >> 6;
+    int bm, bn;
+    gemm_tile(blockIdx.x, a.Mh / BM, a.Npad / BN, a.sband, bm, bn);
+    const int m0 = bm * BM, n0 = bn * BN;
+    const T* A = reinterpret_cast<const T*>(a.A);
+    const T* B = reinterpret_cast<const T*>(a.B);
+    f16v acc[2][TJ];  // [0]: P = sum Re Re, [1]: Q = sum Im Im
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[0][j][r] = acc[1][j][r] = 0.f;
+    const int KH = a.K16 / 2, nk = KH / BK;
+    // chunk: per operand 128 rows x (32 Re + 32 Im) k = 1024 pieces of 16 bytes, 4 per thread:
+    // piece tid + 256 v -> half (v >> 1), row tid / 4 + 64 (v & 1), k offset 8 (tid % 4)
+    const int prow = tid >> 2, pcol = (tid & 3) * 8;
+    const T* Ag = A + (size_t)(m0 + prow) * a.K16 + pcol;
+    const T* Bg = B + (size_t)(n0 + prow) * a.K16 + pcol;
+    const size_t r64 = (size_t)64 * a.K16;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define PQ_LOAD(kc)                                                          \
+    do {                                                                     \
+        const T* ag = Ag + (kc) * BK;                                        \
+        const T* bg = Bg + (kc) * BK;                                        \
+        ra0 = *reinterpret_cast<const uint4*>(ag);                           \
+        ra1 = *reinterpret_cast<const uint4*>(ag + r64);                     \
+        ra2 = *reinterpret_cast<const uint4*>(ag + KH);                      \
+        ra3 = *reinterpret_cast<const uint4*>(ag + r64 + KH);                \
+        rb0 = *reinterpret_cast<const uint4*>(bg);                           \
+        rb1 = *reinterpret_cast<const uint4*>(bg + r64);                     \
+        rb2 = *reinterpret_cast<const uint4*>(bg + KH);                      \
+        rb3 = *reinterpret_cast<const uint4*>(bg + r64 + KH);                \
+    } while (0)
+#define PQ_STORE()                                                           \
+    do {                                                                     \
+        *reinterpret_cast<uint4*>(&As[0][prow][pcol]) = ra0;                 \
+        *reinterpret_cast<uint4*>(&As[0][prow + 64][pcol]) = ra1;            \
+        *reinterpret_cast<uint4*>(&As[1][prow][pcol]) = ra2;                 \
+        *reinterpret_cast<uint4*>(&As[1][prow + 64][pcol]) = ra3;            \
+        *reinterpret_cast<uint4*>(&Bs[0][prow][pcol]) = rb0;                 \
+        *reinterpret_cast<uint4*>(&Bs[0][prow + 64][pcol]) = rb1;            \
+        *reinterpret_cast<uint4*>(&Bs[1][prow][pcol]) = rb2;                 \
+        *reinterpret_cast<uint4*>(&Bs[1][prow + 64][pcol]) = rb3;            \
+    } while (0)
+    PQ_LOAD(0);
+    PQ_STORE();
+    __syncthreads();
+    for (int kc = 0; kc < nk; ++kc) {
+        if (kc + 1 < nk) PQ_LOAD(kc + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const int ko = kk + 8 * (lane >> 5);
+            const v8 are = *reinterpret_cast<const v8*>(&As[0][wm * 32 + (lane & 31)][ko]);
+            const v8 aim = *reinterpret_cast<const v8*>(&As[1][wm * 32 + (lane & 31)][ko]);
+            v8 bre[TJ], bim[TJ];
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) {
+                bre[j] = *reinterpret_cast<const v8*>(&Bs[0][j * 32 + (lane & 31)][ko]);
+                bim[j] = *reinterpret_cast<const v8*>(&Bs[1][j * 32 + (lane & 31)][ko]);
+            }
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) {
+                acc[0][j] = Lowp<T>::mfma(are, bre[j], acc[0][j]);
+                acc[1][j] = Lowp<T>::mfma(aim, bim[j], acc[1][j]);
+            }
+        }
+        __syncthreads();
+        if (kc + 1 < nk) {
+            PQ_STORE();
+            __syncthreads();
+        }
+    }
+#undef PQ_LOAD
+#undef PQ_STORE
+    // the tile's 256 rotations (128 pairs): At of its images and ln pdf_direction into LDS (the operand
+    // stage is free: the loop ended with a barrier)
+    float* Ats = reinterpret_cast<float*>(&As[0][0][0]);
+    const int img0 = n0 / a.T, nsl = (min(n0 + BN, a.Ncols) - 1) / a.T - img0 + 1;
+    for (int v = tid; v < (nsl + 1) * 64; v += 256) {
+        const int sl = v >> 6, lr = (v & 63) * 4, hh = m0 + (lr & 127);
+        const int m = (hh / a.psih) * 2 * a.psih + hh % a.psih + (lr >> 7) * a.psih;
+        const float4 x = sl < nsl ? *reinterpret_cast<const float4*>(a.At + (size_t)(img0 + sl) * a.Mpad + m)
+                                  : *reinterpret_cast<const float4*>(a.list.lpd + m);
+        *reinterpret_cast<float4*>(Ats + sl * 256 + lr) = x;
+    }
+    __syncthreads();
+    // the cross terms of rotation m and of m + psih are P + Q and P - Q
+    const PairRows rows(m0, wm * 32 + 4 * (lane >> 5), a.psih, Ats, Ats + nsl * 256);
+    if constexpr (LIST)
+        gemm_epilogue_list<true, 2, TJ, BN>(acc, rows, n0, n0, lane, tid, a.At, a.inv_scale, a.emin_key, a.Mtot,
+                                            a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb, slot_lim, slot_cnt);
+    else
+        gemm_epilogue_e<2, TJ, BN>(acc, rows, n0, n0, lane, tid, a.At, a.inv_scale, a.list.lpd, a.E, a.emin_key,
+                                   a.Mtot, a.Mpad, a.Ncols, a.T, slot_min);
 }
 
 // sF: power of two that brings max |FcT| into [0.5, 1) (exact rescaling)
@@ -231,15 +408,19 @@ DEV float table_scale(const int* absmax_bits) {
     return mx > 0.f ? ldexpf(1.f, -ilogbf(mx) - 1) : 1.f;
 }
 
-// A16[m][k] = sF * FcT[k][m] for k < Kfp = 2*Pc_pad, zero beyond and for rows m >= Mtot.
+// A16[row][k] = sF * FcT[.][m] for rotation m = rowmap(row): Re of pixel p at k = p, Im at k = K16/2 + p
+// (p < Pc_pad), zero elsewhere and for rows past the table.  psih = 0: row = m (rows m >= Mtot zero);
+// psih > 0: the half table of the paired cross term, row h = rotation (h / psih) * 2 psih + h % psih.
 template <typename T>
-__global__ void k_fct_to_lowp(const float* __restrict__ FcT, int Kfp, int Mpad, int Mtot,
+__global__ void k_fct_to_lowp(const float* __restrict__ FcT, int Pc_pad, int Mpad, int Mtot, int rows, int psih,
                               const int* __restrict__ absmax_bits, T* __restrict__ A16, int K16) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)Mpad * K16) return;
+    if (idx >= (size_t)rows * K16) return;
     const float sF = table_scale(absmax_bits);
-    const int m = (int)(idx / K16), k = (int)(idx % K16);
-    const float v = (m < Mtot && k < Kfp) ? FcT[(size_t)k * Mpad + m] * sF : 0.f;
+    const int row = (int)(idx / K16), k = (int)(idx % K16), KH = K16 / 2;
+    const int m = psih > 0 ? (row / psih) * 2 * psih + row % psih : row;
+    const int kf = k < Pc_pad ? k : (k >= KH && k < KH + Pc_pad ? Pc_pad + (k - KH) : -1);
+    const float v = (m < Mtot && kf >= 0) ? FcT[(size_t)kf * Mpad + m] * sF : 0.f;
     A16[idx] = static_cast<T>(v);
 }
 
@@ -253,7 +434,7 @@ __global__ void k_absmax(const float* __restrict__ x, size_t n, int* __restrict_
     if (threadIdx.x == 0) atomicMax(out, __float_as_int(m));
 }
 
-// W16[img*T + t][k]: s_img * Minv*CTF*X*S (Re at k = p, Im at k = Pc_pad + p, zero in the padding);
+// W16[img*T + t][k]: s_img * Minv*CTF*X*S (Re at k = p, Im at k = K16/2 + p, zero in the padding);
 // one block per image.  s_img is the power of two that brings max |Minv*CTF*X| into [0.5, 1).
 template <typename T>
 __global__ void k_prep_lowp(PrepLowpArgs a) {
@@ -281,15 +462,15 @@ __global__ void k_prep_lowp(PrepLowpArgs a) {
         for (int t = 0; t < a.T; ++t) {
             const float2 XS = cmul(X, a.Sc[(size_t)t * a.Pc + p]);
             W[(size_t)t * a.K16 + p] = static_cast<T>(mc * XS.x);
-            W[(size_t)t * a.K16 + a.Pc_pad + p] = static_cast<T>(mc * XS.y);
+            W[(size_t)t * a.K16 + a.K16 / 2 + p] = static_cast<T>(mc * XS.y);
         }
     }
-    // padding columns: [Pc, Pc_pad), [Pc_pad + Pc, 2 Pc_pad), [2 Pc_pad, K16)
+    // padding columns: [Pc, K16/2), [K16/2 + Pc, K16)
     const int npad = a.K16 - 2 * a.Pc;
     for (int q = threadIdx.x; q < a.T * npad; q += blockDim.x) {
         const int t = q / npad, j = q - t * npad;
-        const int d = a.Pc_pad - a.Pc;
-        const int k = j < d ? a.Pc + j : (j < 2 * d ? a.Pc_pad + a.Pc + (j - d) : 2 * a.Pc_pad + (j - 2 * d));
+        const int d = a.K16 / 2 - a.Pc;
+        const int k = j < d ? a.Pc + j : a.K16 / 2 + a.Pc + (j - d);
         W[(size_t)t * a.K16 + k] = static_cast<T>(0.f);
     }
 }
@@ -298,6 +479,23 @@ __global__ void k_prep_lowp(PrepLowpArgs a) {
 
 hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a) {
     if (a.Mpad % 256 || a.Npad % 256 || a.K16 % 64 || a.K16 <= 0) return hipErrorInvalidValue;
+    if (a.psih > 0) {  // paired cross term
+        if (a.psih % 4 || a.Mh % 128 || 2 * a.Mh != a.Mtot || a.Mtot % 4 || a.T < 8 || !a.list.lpd)
+            return hipErrorInvalidValue;
+        const bool lst = a.list.count != nullptr;
+        if (!lst && (!a.E || !a.list.lpd)) return hipErrorInvalidValue;
+        const unsigned tiles = (unsigned)((a.Mh / 128) * (a.Npad / 128));
+        if (precision == 1) {
+            if (lst) k_gemm_lowp_pq<__bf16, true><<<tiles, 256, 0, s>>>(a);
+            else k_gemm_lowp_pq<__bf16, false><<<tiles, 256, 0, s>>>(a);
+        } else if (precision == 2) {
+            if (lst) k_gemm_lowp_pq<_Float16, true><<<tiles, 256, 0, s>>>(a);
+            else k_gemm_lowp_pq<_Float16, false><<<tiles, 256, 0, s>>>(a);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const int tiles = (a.Mpad / 256) * (a.Npad / 256);
     const bool list = a.list.count != nullptr;
     if (!list && !a.E) return hipErrorInvalidValue;
@@ -313,14 +511,16 @@ hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a)
     return hipGetLastError();
 }
 
-hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Kfp, int Mpad, int Mtot,
-                              const int* fct_absmax, void* A16, int K16) {
-    const size_t tot = (size_t)Mpad * K16;
+hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Pc_pad, int Mpad, int Mtot,
+                              int rows, int psih, const int* fct_absmax, void* A16, int K16) {
+    if (K16 % 2 || 2 * Pc_pad > K16) return hipErrorInvalidValue;
+    const size_t tot = (size_t)rows * K16;
     const unsigned blocks = (unsigned)((tot + 255) / 256);
     if (precision == 1)
-        k_fct_to_lowp<__bf16><<<blocks, 256, 0, s>>>(FcT, Kfp, Mpad, Mtot, fct_absmax, (__bf16*)A16, K16);
+        k_fct_to_lowp<__bf16><<<blocks, 256, 0, s>>>(FcT, Pc_pad, Mpad, Mtot, rows, psih, fct_absmax, (__bf16*)A16, K16);
     else if (precision == 2)
-        k_fct_to_lowp<_Float16><<<blocks, 256, 0, s>>>(FcT, Kfp, Mpad, Mtot, fct_absmax, (_Float16*)A16, K16);
+        k_fct_to_lowp<_Float16><<<blocks, 256, 0, s>>>(FcT, Pc_pad, Mpad, Mtot, rows, psih, fct_absmax, (_Float16*)A16,
+                                                        K16);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -101,6 +101,10 @@ struct LowpGemmArgs {
     int K16, Mpad, Npad, Mtot, Ncols, T;
     int sband;
     ListArgs list;            // list.count != nullptr: list output instead of E
+    // paired cross term (psih > 0): A holds only the rotations with psi in the first half turn, Mh rows;
+    // rotation m + psih (psi + 180 deg) has the conjugate projection, so with P = sum Re*Re and
+    // Q = sum Im*Im over the K16/2 halves, C(m) = P + Q and C(m + psih) = P - Q: half the MFMA work
+    int Mh, psih;
 };
 
 struct PrepLowpArgs {
@@ -420,8 +424,10 @@ hipError_t launch_fm_window(hipStream_t s, const double2* F, int padori, int dim
 
 // precision: 1 BF16, 2 FP16
 hipError_t launch_absmax(hipStream_t s, const float* x, size_t n, int* out);
-hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Kfp, int Mpad, int Mtot,
-                              const int* fct_absmax, void* A16, int K16);
+// A16 rows from FcT (Re at k = p, Im at k = K16/2 + p): rows = Mpad (row = rotation) with psih = 0, or
+// rows = Mtot/2 of the paired cross term (row h = rotation (h / psih) 2 psih + h % psih) with psih > 0
+hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Pc_pad, int Mpad, int Mtot,
+                              int rows, int psih, const int* fct_absmax, void* A16, int K16);
 hipError_t launch_prep_lowp(hipStream_t s, int precision, int n, const PrepLowpArgs& a);
 hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a);
 

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
         __syncthreads();
     }
     if constexpr (LIST)
-        gemm_epilogue_list<false, 2, 2, BN>(acc, m0 + wm * 64, n0 + wn * 64, n0, lane, tid, a.At, nullptr,
+        gemm_epilogue_list<false, 2, 2, BN>(acc, ContigRows{m0 + wm * 64, lane >> 5}, n0 + wn * 64, n0, lane, tid, a.At, nullptr,
                                             a.emin_key, a.Mtot, a.Mpad, a.Ncols, a.T, a.list, slot_min, slot_vb,
                                             slot_lim, slot_cnt);
     else
