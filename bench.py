@@ -244,10 +244,15 @@ def main():
     gemm_tflops = gemm_flops * gemm_calls / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     peak = MFMA_PEAK_TFLOPS[head]
     listed = stats.get("mode") == 0  # coarse list output (the product path) rather than the full pose term
-    kname = {"fp32": "k_coarse_gemm (coarse cross term, FP32 MFMA)",
-             "bf16": "k_gemm_lowp%s<bf16> (coarse cross term, BF16 MFMA)" % ("_list" if listed else ""),
-             "fp16": "k_gemm_lowp%s<fp16> (coarse cross term, FP16 MFMA)" % ("_list" if listed else "")}[head]
-    traffic = measured_traffic("k_gemm_lowp_list<_Float16>") if head == "fp16" and listed else None
+    # psi and psi + 180 deg share one half-table row in the mixed arm when the in-plane sampling pairs them
+    # (an even count of psi steps per half turn, the half table a multiple of 128 rows, >= 8 shifts)
+    paired = head != "fp32" and prob["nr_psi"] % 8 == 0 and (M // 2) % 128 == 0 and T >= 8
+    lowp = ("k_gemm_lowp_pq<%s, " + ("list>" if listed else "E>")) if paired else \
+        ("k_gemm_lowp_list<%s>" if listed else "k_gemm_lowp<%s>")
+    kname = "k_coarse_gemm (coarse cross term, FP32 MFMA)" if head == "fp32" else \
+        (lowp % head) + " (coarse cross term, %s MFMA%s)" % (head.upper(), ", psi/psi+180 paired" if paired else "")
+    row = ("k_gemm_lowp_pq<_Float16, true>" if paired else "k_gemm_lowp_list<_Float16>")
+    traffic = measured_traffic(row) if head == "fp16" and listed else None
     roofline = {"kernel": kname, "bound": "mfma",
                 "achieved": round(gemm_tflops, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(gemm_tflops / peak, 4), "traffic": traffic,

@@ -295,10 +295,12 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
     // 2. bounds per image
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        if (!ok_j[j]) continue;
-        if (emin_j[j] < INFINITY) atomicMin(&slot_min[sl_j[j]], float_order_key(emin_j[j]));
-        if (vmin_j[j] < INFINITY)
-            atomicMin(&slot_vb[sl_j[j]], float_order_key(vmin_j[j] + 1e-6f * fabsf(vmin_j[j]) + 1e-4f));
+        // branch-free: padding columns and infinite minima contribute INT_MAX (no change)
+        const int kmin = ok_j[j] && emin_j[j] < INFINITY ? float_order_key(emin_j[j]) : INT_MAX;
+        const int kvb = ok_j[j] && vmin_j[j] < INFINITY
+                            ? float_order_key(vmin_j[j] + 1e-6f * fabsf(vmin_j[j]) + 1e-4f) : INT_MAX;
+        atomicMin(&slot_min[sl_j[j]], kmin);
+        atomicMin(&slot_vb[sl_j[j]], kvb);
     }
     __syncthreads();
     for (int q = tid; q < nslot; q += blockDim.x) {
