@@ -31,6 +31,24 @@ DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
     }
 }
 
+// gemm_tile, except that without row bands (sband == 0) the tiles are grouped by XCD: blocks reach the
+// 8 XCDs round-robin (XCD = bid % 8) and each XCD takes whole column tiles, the nbm row tiles of one
+// column tile in a row, so that each column tile of the image operand is read into one L2 only.  The
+// grid holds gemm_grouped_blocks(nbm, nbn, sband) blocks; false: a padding block, which has no tile.
+DEV bool gemm_tile_grouped(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
+    if (sband > 0) {
+        gemm_tile(bid, nbm, nbn, sband, bm, bn);
+        return true;
+    }
+    const int x = bid & 7, i = bid >> 3;
+    bm = i % nbm;
+    bn = (i / nbm) * 8 + x;
+    return bn < nbn;
+}
+inline unsigned gemm_grouped_blocks(int nbm, int nbn, int sband) {
+    return (unsigned)nbm * (unsigned)(sband > 0 ? nbn : (nbn + 7) / 8 * 8);
+}
+
 // Images one tile of TN columns can touch (T >= 1): TN + 2.
 template <int TN>
 constexpr int epi_slots() { return TN + 2; }

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_lowp_list(LowpGemmArgs a) {
     __shared__ int slot_cnt[epi_slots<BN>()];
     const int tid = threadIdx.x, lane = tid & 63, wm = tid >> 6;
     int bm, bn;
-    gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
+    if (!gemm_tile_grouped(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn)) return;
     const int m0 = bm * BM, n0 = bn * BN;
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_lowp_pq(LowpGemmArgs a) {
     __shared__ int slot_cnt[LIST ? epi_slots<BN>() : 1];
     const int tid = threadIdx.x, lane = tid & 63, wm = tid >> 6;
     int bm, bn;
-    gemm_tile(blockIdx.x, a.Mh / BM, a.Npad / BN, a.sband, bm, bn);
+    if (!gemm_tile_grouped(blockIdx.x, a.Mh / BM, a.Npad / BN, a.sband, bm, bn)) return;
     const int m0 = bm * BM, n0 = bn * BN;
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
@@ -484,7 +484,7 @@ hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a)
             return hipErrorInvalidValue;
         const bool lst = a.list.count != nullptr;
         if (!lst && (!a.E || !a.list.lpd)) return hipErrorInvalidValue;
-        const unsigned tiles = (unsigned)((a.Mh / 128) * (a.Npad / 128));
+        const unsigned tiles = gemm_grouped_blocks(a.Mh / 128, a.Npad / 128, a.sband);
         if (precision == 1) {
             if (lst) k_gemm_lowp_pq<__bf16, true><<<tiles, 256, 0, s>>>(a);
             else k_gemm_lowp_pq<__bf16, false><<<tiles, 256, 0, s>>>(a);
@@ -497,13 +497,14 @@ hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a)
         return hipGetLastError();
     }
     const int tiles = (a.Mpad / 256) * (a.Npad / 256);
+    const unsigned list_blocks = gemm_grouped_blocks(a.Mpad / 256, a.Npad / 128, a.sband);
     const bool list = a.list.count != nullptr;
     if (!list && !a.E) return hipErrorInvalidValue;
     if (precision == 1) {
-        if (list) k_gemm_lowp_list<__bf16><<<2 * tiles, 256, 0, s>>>(a);
+        if (list) k_gemm_lowp_list<__bf16><<<list_blocks, 256, 0, s>>>(a);
         else k_gemm_lowp<__bf16><<<tiles, 512, 0, s>>>(a);
     } else if (precision == 2) {
-        if (list) k_gemm_lowp_list<_Float16><<<2 * tiles, 256, 0, s>>>(a);
+        if (list) k_gemm_lowp_list<_Float16><<<list_blocks, 256, 0, s>>>(a);
         else k_gemm_lowp<_Float16><<<tiles, 512, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
