@@ -296,11 +296,14 @@ DEV void gemm_epilogue_e(const f16v (&acc)[NI][NJ], const Rows& rows, int c0, in
 // v_mfma_f32_32x32x16_{f16,bf16}); 32-k chunks of both halves (Re and Im) of both operands in one
 // 40 KB LDS stage, the next chunk waiting in registers; two blocks per CU.  The epilogue turns P, Q
 // into the two rotations' cross terms P + Q, P - Q and writes the list (LIST) or E.
+constexpr int kPqBN = 128;         // columns per block: 64 or 128
+constexpr int kPqBlocksPerCU = 2;  // blocks per CU the register budget is sized for
 template <typename T, bool LIST>
-__global__ __launch_bounds__(256, 2) void k_gemm_lowp_pq(LowpGemmArgs a) {
-    constexpr int BM = 128, BN = 128, BK = 32, LDW = BK + 8;  // 80-byte LDS rows: conflict-free b128 reads
+__global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmArgs a) {
+    constexpr int BM = 128, BN = kPqBN, BK = 32, LDW = BK + 8;  // 80-byte LDS rows: conflict-free b128 reads
     typedef typename Lowp<T>::v8 v8;
-    constexpr int TJ = 4;
+    constexpr int TJ = BN / 32;
+    static_assert(BN == 64 || BN == 128, "column tile");
     __shared__ __attribute__((aligned(16))) T As[2][BM][LDW];  // [Re | Im][half-table row][k]
     __shared__ __attribute__((aligned(16))) T Bs[2][BN][LDW];  // [Re | Im][column][k]
     __shared__ int slot_min[epi_slots<BN>()];
@@ -335,9 +338,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_lowp_pq(LowpGemmArgs a) {
         ra2 = *reinterpret_cast<const uint4*>(ag + KH);                      \
         ra3 = *reinterpret_cast<const uint4*>(ag + r64 + KH);                \
         rb0 = *reinterpret_cast<const uint4*>(bg);                           \
-        rb1 = *reinterpret_cast<const uint4*>(bg + r64);                     \
+        if constexpr (BN == 128) rb1 = *reinterpret_cast<const uint4*>(bg + r64); \
         rb2 = *reinterpret_cast<const uint4*>(bg + KH);                      \
-        rb3 = *reinterpret_cast<const uint4*>(bg + r64 + KH);                \
+        if constexpr (BN == 128) rb3 = *reinterpret_cast<const uint4*>(bg + r64 + KH); \
     } while (0)
 #define PQ_STORE()                                                           \
     do {                                                                     \
@@ -346,9 +349,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_lowp_pq(LowpGemmArgs a) {
         *reinterpret_cast<uint4*>(&As[1][prow][pcol]) = ra2;                 \
         *reinterpret_cast<uint4*>(&As[1][prow + 64][pcol]) = ra3;            \
         *reinterpret_cast<uint4*>(&Bs[0][prow][pcol]) = rb0;                 \
-        *reinterpret_cast<uint4*>(&Bs[0][prow + 64][pcol]) = rb1;            \
+        if constexpr (BN == 128) *reinterpret_cast<uint4*>(&Bs[0][prow + 64][pcol]) = rb1; \
         *reinterpret_cast<uint4*>(&Bs[1][prow][pcol]) = rb2;                 \
-        *reinterpret_cast<uint4*>(&Bs[1][prow + 64][pcol]) = rb3;            \
+        if constexpr (BN == 128) *reinterpret_cast<uint4*>(&Bs[1][prow + 64][pcol]) = rb3; \
     } while (0)
     PQ_LOAD(0);
     PQ_STORE();
@@ -484,7 +487,7 @@ hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a)
             return hipErrorInvalidValue;
         const bool lst = a.list.count != nullptr;
         if (!lst && (!a.E || !a.list.lpd)) return hipErrorInvalidValue;
-        const unsigned tiles = gemm_grouped_blocks(a.Mh / 128, a.Npad / 128, a.sband);
+        const unsigned tiles = gemm_grouped_blocks(a.Mh / 128, a.Npad / kPqBN, a.sband);
         if (precision == 1) {
             if (lst) k_gemm_lowp_pq<__bf16, true><<<tiles, 256, 0, s>>>(a);
             else k_gemm_lowp_pq<__bf16, false><<<tiles, 256, 0, s>>>(a);
