@@ -264,6 +264,13 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
             const float4 l0 = *reinterpret_cast<const float4*>(rows.lp + c0l);
             const float4 l1 = *reinterpret_cast<const float4*>(rows.lp + c1l);
             const float lp0[4] = {l0.x, l0.y, l0.z, l0.w}, lp1[4] = {l1.x, l1.y, l1.z, l1.w};
+            // +inf for rows without a prior (they take no part in the minimum of E), else 0
+            float pen0[4], pen1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                pen0[u] = lp0[u] > -INFINITY ? 0.f : INFINITY;
+                pen1[u] = lp1[u] > -INFINITY ? 0.f : INFINITY;
+            }
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const float* Ar = rows.at + sl_j[j] * rows.kTileRows;
@@ -277,10 +284,8 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
                     const float e1 = e_value<SCALED>(at1[u], P - Q, s_j[j]);
                     acc[0][j][4 * q + u] = e0;
                     acc[1][j][4 * q + u] = e1;
-                    emin_j[j] = fminf(emin_j[j], lp0[u] > -INFINITY ? e0 : INFINITY);
-                    emin_j[j] = fminf(emin_j[j], lp1[u] > -INFINITY ? e1 : INFINITY);
-                    vmin_j[j] = fminf(vmin_j[j], e0 - lp0[u]);
-                    vmin_j[j] = fminf(vmin_j[j], e1 - lp1[u]);
+                    emin_j[j] = fminf(fminf(emin_j[j], e0 + pen0[u]), e1 + pen1[u]);  // v_min3
+                    vmin_j[j] = fminf(fminf(vmin_j[j], e0 - lp0[u]), e1 - lp1[u]);
                 }
             }
         }
@@ -293,6 +298,9 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
             float4 lp = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
             if (!Rows::kPadding || base >= 0) lp = *reinterpret_cast<const float4*>(L.lpd + base);
             const float lpd[4] = {lp.x, lp.y, lp.z, lp.w};
+            float pen[4];  // +inf for rows without a prior (no part in the minimum of E), else 0
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pen[u] = lpd[u] > -INFINITY ? 0.f : INFINITY;
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 float4 at = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
@@ -304,7 +312,7 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
                     const float e = (!Rows::kPadding || base >= 0) ? e_value<SCALED>(atv[u], acc[i][j][4 * q + u], s_j[j])
                                                                     : INFINITY;
                     acc[i][j][4 * q + u] = e;
-                    emin_j[j] = fminf(emin_j[j], lpd[u] > -INFINITY ? e : INFINITY);  // rows with a prior
+                    emin_j[j] = fminf(emin_j[j], e + pen[u]);  // rows with a prior
                     vmin_j[j] = fminf(vmin_j[j], e - lpd[u]);  // +inf without a prior
                 }
             }
@@ -354,8 +362,11 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
             int cnt = 0;
+            // vmin_j is the minimum of the same differences over the lane's rows: none passes unless it does
+            if (vmin_j[j] < lim_j[j]) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) cnt += acc[i][j][r] - lpd[r] < lim_j[j];
+                for (int r = 0; r < 16; ++r) cnt += acc[i][j][r] - lpd[r] < lim_j[j];
+            }
             off[i][j] = cnt ? atomicAdd(&slot_cnt[sl_j[j]], cnt) : -1;
         }
     }
