@@ -25,13 +25,14 @@ from test_gpu_parity import _case, _run_both
 pytestmark = pytest.mark.gpu
 
 U = {"bf16": 2.0 ** -8, "fp16": 2.0 ** -11}
-CASES = ["3d", "3d_window_groups_offsets", "2d_two_pass", "3d_full_map3d"]
+CASES = ["3d", "3d_window_groups_offsets", "2d_two_pass", "3d_full_map3d", "3d_paired_k1"]
+FULL_SIZE = ("3d_full_map3d", "3d_paired_k1")  # no dense fine dumps: per-image results, sums and grids
 
 
 @pytest.fixture(scope="module", params=CASES)
 def case(request):
     prob, batch, truth = _case(request.param)
-    dense = request.param != "3d_full_map3d"
+    dense = request.param not in FULL_SIZE
     ref = O.estep(prob, batch, dump_coarse=True, dump_fine=dense, dump_fine_weight=dense)
     return request.param, prob, batch, ref
 

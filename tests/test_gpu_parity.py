@@ -49,6 +49,10 @@ def _case(name):
     if name == "3d_full_map3d":
         return synth.make_problem(is3d=True, ori=128, current=128, coarse=24, K=4, order=2, offset_range=5.0,
                                   oversampling=1, n=3, snr=0.05, seed=14, nblobs=200)
+    if name == "3d_paired_k1":  # 4608 rotations, one class: the paired cross term over 18 row tiles (not a
+        # multiple of the 8 XCDs: the XCD-grouped tile order)
+        return synth.make_problem(is3d=True, ori=64, current=64, coarse=24, K=1, order=2, offset_range=3.0,
+                                  oversampling=1, n=4, snr=1.0, seed=21, nblobs=120)
     raise KeyError(name)
 
 
