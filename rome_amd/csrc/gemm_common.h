@@ -264,13 +264,6 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
             const float4 l0 = *reinterpret_cast<const float4*>(rows.lp + c0l);
             const float4 l1 = *reinterpret_cast<const float4*>(rows.lp + c1l);
             const float lp0[4] = {l0.x, l0.y, l0.z, l0.w}, lp1[4] = {l1.x, l1.y, l1.z, l1.w};
-            // +inf for rows without a prior (they take no part in the minimum of E), else 0
-            float pen0[4], pen1[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                pen0[u] = lp0[u] > -INFINITY ? 0.f : INFINITY;
-                pen1[u] = lp1[u] > -INFINITY ? 0.f : INFINITY;
-            }
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const float* Ar = rows.at + sl_j[j] * rows.kTileRows;
@@ -284,7 +277,8 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
                     const float e1 = e_value<SCALED>(at1[u], P - Q, s_j[j]);
                     acc[0][j][4 * q + u] = e0;
                     acc[1][j][4 * q + u] = e1;
-                    emin_j[j] = fminf(fminf(emin_j[j], e0 + pen0[u]), e1 + pen1[u]);  // v_min3
+                    // the staged At is +inf on rows without a prior, so is their E: out of the minimum
+                    emin_j[j] = fminf(fminf(emin_j[j], e0), e1);  // v_min3
                     vmin_j[j] = fminf(fminf(vmin_j[j], e0 - lp0[u]), e1 - lp1[u]);
                 }
             }

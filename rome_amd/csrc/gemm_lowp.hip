@@ -390,8 +390,15 @@ __global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmAr
     for (int v = tid; v < (nsl + 1) * 64; v += 256) {
         const int sl = v >> 6, lr = (v & 63) * 4, hh = m0 + (lr & 127);
         const int m = (hh / a.psih) * 2 * a.psih + hh % a.psih + (lr >> 7) * a.psih;
-        const float4 x = sl < nsl ? *reinterpret_cast<const float4*>(a.At + (size_t)(img0 + sl) * a.Mpad + m)
-                                  : *reinterpret_cast<const float4*>(a.list.lpd + m);
+        const float4 l = *reinterpret_cast<const float4*>(a.list.lpd + m);
+        float4 x = l;
+        if (sl < nsl) {  // At, +inf on rows without a prior: their E is +inf, outside every minimum and list
+            x = *reinterpret_cast<const float4*>(a.At + (size_t)(img0 + sl) * a.Mpad + m);
+            x.x += l.x > -INFINITY ? 0.f : INFINITY;
+            x.y += l.y > -INFINITY ? 0.f : INFINITY;
+            x.z += l.z > -INFINITY ? 0.f : INFINITY;
+            x.w += l.w > -INFINITY ? 0.f : INFINITY;
+        }
         *reinterpret_cast<float4*>(Ats + sl * 256 + lr) = x;
     }
     __syncthreads();
