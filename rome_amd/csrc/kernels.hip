@@ -704,6 +704,7 @@ __global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
         for (int o = 0; o < NOR; ++o)
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
+                if (j / NOT >= ng) continue;  // translation slot unused in this group (uniform)
                 double v = wave_sum((double)B[o][j]);
                 if (lane == 0) red[wid][o * NJ + j] = v;
             }
@@ -956,6 +957,7 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
     const int u = a.m2u[m];
     const double ox = a.old_offx[img], oy = a.old_offy[img];
     for (int s = tid; s < a.oriF; s += NT) shell[s] = 0.;
+    int sig = 0;
     for (int e = tid; e < nt * NOT; e += NT) {
         const int tb = e / NOT, ot = e - tb * NOT;
         const int t = a.blk_t[b0 + tb];
@@ -965,8 +967,13 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
                     pdf_offset(a.trans_x[t], a.trans_y[t], ox, oy, a.prior_x[k], a.prior_y[k], a.sigma2_offset);
         wv = dd > 700. ? 0. : wv * exp(-dd);
         wts[e] = wv;
+        sig |= wv >= thr;
     }
-    __syncthreads();
+    // a slice without a significant sample adds nothing: leave before reading its pixels
+    if (!__syncthreads_or(sig)) {
+        if (tid == 0) a.slice_w[(size_t)q * NOR + o] = 0.;
+        return;
+    }
     const float2* X = a.Xf + (size_t)img * a.Pf;
     const float* Ct = a.Ctff + (size_t)img * a.Pf;
     const float2* NM = a.NMf + (size_t)img * a.Pf;
