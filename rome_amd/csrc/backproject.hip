@@ -79,6 +79,7 @@ DEV int brick_of(const BpArgs& a, const Sample& sm) {
 
 __global__ __launch_bounds__(256) void k_bp_count(BpArgs a) {
     __shared__ unsigned keys[kHash], cnt[kHash];
+    if (!(a.slice_w[blockIdx.x] > 0.)) return;  // a slice without weight has no record (whole block)
     for (int h = threadIdx.x; h < kHash; h += blockDim.x) {
         keys[h] = 0u;
         cnt[h] = 0u;
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(1024) void k_bp_scan(const unsigned* __restrict__ c
 
 __global__ __launch_bounds__(256) void k_bp_fill(BpArgs a) {
     __shared__ unsigned keys[kHash], cnt[kHash], base[kHash];
+    if (!(a.slice_w[blockIdx.x] > 0.)) return;  // a slice without weight has no record (whole block)
     for (int h = threadIdx.x; h < kHash; h += blockDim.x) {
         keys[h] = 0u;
         cnt[h] = 0u;
