@@ -453,7 +453,9 @@ __global__ __launch_bounds__(1024) void k_weights_coarse(WeightParams wp, const 
 
 // Coarse weights from the list the cross-term GEMM kept (every entry within kWin of the minimum
 // u, and more): entries i = 0 .. count-1 of the image's list, argmax keys in the reference's order.
-__global__ __launch_bounds__(1024) void k_weights_list(WeightParams wp, ListArgs L, const double* __restrict__ C2c,
+// 8 waves per SIMD: two 1024-thread blocks per CU (64 VGPRs; the few spills cost less than the
+// second block gains on this latency-bound pass)
+__global__ __launch_bounds__(1024, 8) void k_weights_list(WeightParams wp, ListArgs L, const double* __restrict__ C2c,
                                                        const int* __restrict__ emin_key, WeightOut wo,
                                                        SigArgs sig, Cand cand) {
     __shared__ WShared sh;
@@ -501,7 +503,8 @@ __global__ void k_list_reach(WeightParams wp, float* __restrict__ reach) {
     if (threadIdx.x == 0) reach[img] = lo > -INFINITY ? (float)(kWin + (hi - lo) + 1.0) : INFINITY;
 }
 
-__global__ __launch_bounds__(1024) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
+// two blocks per CU, as k_weights_list
+__global__ __launch_bounds__(1024, 8) void k_weights_fine(WeightParams wp, const double* __restrict__ D2,
                                                        const int* __restrict__ blk_task, const int* __restrict__ blk_t,
                                                        const int* __restrict__ task_m,
                                                        const long long* __restrict__ img_blk, int log2NE, WeightOut wo,
