@@ -28,7 +28,7 @@ namespace rome {
 namespace {
 
 constexpr int kBrick = 8, kBox = kBrick + 1, kBox3 = kBox * kBox * kBox;
-constexpr unsigned kChunk = 2048;  // records per brick block
+constexpr unsigned kChunk = 1024;  // records per brick block (41 KB of LDS: three blocks per CU)
 constexpr int kHash = 512;         // LDS hash slots for the <= 256 bricks of one count / fill block
 
 // Slot of brick b in the block's hash table (keys b + 1, 0 = empty; linear probing).  At most 256
