@@ -162,7 +162,7 @@ struct rome_ctx {
     DevBuf<float> WcT, MC2c, At, E, Minvf, Ctff, MC2bp;
     DevBuf<double> C2c, C2f;
     DevBuf<float2> Xf, NMf;
-    DevBuf<double> c_dmin, c_sumw, c_maxw, c_thr, f_dmin, f_sumw, f_maxw, f_thr, wnorm, meta;
+    DevBuf<double> c_dmin, c_sumw, c_maxw, c_thr, c_tail, f_dmin, f_sumw, f_maxw, f_thr, wnorm, meta;
     DevBuf<long long> c_amax, f_amax;
     DevBuf<int> c_nsig, c_status, f_status;
     DevBuf<uint32_t> sigmask;
@@ -203,9 +203,10 @@ struct rome_ctx {
     std::vector<double> rc_tab_h;
     // stage timing with HIP events on the engine stream (rome_set_profiling)
     static constexpr int kStages = 11;
+    static constexpr int kStageRecords = 4;  // a stage can run more than once per batch (a coarse rerun)
     bool profiling = false;
-    hipEvent_t ev[kStages][2] = {};
-    bool ev_used[kStages] = {};
+    hipEvent_t ev[kStages][kStageRecords][2] = {};
+    int ev_used[kStages] = {};
     double stage_ms[kStages] = {};
     long long stage_calls[kStages] = {};
     long long last_U = 0;
@@ -237,24 +238,29 @@ int fail(rome_ctx* c, int code, const char* fmt, ...) {
                         __FILE__, __LINE__);                                                           \
     } while (0)
 
+// Every run of a stage within a batch gets its own event pair (up to kStageRecords; a batch whose
+// coarse list overflowed runs the coarse stages twice), and the batch's time of the stage is their sum.
 void stage_begin(rome_ctx* ctx, int i) {
-    if (!ctx->profiling) return;
-    (void)hipEventRecord(ctx->ev[i][0], ctx->stream);
-    ctx->ev_used[i] = true;
+    if (!ctx->profiling || ctx->ev_used[i] >= rome_ctx::kStageRecords) return;
+    (void)hipEventRecord(ctx->ev[i][ctx->ev_used[i]][0], ctx->stream);
 }
 void stage_end(rome_ctx* ctx, int i) {
-    if (ctx->profiling) (void)hipEventRecord(ctx->ev[i][1], ctx->stream);
+    if (!ctx->profiling || ctx->ev_used[i] >= rome_ctx::kStageRecords) return;
+    (void)hipEventRecord(ctx->ev[i][ctx->ev_used[i]][1], ctx->stream);
+    ++ctx->ev_used[i];
 }
 void stage_collect(rome_ctx* ctx) {
     if (!ctx->profiling) return;
     for (int i = 0; i < rome_ctx::kStages; ++i) {
         if (!ctx->ev_used[i]) continue;
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->ev[i][0], ctx->ev[i][1]) == hipSuccess) {
-            ctx->stage_ms[i] += ms;
-            ctx->stage_calls[i] += 1;
+        double sum = 0.;
+        for (int q = 0; q < ctx->ev_used[i]; ++q) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ctx->ev[i][q][0], ctx->ev[i][q][1]) == hipSuccess) sum += ms;
         }
-        ctx->ev_used[i] = false;
+        ctx->stage_ms[i] += sum;
+        ctx->stage_calls[i] += 1;
+        ctx->ev_used[i] = 0;
     }
 }
 
@@ -421,6 +427,8 @@ WeightParams weight_params(rome_ctx* ctx) {
     wp.prior_y = ctx->prior_y.p;
     wp.sigma2_offset = ctx->cfg.sigma2_offset;
     wp.adaptive_fraction = ctx->cfg.adaptive_fraction;
+    // a single pass marks its significant samples with a margin: the superset gets exact diff2
+    wp.mark_margin = ctx->two_pass ? 0.0 : kRecomputeMargin;
     return wp;
 }
 
@@ -526,8 +534,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(launch_list_reach(s, n, wp, ctx->reach.p));
     }
     stage_end(ctx, 1);
-    for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_maxw, &ctx->c_thr, &ctx->f_dmin, &ctx->f_sumw, &ctx->f_maxw,
-                    &ctx->f_thr, &ctx->wnorm})
+    for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_maxw, &ctx->c_thr, &ctx->c_tail, &ctx->f_dmin, &ctx->f_sumw,
+                    &ctx->f_maxw, &ctx->f_thr, &ctx->wnorm})
         HIPCHK(b->reserve(n));
     HIPCHK(ctx->meta.reserve((size_t)9 * n));
     HIPCHK(ctx->c_amax.reserve(n));
@@ -547,8 +555,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->nblk.reserve(n));
     HIPCHK(ctx->task_off.reserve(n + 1));
     HIPCHK(ctx->blk_off.reserve(n + 1));
-    const WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p, ctx->c_nsig.p,
-                           ctx->c_status.p};
+    const WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p,  ctx->c_maxw.p,
+                           ctx->c_amax.p, ctx->c_thr.p,   ctx->c_nsig.p,
+                           ctx->c_status.p, ctx->two_pass ? nullptr : ctx->c_tail.p};
     const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
     long long tot[2];
     int U = 0;
@@ -700,30 +709,25 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     fa.C2f = ctx->C2f.p;
     fa.D2 = ctx->D2.p;
     fa.Pf = ctx->Pf;
-    const double* thr_p;
-    const double* sum_p;
-    const double* dmin_p;
-    WeightOutArgs wf{ctx->f_dmin.p, ctx->f_sumw.p, ctx->f_maxw.p, ctx->f_amax.p, ctx->f_thr.p, nullptr,
-                     ctx->f_status.p};
-    if (ctx->two_pass) {
-        stage_begin(ctx, 6);
-        HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa));
-        stage_end(ctx, 6);
-        stage_begin(ctx, 7);
-        HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE,
-                                   wf, cand));
-        stage_end(ctx, 7);
-        thr_p = ctx->f_thr.p;
-        sum_p = ctx->f_sumw.p;
-        dmin_p = ctx->f_dmin.p;
-    } else {
-        // single pass: the pass's own significant entries feed the sums (2D, oversampling 0)
-        HIPCHK(launch_fill_d2(s, ctx->blk_task.p, ctx->blk_t.p, ctx->task_img.p, ctx->task_m.p, nblocks, ctx->E.p,
-                              ctx->C2c.p, M, T, ctx->D2.p));
-        thr_p = ctx->c_thr.p;
-        sum_p = ctx->c_sumw.p;
-        dmin_p = ctx->c_dmin.p;
-    }
+    // Two passes: the fine pass over the coarse-significant samples (src/map3d_optimizer.cpp:1261-1372).
+    // Single pass (oversampling 0): the same kernels recompute, with NOR = NOT = 1, the diff2 of the
+    // samples the FP32 cross term marked significant (with kRecomputeMargin nats of slack); their weights,
+    // threshold and NR_SIGN are then formed from those values plus the mass of the unmarked rest, so the
+    // sums see the diff2 of per-pixel products with an FP64 reduction instead of the cross term's long
+    // FP32 accumulation (src/map2d_optimizer.cpp:1006-1047 computes them once, directly, in FP64).
+    WeightOutArgs wf{ctx->f_dmin.p, ctx->f_sumw.p, ctx->f_maxw.p, ctx->f_amax.p, ctx->f_thr.p,
+                     ctx->two_pass ? nullptr : ctx->c_nsig.p, ctx->f_status.p, nullptr};
+    const TailArgs tail = ctx->two_pass ? TailArgs{nullptr, nullptr} : TailArgs{ctx->c_tail.p, ctx->c_dmin.p};
+    stage_begin(ctx, 6);
+    HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa));
+    stage_end(ctx, 6);
+    stage_begin(ctx, 7);
+    HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE, wf,
+                               cand, tail));
+    stage_end(ctx, 7);
+    const double* thr_p = ctx->f_thr.p;
+    const double* sum_p = ctx->f_sumw.p;
+    const double* dmin_p = ctx->f_dmin.p;
     // ---- updateOtherParams + backProjection
     if (ctx->post_enabled) {
         PostArgs pp;
@@ -846,7 +850,6 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     // ---- metadata of the last pass (on the device; read back with the results)
     MetaArgs ma;
     ma.n = n;
-    ma.two_pass = ctx->two_pass;
     ma.is3d = ctx->is3d;
     ma.R = ctx->R;
     ma.PSI = ctx->PSI;
@@ -854,12 +857,12 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     ma.NOT = NOT;
     ma.T = T;
     ma.Mtot = M;
-    ma.amax = ctx->two_pass ? ctx->f_amax.p : ctx->c_amax.p;
+    ma.amax = ctx->f_amax.p;
     ma.img_blk = ctx->blk_off.p;
     ma.blk_task = ctx->blk_task.p;
     ma.blk_t = ctx->blk_t.p;
     ma.task_m = ctx->task_m.p;
-    ma.maxw = ctx->two_pass ? ctx->f_maxw.p : ctx->c_maxw.p;
+    ma.maxw = ctx->f_maxw.p;
     ma.sumw = sum_p;
     ma.offx = ctx->in.offx;
     ma.offy = ctx->in.offy;
@@ -900,9 +903,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
                             {dmin_p, res ? res->min_diff2 : nullptr, nd},
                             {sum_p, res ? res->sum_weight : nullptr, nd},
                             {thr_p, res ? res->sig_weight : nullptr, nd},
-                            {ctx->c_dmin.p, res ? res->min_diff2_coarse : nullptr, nd},
-                            {ctx->c_sumw.p, res ? res->sum_weight_coarse : nullptr, nd},
-                            {ctx->c_thr.p, res ? res->sig_weight_coarse : nullptr, nd},
+                            // a single pass is its own coarse pass: its exact values
+                            {ctx->two_pass ? ctx->c_dmin.p : dmin_p, res ? res->min_diff2_coarse : nullptr, nd},
+                            {ctx->two_pass ? ctx->c_sumw.p : sum_p, res ? res->sum_weight_coarse : nullptr, nd},
+                            {ctx->two_pass ? ctx->c_thr.p : thr_p, res ? res->sig_weight_coarse : nullptr, nd},
                             {ctx->wnorm.p, res ? res->wsum_norm_correction : nullptr, nd}});
         if (rc) return rc;
     }
@@ -951,9 +955,10 @@ void rome_destroy(rome_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < rome_ctx::kStages; ++i)
-        for (int j = 0; j < 2; ++j)
-            if (ctx->ev[i][j]) (void)hipEventDestroy(ctx->ev[i][j]);
+    for (auto& st : ctx->ev)
+        for (auto& pr : st)
+            for (hipEvent_t e : pr)
+                if (e) (void)hipEventDestroy(e);
     for (auto& p : ctx->fft_plans) (void)hipfftDestroy(p.h);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1389,14 +1394,15 @@ int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9
 int rome_set_profiling(rome_ctx* ctx, int on) {
     if (!ctx) return ROME_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    if (on && !ctx->ev[0][0])
-        for (int i = 0; i < rome_ctx::kStages; ++i)
-            for (int j = 0; j < 2; ++j) HIPCHK(hipEventCreate(&ctx->ev[i][j]));
+    if (on && !ctx->ev[0][0][0])
+        for (auto& st : ctx->ev)
+            for (auto& pr : st)
+                for (hipEvent_t& e : pr) HIPCHK(hipEventCreate(&e));
     ctx->profiling = on != 0;
     for (int i = 0; i < rome_ctx::kStages; ++i) {
         ctx->stage_ms[i] = 0.;
         ctx->stage_calls[i] = 0;
-        ctx->ev_used[i] = false;
+        ctx->ev_used[i] = 0;
     }
     return ROME_OK;
 }

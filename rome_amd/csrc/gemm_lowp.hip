@@ -57,6 +57,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_lowp(LowpGemmArgs a) {
     int bm, bn;
     gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
     const int m0 = bm * BM, n0 = bn * BN;
+    if (n0 >= a.Ncols) return;  // a tile of padding columns only (block-uniform, before any barrier)
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
     f16v acc[TI][TJ];
@@ -146,6 +147,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_lowp_list(LowpGemmArgs a) {
     int bm, bn;
     if (!gemm_tile_grouped(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn)) return;
     const int m0 = bm * BM, n0 = bn * BN;
+    if (n0 >= a.Ncols) return;  // a tile of padding columns only (block-uniform, before any barrier)
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
     f16v acc[TI][TJ];
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmAr
     int bm, bn;
     if (!gemm_tile_grouped(blockIdx.x, a.Mh / BM, a.Npad / BN, a.sband, bm, bn)) return;
     const int m0 = bm * BM, n0 = bn * BN;
+    if (n0 >= a.Ncols) return;  // a tile of padding columns only (block-uniform, before any barrier)
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
     f16v acc[2][TJ];  // [0]: P = sum Re Re, [1]: Q = sum Im Im

@@ -50,6 +50,12 @@ struct PrepArgs {
 // below exp(-48) of the largest).
 constexpr double kWeightWindow = 64.0;
 
+// Single pass: the FP32 cross term marks every sample whose weight is within this many nats of the
+// significance threshold; those get their diff2 recomputed exactly before weights and sums are formed.
+// The FP32 cross term's error (about 1e-4 nat, below 1e-2 nat at the largest sizes) is far inside it,
+// so the exact significant set lies inside the marked one.
+constexpr double kRecomputeMargin = 2.0;
+
 // Coarse list output of the cross-term GEMMs (instead of the full pose term E): per image the
 // entries e = t*Mtot + m that can lie within kWeightWindow of the image's minimum u, with their E
 // value.  With v = E - ln pdf_direction(m) and D = max - min of ln pdf_offset over the image's
@@ -139,6 +145,9 @@ struct WeightParams {
     const double* prior_x;   // [K]
     const double* prior_y;
     double sigma2_offset, adaptive_fraction;
+    // Significance marking: 0 marks the samples with w >= threshold; m > 0 marks every sample with
+    // w >= threshold * exp(-m) (the superset a single pass recomputes exactly).
+    double mark_margin;
 };
 
 struct WeightOutArgs {
@@ -149,6 +158,14 @@ struct WeightOutArgs {
     double* thr;
     int* nsig;
     int* status;
+    double* tail;  // optional: weight mass of the entries left unmarked (sum - marked), anchored at dmin
+};
+
+// Weight mass that a recomputed pass does not hold (the unmarked entries of the pass before it) and
+// the minimum diff2 it is anchored at; added to the pass's own sum after re-anchoring.
+struct TailArgs {
+    const double* mass;
+    const double* dmin;
 };
 
 struct FineArgs {
@@ -340,7 +357,7 @@ hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, con
 hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float* reach);
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
-                               const WeightOutArgs& wo, const CandArgs& cand);
+                               const WeightOutArgs& wo, const CandArgs& cand, TailArgs tail = TailArgs{nullptr, nullptr});
 // Task lists of the fine pass from the significant samples: per image the number of tasks
 // (rotations with any significant translation) and blocks (significant samples), their exclusive
 // scans (task_off, blk_off [n + 1]), used[m] = 1 for every rotation some image needs (zero before)
@@ -359,8 +376,6 @@ hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a
 // The binned 3D form with one block per slice (hipErrorNotSupported outside NOR = 8, NOT = 4,
 // Pf <= 8192): wnorm must be zero before the launch.
 hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a);
-hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
-                          long long nblocks, const float* E, const double* C2c, int M, int T, double* D2);
 // Gathers up to kPackSegs device arrays into one contiguous staging buffer (one launch), so that a
 // group of small read-backs is one copy to pinned host memory.  Sizes and offsets are multiples of 4.
 constexpr int kPackSegs = 16;
@@ -376,16 +391,18 @@ hipError_t launch_pack(hipStream_t s, const PackArgs& a);
 // Per-image metadata of the last pass (the max-posterior sample): meta columns 0-6 (PMAX, CLASS,
 // ROT, TILT, PSI, XOFF, YOFF; 7 and 8 zero, filled by the sums kernel) and argmax = (class, dir, psi,
 // over_rot, trans, over_trans).  src/map3d_optimizer.cpp:1786-1805, src/map2d_optimizer.cpp:1200-1218.
+// The last pass is always a block list (the fine pass, or the single pass's exact recompute of its
+// significant samples, NOR = NOT = 1).
 struct MetaArgs {
-    int n, two_pass, is3d, R, PSI, NOR, NOT, T, Mtot;
-    const long long* amax;     // [n] argmax key: fine entry (b - img_blk[i]) * NE + o * NOT + ot, or coarse m * T + t
-    const long long* img_blk;  // [n + 1] fine blocks per image (two pass)
+    int n, is3d, R, PSI, NOR, NOT, T, Mtot;
+    const long long* amax;     // [n] argmax key: entry (b - img_blk[i]) * NE + o * NOT + ot of the last pass
+    const long long* img_blk;  // [n + 1] blocks per image
     const int* blk_task;
     const int* blk_t;
     const int* task_m;
     const double *maxw, *sumw, *offx, *offy;
-    const double* pose;        // fine [R][NOR][3 or 1] (two pass) or coarse [R][3 or 1]
-    const double *sx, *sy;     // shifts of the pass: fine [T][NOT] or coarse [T]
+    const double* pose;        // poses of the last pass [R][NOR][3 or 1]
+    const double *sx, *sy;     // shifts of the last pass [T][NOT]
     double* meta;              // [n][9]
     int* argmax;               // [n][6]
 };

@@ -318,6 +318,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemm(GemmArgs a) {
     int bm, bn;
     gemm_tile(blockIdx.x, a.Mpad / BM, a.Npad / BN, a.sband, bm, bn);
     const int m0 = bm * BM, n0 = bn * BN;
+    if (n0 >= a.Ncols) return;  // a tile of padding columns only (block-uniform, before any barrier)
     // global load mapping: 16 rows x 128 cols = 2048 floats = 512 float4 -> 2 per thread per operand
     const int lr = tid >> 5, lc = (tid & 31) * 4;  // rows lr and lr+8
     f16v acc[2][2];
@@ -1075,18 +1076,6 @@ __global__ void k_store_sums(StoreArgs a) {
 }
 
 // ------------------------------------------------------------------ small helpers
-// Single-pass mode: the significant entries take their diff2 from the pass's own
-// coarse values, so weights, threshold and sums all come from one set of numbers.
-__global__ void k_fill_d2(const int* __restrict__ blk_task, const int* __restrict__ blk_t, const int* __restrict__ task_img,
-                          const int* __restrict__ task_m, long long nblocks, const float* __restrict__ E,
-                          const double* __restrict__ C2c, int M, int T, double* __restrict__ D2) {
-    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    const int q = blk_task[b];
-    const int img = task_img[q];
-    D2[b] = (double)E[((size_t)img * T + blk_t[b]) * M + task_m[q]] + C2c[img];
-}
-
 // Max-posterior sample of each image -> metadata row and argmax tuple (one thread per image).  An image
 // without one (no weight; the caller reports it) gets zeros.
 __global__ void k_meta(MetaArgs a) {
@@ -1098,20 +1087,13 @@ __global__ void k_meta(MetaArgs a) {
     for (int c = 0; c < 6; ++c) am[c] = 0;
     const long long key = a.amax[i];
     if (!(a.sumw[i] > 0.) || key < 0) return;
-    int m, t, o = 0, ot = 0;
-    if (a.two_pass) {
-        const int NE = a.NOR * a.NOT;
-        const long long b = a.img_blk[i] + key / NE;
-        if (b >= a.img_blk[i + 1]) return;
-        m = a.task_m[a.blk_task[b]];
-        t = a.blk_t[b];
-        o = (int)((key % NE) / a.NOT);
-        ot = (int)(key % a.NOT);
-    } else {
-        if (key >= (long long)a.Mtot * a.T) return;
-        m = (int)(key / a.T);
-        t = (int)(key % a.T);
-    }
+    const int NE = a.NOR * a.NOT;
+    const long long b = a.img_blk[i] + key / NE;
+    if (b >= a.img_blk[i + 1]) return;
+    const int m = a.task_m[a.blk_task[b]];
+    const int t = a.blk_t[b];
+    const int o = (int)((key % NE) / a.NOT);
+    const int ot = (int)(key % a.NOT);
     const int k = m / a.R, r = m % a.R;
     am[0] = k;
     am[1] = r / a.PSI;
@@ -1121,7 +1103,7 @@ __global__ void k_meta(MetaArgs a) {
     am[5] = ot;
     md[0] = a.maxw[i] / a.sumw[i];
     md[1] = k + 1;
-    const size_t pr = a.two_pass ? (size_t)r * a.NOR + o : (size_t)r;
+    const size_t pr = (size_t)r * a.NOR + o;
     if (a.is3d) {
         md[2] = a.pose[3 * pr];
         md[3] = a.pose[3 * pr + 1];
@@ -1132,9 +1114,8 @@ __global__ void k_meta(MetaArgs a) {
         if (ps < -180.) ps += 360.;
         md[4] = ps;
     }
-    const int nots = a.two_pass ? a.NOT : 1;
-    md[5] = a.offx[i] + a.sx[(size_t)t * nots + ot];
-    md[6] = a.offy[i] + a.sy[(size_t)t * nots + ot];
+    md[5] = a.offx[i] + a.sx[(size_t)t * a.NOT + ot];
+    md[6] = a.offy[i] + a.sy[(size_t)t * a.NOT + ot];
 }
 
 // ------------------------------------------------------------------ launch wrappers
@@ -1327,14 +1308,6 @@ hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, 
     const long long blocks = 8 * ((ntask * NOR + 7) / 8);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     k_post_slice<8, 4, 8, 1024><<<(unsigned)blocks, 1024, 0, s>>>(a, ntask);
-    LAUNCH_CHECK();
-    return hipSuccess;
-}
-
-hipError_t launch_fill_d2(hipStream_t s, const int* blk_task, const int* blk_t, const int* task_img, const int* task_m,
-                          long long nblocks, const float* E, const double* C2c, int M, int T, double* D2) {
-    if (nblocks == 0) return hipSuccess;
-    k_fill_d2<<<(unsigned)((nblocks + 255) / 256), 256, 0, s>>>(blk_task, blk_t, task_img, task_m, nblocks, E, C2c, M, T, D2);
     LAUNCH_CHECK();
     return hipSuccess;
 }

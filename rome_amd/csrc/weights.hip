@@ -39,6 +39,7 @@ struct WeightOut {
     double* thr;
     int* nsig;
     int* status;
+    double* tail;
 };
 
 struct Cand {
@@ -75,7 +76,8 @@ struct WShared {
 // first maximum in that order (ties between equal weights).
 template <typename Get, typename Pre, typename Key>
 __device__ void weight_pass(int img, size_t count, double cimg, const WeightParams& wp, Get get, Pre pre, Key key,
-                            WeightOut wo, WShared& sh, SigArgs sig, Cand cand, const int* emin_key) {
+                            WeightOut wo, WShared& sh, SigArgs sig, Cand cand, const int* emin_key,
+                            TailArgs tail = TailArgs{nullptr, nullptr}) {
     constexpr int NT = 1024;
     const int tid = threadIdx.x;
     const int T = wp.T;
@@ -193,6 +195,9 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         bound = pre_bound(u0);
     }
     sum = block_reduce(sum, [](double a, double b) { return a + b; }, sh.red_d);
+    // mass of the previous pass's unmarked entries, re-anchored from its minimum to this one:
+    // pdf exp(-(d - dmin)) = pdf exp(-(d - dmin_prev)) exp(dmin - dmin_prev)
+    if (tail.mass) sum += tail.mass[img] * exp(dmin - tail.dmin[img]);
     {
         const double m2 = block_reduce(mx, [](double a, double b) { return fmax(a, b); }, sh.red_d);
         const long long c = (mx == m2 && am >= 0) ? am : (long long)0x7fffffffffffffffLL;
@@ -346,36 +351,44 @@ __device__ void weight_pass(int img, size_t count, double cimg, const WeightPara
         if (wo.nsig) wo.nsig[img] = status == 1 ? 0 : (int)sh.nsig;
         wo.status[img] = status | (listed ? 0 : 4);
     }
-    // ---- mark: w >= threshold, with the weight the reference compares
+    // ---- mark: w >= threshold, with the weight the reference compares (or the superset within
+    // mark_margin nats of it); the mass left unmarked goes to wo.tail
+    const double thr_mark = wp.mark_margin > 0. ? thr * exp(-wp.mark_margin) : thr;
+    double marked = 0.;
     if (sig.keys) {
         if (tid == 0) sh.nmark = 0;
         __syncthreads();
         if (status != 1)
             visit(0.0, kWin, [&](double w, double, long long e) {
-                if (!(w >= thr)) return;
+                if (!(w >= thr_mark)) return;
                 double d;
                 int m, t;
                 get((size_t)e, d, m, t);
+                marked += w;
                 const int slot = atomicAdd(&sh.nmark, 1);
                 if (slot < sig.cap) sig.keys[(size_t)img * sig.cap + slot] = m * T + t;
             });
         __syncthreads();
         if (tid == 0) sig.cnt[img] = sh.nmark;
-        return;
+    } else if (sig.mask) {
+        const int TW = sig.TW;
+        uint32_t* msk = sig.mask + (size_t)img * wp.Mtot * TW;
+        for (size_t q = tid; q < (size_t)wp.Mtot * TW; q += NT) msk[q] = 0u;
+        __syncthreads();
+        if (status != 1)
+            visit(0.0, kWin, [&](double w, double, long long e) {
+                if (!(w >= thr_mark)) return;
+                double d;
+                int m, t;
+                get((size_t)e, d, m, t);
+                marked += w;
+                atomicOr(&msk[(size_t)m * TW + (t >> 5)], 1u << (t & 31));
+            });
     }
-    if (!sig.mask) return;
-    const int TW = sig.TW;
-    uint32_t* msk = sig.mask + (size_t)img * wp.Mtot * TW;
-    for (size_t q = tid; q < (size_t)wp.Mtot * TW; q += NT) msk[q] = 0u;
-    __syncthreads();
-    if (status == 1) return;
-    visit(0.0, kWin, [&](double w, double, long long e) {
-        if (!(w >= thr)) return;
-        double d;
-        int m, t;
-        get((size_t)e, d, m, t);
-        atomicOr(&msk[(size_t)m * TW + (t >> 5)], 1u << (t & 31));
-    });
+    if (wo.tail) {
+        marked = block_reduce(marked, [](double a, double b) { return a + b; }, sh.red_d);
+        if (tid == 0) wo.tail[img] = fmax(sum - marked, 0.);
+    }
 }
 
 // Coarse entries e = t * Mtot + m (E rows are rotation-contiguous).
@@ -508,7 +521,7 @@ __global__ __launch_bounds__(1024, 8) void k_weights_fine(WeightParams wp, const
                                                        const int* __restrict__ blk_task, const int* __restrict__ blk_t,
                                                        const int* __restrict__ task_m,
                                                        const long long* __restrict__ img_blk, int log2NE, WeightOut wo,
-                                                       Cand cand) {
+                                                       Cand cand, TailArgs tail) {
     __shared__ WShared sh;
     const int img = blockIdx.x;
     const long long b0 = img_blk[img];
@@ -523,7 +536,8 @@ __global__ __launch_bounds__(1024, 8) void k_weights_fine(WeightParams wp, const
     };
     const AllPre pre{count};
     auto key = [](size_t e, int, int) { return (long long)e; };
-    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, SigArgs{nullptr, 1, nullptr, nullptr, 0}, cand, nullptr);
+    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, SigArgs{nullptr, 1, nullptr, nullptr, 0}, cand, nullptr,
+                tail);
 }
 
 }  // namespace
@@ -532,7 +546,7 @@ hipError_t launch_weights_coarse(hipStream_t s, int n, const WeightParams& wp, c
                                  const int* emin_key, const WeightOutArgs& wo, const SigArgs& sig,
                                  const CandArgs& c) {
     if (sig.keys && ((long long)wp.Mtot * wp.T > 0x7fffffffLL || sig.cap > kSigCap)) return hipErrorInvalidValue;
-    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status, wo.tail};
     k_weights_coarse<<<n, 1024, 0, s>>>(wp, E, C2c, emin_key, o, sig, Cand{c.w, c.du, c.e, c.cap});
     return hipGetLastError();
 }
@@ -542,7 +556,7 @@ hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, con
                                const CandArgs& c) {
     if (wp.K * wp.T > kMaxKT) return hipErrorInvalidValue;
     if (sig.keys && ((long long)wp.Mtot * wp.T > 0x7fffffffLL || sig.cap > kSigCap)) return hipErrorInvalidValue;
-    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status, wo.tail};
     k_weights_list<<<n, 1024, 0, s>>>(wp, list, C2c, emin_key, o, sig, Cand{c.w, c.du, c.e, c.cap});
     return hipGetLastError();
 }
@@ -555,12 +569,13 @@ hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float
 
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
-                               const WeightOutArgs& wo, const CandArgs& c) {
+                               const WeightOutArgs& wo, const CandArgs& c, TailArgs tail) {
     int log2NE = 0;
     while ((1 << log2NE) < NE) ++log2NE;
     if ((1 << log2NE) != NE) return hipErrorInvalidValue;
-    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status};
-    k_weights_fine<<<n, 1024, 0, s>>>(wp, D2, blk_task, blk_t, task_m, img_blk, log2NE, o, Cand{c.w, c.du, c.e, c.cap});
+    WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status, wo.tail};
+    k_weights_fine<<<n, 1024, 0, s>>>(wp, D2, blk_task, blk_t, task_m, img_blk, log2NE, o, Cand{c.w, c.du, c.e, c.cap},
+                                      tail);
     return hipGetLastError();
 }
 

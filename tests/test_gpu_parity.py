@@ -3,7 +3,9 @@
 Tolerances of the plain FP32 arm: diff2 within 1e-5 relative (coarse and fine); identical max-posterior
 class and pose except near-ties (the two candidates' weights within 1e-3 relative); PMAX and offsets
 within 1e-4; weighted sums within 1e-4 relative; backprojector data and weight grids within 1e-5
-relative L2.  Cases cover 3D two-pass, 2D single-pass (MAP2D default), 2D two-pass, current_size <
+relative L2; NR_SIGN identical for the single pass (exact recompute) and, for the FP32 coarse pass of two
+passes, significant sets that differ only inside the band its measured diff2 error allows.  Zero priors
+(directions, a class) get no weight.  Cases cover 3D two-pass, 2D single-pass (MAP2D default), 2D two-pass, current_size <
 ori_size (highres_Xi2 and the sigma2 tail), several groups, old offsets and offset priors, CTF off, and
 the full MAP3D 128^2 / K=4 / 4608 x 81 search on a few particles.
 """
@@ -48,7 +50,17 @@ def _case(name):
                                   offset_range=3.0, n=12, snr=0.05, seed=41, angpix=2.0)
     if name == "3d_full_map3d":
         return synth.make_problem(is3d=True, ori=128, current=128, coarse=24, K=4, order=2, offset_range=5.0,
-                                  oversampling=1, n=3, snr=0.05, seed=14, nblobs=200)
+                                  oversampling=1, n=32, snr=0.05, seed=14, nblobs=200)
+    if name == "3d_zero_prior":  # directions and a class without prior (iteration >= 2 after an M-step)
+        prob, batch, truth = synth.make_problem(is3d=True, ori=32, current=32, coarse=16, K=3, order=1, n=8,
+                                                snr=0.5, seed=31)
+        zero_priors(prob)
+        return prob, batch, truth
+    if name == "2d_zero_prior":
+        prob, batch, truth = synth.make_problem(is3d=False, ori=64, current=64, coarse=64, K=4, psi_step=10.0,
+                                                oversampling=0, offset_range=3.0, n=10, snr=0.2, seed=32)
+        zero_priors(prob)
+        return prob, batch, truth
     if name == "3d_paired_k1":  # 4608 rotations, one class: the paired cross term over 18 row tiles (not a
         # multiple of the 8 XCDs: the XCD-grouped tile order)
         return synth.make_problem(is3d=True, ori=64, current=64, coarse=24, K=1, order=2, offset_range=3.0,
@@ -56,11 +68,25 @@ def _case(name):
     raise KeyError(name)
 
 
-CASES = ["3d", "3d_window_groups_offsets", "3d_noctf", "2d_single_pass", "2d_two_pass", "2d_config0", "3d_full_map3d"]
-# Backprojector grids, relative L2.  The MAP2D workload at SNR 0.05 with 10 classes has broad posteriors:
-# thousands of poses per particle carry weight, so the FP32 rounding of diff2 (about 1e-5 nat) reaches the
-# grids at up to 5e-5; the other cases are dominated by a few poses and agree to 1e-5.
-GRID_TOL = {"2d_config0": 1e-4}
+def zero_priors(prob):
+    """The prior state of iteration >= 2: M-step pdf_direction = wsum / sum leaves directions without weight
+    at 0 and a class can empty (src/map3d_optimizer.cpp:2696-2705); the E-step then skips them
+    (src/map3d_optimizer.cpp:1133, 1205).  Zeroes a block of directions of class 0, every other direction of
+    class 1 and the whole last class (pdf_class = 0)."""
+    K, D = prob["nr_classes"], prob["nr_dir"]
+    pd = np.array(prob["pdf_direction"], dtype=float).reshape(K, D)
+    pd[0, : max(1, D // 3)] = 0.0
+    if K > 2:
+        pd[1, ::2] = 0.0
+    pc = np.array(prob["pdf_class"], dtype=float)
+    if K > 1:
+        pc[K - 1] = 0.0
+    prob["pdf_direction"] = pd
+    prob["pdf_class"] = pc / pc.sum()
+
+
+CASES = ["3d", "3d_window_groups_offsets", "3d_noctf", "2d_single_pass", "2d_two_pass", "2d_config0", "3d_full_map3d",
+         "3d_zero_prior", "2d_zero_prior"]
 
 
 @pytest.fixture(scope="module", params=CASES)
@@ -136,14 +162,111 @@ def test_estep_parity(engine, case):
     assert _rel(d[m], o[m]) < 1e-5
     assert _rel(r["min_diff2_coarse"], ref["min_diff2_coarse"]) < 1e-5
     if prob["adaptive_oversampling"]:
-        assert np.all(np.abs(r["nr_sign"] - ref["nr_sign"]) <= 2 + 0.02 * ref["nr_sign"])
+        _check_significant_set(engine, prob, batch, ref, d, r)
+    else:  # the single pass recomputes its marked samples exactly: the reference's own count
+        assert np.array_equal(r["nr_sign"], ref["nr_sign"]), (r["nr_sign"], ref["nr_sign"])
+    _check_zero_prior(engine, prob, r)
     if "fine_diff2" not in ref:  # full-size case: per-image results, sums and grids only
-        assert np.array_equal(r["argmax"], ref["argmax"])
+        check_argmax_near_ties(engine, prob, batch, r, ref, name)
         np.testing.assert_allclose(r["min_diff2"], ref["min_diff2"], rtol=1e-5)
         ref["fine_weight"] = None
     else:
         _check_lists(engine, r, ref, batch["n"], name)
     _check_sums(r, s, ref, prob, engine, name)
+
+
+def _ln_prior(prob, batch):
+    """ln(pdf_direction * pdf_offset) per (image, class, dir, psi, t) as the reference forms it
+    (src/map3d_optimizer.cpp:1704-1726, pdf_offset of src/ml_model.h:231-242 on the coarse translations)."""
+    K, D, PSI = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"]
+    pd = np.asarray(prob["pdf_direction"], float).reshape(K, D)
+    pc = np.asarray(prob["pdf_class"], float)
+    tx, ty = np.asarray(prob["trans_x"]), np.asarray(prob["trans_y"])
+    ox = tx[None, None, :] + np.asarray(batch["old_offx"])[:, None, None] - np.asarray(prob["prior_offx"])[None, :, None]
+    oy = ty[None, None, :] + np.asarray(batch["old_offy"])[:, None, None] - np.asarray(prob["prior_offy"])[None, :, None]
+    s2 = prob["sigma2_offset"]
+    with np.errstate(divide="ignore"):
+        lpo = -(ox * ox + oy * oy) / (2 * s2) - np.log(2 * np.pi * s2)  # [n][K][T]
+        lpd = np.where((pd > 0) & (pc[:, None] > 0), np.log(np.where(pd > 0, pd, 1.0)), -np.inf)  # [K][D]
+    return lpd[None, :, :, None, None] + lpo[:, :, None, None, :] + np.zeros((1, 1, 1, PSI, 1))
+
+
+def _check_significant_set(engine, prob, batch, ref, d, r):
+    """Coarse significant samples of the FP32 pass against the oracle's.  With eps the largest coarse diff2
+    error of the image (measured), each GPU weight is the oracle's times exp(delta), |delta| <= 2 eps, once
+    both are anchored at their minima, and the sums agree to the same factor.  Both significant sets are
+    top-mass prefixes (cumulative mass from the largest weight first exceeding 0.999 of the sum), so they
+    can differ only in samples (a) whose oracle ln-weight is within 4 eps of the oracle's threshold, or
+    (b) whose oracle cumulative mass lies within 4 eps of the sum (plus the sample's own weight) of the
+    crossing 0.999 * sum.  The test holds the symmetric difference of the two sets to that band, which also
+    bounds the NR_SIGN difference by the band's size."""
+    n = batch["n"]
+    K, D, PSI, T = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"], prob["nr_trans"]
+    keys, _ = engine.get_fine_list()
+    ne = (8 if prob["is_3d"] else 2) * 4
+    gsig = np.zeros(n * K * D * PSI * T, bool)
+    gsig[np.unique(keys // ne)] = True
+    gsig = gsig.reshape(n, -1)
+    osig = ref["coarse_sig"].astype(bool).reshape(n, -1)
+    o = ref["coarse_diff2"].reshape(n, -1)
+    lnw = (_ln_prior(prob, batch) - (ref["coarse_diff2"] - ref["min_diff2_coarse"][:, None, None, None, None]))
+    lnw = lnw.reshape(n, -1)
+    f = prob["adaptive_fraction"]
+    for i in range(n):
+        m = ~np.isnan(o[i]) & np.isfinite(lnw[i])
+        eps = float(np.max(np.abs(d.reshape(n, -1)[i][m] - o[i][m])))
+        w = np.where(m, np.exp(np.where(m, lnw[i], -np.inf)), 0.0)
+        order = np.argsort(-w, kind="stable")
+        cum = np.empty_like(w)
+        cum[order] = np.cumsum(w[order])
+        tot = cum[order[-1]]
+        lthr = np.log(ref["sig_weight_coarse"][i])
+        band = m & ((np.abs(lnw[i] - lthr) <= 4 * eps + 1e-9) | (np.abs(cum - f * tot) <= 4 * eps * tot + w))
+        diff = gsig[i] ^ osig[i]
+        assert not np.any(diff & ~band), (i, int(diff.sum()), int((diff & ~band).sum()), eps)
+        assert abs(int(r["nr_sign"][i]) - int(ref["nr_sign"][i])) <= int(band.sum()), i
+
+
+def check_argmax_near_ties(engine, prob, batch, r, ref, name, tol=1e-3):
+    """Max-posterior poses without the oracle's dense dumps: where the GPU's pose differs from the oracle's,
+    both must be in the GPU's last-pass list with weights (diff2 minus ln prior) within tol relative of each
+    other: a near-tie."""
+    n = batch["n"]
+    K, D, PSI, T = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"], prob["nr_trans"]
+    nor, nt = ((8 if prob["is_3d"] else 2), 4) if prob["adaptive_oversampling"] else (1, 1)
+    bad = [i for i in range(n) if not np.array_equal(r["argmax"][i], ref["argmax"][i])]
+    if not bad:
+        return
+    keys, d2 = engine.get_fine_list()
+    pos = {int(k): j for j, k in enumerate(keys)}
+    lp = _ln_prior(prob, batch)
+
+    def u_of(i, a):
+        k, dd, ps, o_, t, ot = (int(v) for v in a)
+        key = ((((i * K + k) * D + dd) * PSI + ps) * T + t) * nor * nt + o_ * nt + ot
+        assert key in pos, f"{name} image {i}: pose {a} not in the GPU list"
+        return d2[pos[key]] - lp[i, k, dd, ps, t]
+
+    for i in bad:
+        du = abs(u_of(i, r["argmax"][i]) - u_of(i, ref["argmax"][i]))
+        assert du <= -np.log1p(-tol), f"{name} image {i}: argmax {r['argmax'][i]} vs {ref['argmax'][i]} ({du:.2e} nat)"
+
+
+def _check_zero_prior(engine, prob, r):
+    """Rotations without prior get no weight: the max-posterior class and direction always have one, and
+    the fine list holds no sample of them."""
+    K, D, PSI, T = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"], prob["nr_trans"]
+    pd = np.asarray(prob["pdf_direction"], float).reshape(K, D)
+    pc = np.asarray(prob["pdf_class"], float)
+    ok = (pd > 0) & (pc[:, None] > 0)
+    if ok.all():
+        return
+    assert np.all(ok[r["argmax"][:, 0], r["argmax"][:, 1]])
+    keys, _ = engine.get_fine_list()
+    ne = ((8 if prob["is_3d"] else 2) * 4) if prob["adaptive_oversampling"] else 1
+    c = keys // ne // T // PSI
+    kd = c % (K * D)
+    assert np.all(ok.reshape(-1)[kd])
 
 
 def _check_lists(engine, r, ref, n, name):
@@ -190,9 +313,8 @@ def _check_sums(r, s, ref, prob, engine, name):
         if np.linalg.norm(ow) == 0:
             assert np.linalg.norm(wg) == 0
             continue
-        tol = GRID_TOL.get(name, 1e-5)
-        assert np.linalg.norm(dg - od) <= tol * np.linalg.norm(od), name
-        assert np.linalg.norm(wg - ow) <= tol * np.linalg.norm(ow), name
+        assert np.linalg.norm(dg - od) <= 1e-5 * np.linalg.norm(od), name
+        assert np.linalg.norm(wg - ow) <= 1e-5 * np.linalg.norm(ow), name
 
 
 def test_single_pass_in_a_fresh_context():
@@ -286,13 +408,13 @@ def test_update_model_matches_oracle(engine, case):
 def test_config0_map2d_full_iteration(engine):
     """One E+M iteration of the MAP2D workload (64^2, K=10, 36 psi x 29 translations): the engine's E-step
     then its reconstruction of every class from its own backprojector, against the oracle's E-step and the
-    oracle's reconstruction from the oracle's grids (maps within 1e-4 relative L2: the grids agree to
-    1e-5), and the model update from the engine's sums against the oracle's."""
+    oracle's reconstruction from the oracle's grids (maps within 1e-5 relative L2), identical max-posterior
+    class and pose except near-ties, and the model update from the engine's sums against the oracle's."""
     import oracle.mstep as OM
     prob, batch, _ = _case("2d_config0")
-    ref = O.estep(prob, batch)
+    ref = O.estep(prob, batch, dump_fine=True, dump_fine_weight=True)
     r, s = _run(engine, prob, batch)
-    assert np.mean(np.all(r["argmax"] == ref["argmax"], axis=1)) >= 0.9
+    _check_lists(engine, r, ref, batch["n"], "config0 iteration")
     ori, cur = prob["ori_size"], prob["current_size"]
     tau2 = np.full(ori // 2 + 1, 0.5)
     shape = (prob["pad"], prob["pad"] // 2 + 1)
@@ -304,7 +426,7 @@ def test_config0_map2d_full_iteration(engine):
         od = (od[0::2] + 1j * od[1::2]).reshape(shape)
         ow = np.asarray(ref["bp_weight"][k]).reshape(shape)
         want = OM.reconstruct(od, ow, ori, cur, tau2=tau2)[0]
-        assert np.linalg.norm(vol - want) <= 1e-4 * np.linalg.norm(want), k
+        assert np.linalg.norm(vol - want) <= 1e-5 * np.linalg.norm(want), k
     got = engine.update_model(install=False)
     ref_sums = {k: ref[k] for k in ("wsum_pdf_class", "wsum_pdf_direction", "wsum_sigma2_noise", "wsum_sumw_group",
                                     "wsum_prior_offx", "wsum_prior_offy", "wsum_scalars")}
