@@ -89,9 +89,72 @@ def cpu_baseline(prob, F, ctf, idx, threads):
                  fnomask_im=Fh.imag.copy(), fctf=ch, highres_Xi2=np.zeros(n), power_img=np.zeros((n, oriF)),
                  old_offx=np.zeros(n), old_offy=np.zeros(n), group=np.zeros(n, np.int32), norm=np.ones(n))
     t0 = time.perf_counter()
-    O.estep(prob, batch, nr_threads=threads)
+    ref = O.estep(prob, batch, nr_threads=threads)
     dt = time.perf_counter() - t0
-    return n / dt, dt
+    return n / dt, dt, ref
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def fsc_min(a, b):
+    """Lowest Fourier shell correlation between two real maps over the shells 1 .. ori/2, and the relative L2."""
+    Fa, Fb = np.fft.rfftn(a), np.fft.rfftn(b)
+    ori = a.shape[0]
+    k = [np.fft.fftfreq(ori) * ori] * (a.ndim - 1) + [np.arange(Fa.shape[-1])]
+    r = np.rint(np.sqrt(sum(np.square(g) for g in np.meshgrid(*k, indexing="ij")))).astype(int)
+    num = np.bincount(r.ravel(), (Fa * np.conj(Fb)).real.ravel())
+    da = np.bincount(r.ravel(), (np.abs(Fa) ** 2).ravel())
+    db = np.bincount(r.ravel(), (np.abs(Fb) ** 2).ravel())
+    sh = slice(1, ori // 2 + 1)
+    fsc = num[sh] / np.sqrt(np.maximum(da[sh] * db[sh], 1e-300))
+    return float(np.min(fsc)), float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def parity_vs_oracle(eng, prob, prec, ref, step_sample, ns):
+    """The CPU-baseline sample through one arm, against the FP64 oracle's E-step of the same particles:
+    max-posterior pose agreement, diff2 / PMAX errors, backprojector grids, and the maps both grids give
+    through the same FP64 reconstruction (FSC, relative L2), with the M-step time per class."""
+    from rome_amd import synth
+    synth.load_engine(eng, prob, precision=prec)
+    eng.reset_sums()
+    r = step_sample(ns)
+    same = np.all(r["argmax"] == ref["argmax"], axis=1)
+    d2 = np.abs(r["min_diff2"] - ref["min_diff2"]) / np.abs(ref["min_diff2"])
+    pm = np.abs(r["meta"][:, 0] - ref["meta"][:, 0])
+    out = {"n": int(ns), "pose_agreement_vs_fp64": round(float(np.mean(same)), 5),
+           "class_agreement_vs_fp64": round(float(np.mean(r["argmax"][:, 0] == ref["argmax"][:, 0])), 5),
+           "min_diff2_rel_err_max": float(np.max(d2)), "min_diff2_rel_err_median": float(np.median(d2)),
+           "pmax_abs_err_max": float(np.max(pm)), "pmax_abs_err_median": float(np.median(pm))}
+    pad = prob["pad"]
+    shape = (pad, pad, pad // 2 + 1)
+    tau2 = np.ones(prob["ori_size"] // 2 + 1)
+    grid_l2, fscs, l2s, ms = [], [], [], []
+    for k in range(prob["nr_classes"]):
+        dg, wg = eng.get_backprojector(k)
+        od, ow = ref["bp_data"][k], ref["bp_weight"][k]
+        if np.linalg.norm(ow) == 0:
+            continue
+        grid_l2.append(float(np.linalg.norm(dg - od) / np.linalg.norm(od)))
+        t0 = time.perf_counter()
+        vol = eng.reconstruct(k, tau2)[0]
+        ms.append((time.perf_counter() - t0) * 1e3)
+        want = eng.reconstruct(k, tau2, data=(od[0::2] + 1j * od[1::2]).reshape(shape), weight=ow.reshape(shape))[0]
+        f, l2 = fsc_min(vol, want)
+        fscs.append(f)
+        l2s.append(l2)
+    out.update({"bp_data_rel_l2_max": max(grid_l2) if grid_l2 else None,
+                "map_fsc_min": min(fscs) if fscs else None, "map_rel_l2_max": max(l2s) if l2s else None,
+                "mstep_ms_per_class": round(float(np.median(ms)), 1) if ms else None})
+    return out
 
 
 def pose_differences(meta_a, meta_b, amax_a, amax_b):
@@ -197,8 +260,10 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = []
+        full_batches = 0  # batches whose coarse list overflowed (the pool then ran on the full pose term)
         for i in range(args.steps):
             res.append(step(args.warmup + i))
+            full_batches += eng.coarse_stats()["mode"] != 0
             if args.verbose:
                 log(f"[{prec}] step {i} done", eng.batch_stats(), eng.coarse_stats())
         torch.cuda.synchronize()
@@ -214,8 +279,7 @@ def main():
         prof = eng.profile()
         eng.set_profiling(False)
         return dict(value=args.steps * B / dt, dt=dt, prof=prof, stats={**eng.batch_stats(), **eng.coarse_stats()},
-                    amax=amax, meta=meta,
-                    truth_agree=float(np.mean(agree)))
+                    amax=amax, meta=meta, truth_agree=float(np.mean(agree)), full_batches=full_batches)
 
     arms = {} if (args.only_mixed and args.precision != "fp32") else {"fp32": run_arm("fp32")}
     if args.precision != "fp32":
@@ -243,7 +307,8 @@ def main():
     gemm_flops = 2.0 * M * (B * T) * (2 * Pc)  # algorithmic: Re<CF, XS> = 2 real MACs per complex pixel
     gemm_tflops = gemm_flops * gemm_calls / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     peak = MFMA_PEAK_TFLOPS[head]
-    listed = stats.get("mode") == 0  # coarse list output (the product path) rather than the full pose term
+    # coarse list output (the product path) in every timed batch, rather than the full pose term
+    listed = stats.get("mode") == 0 and H["full_batches"] == 0
     # psi and psi + 180 deg share one half-table row in the mixed arm when the in-plane sampling pairs them
     # (an even count of psi steps per half turn, the half table a multiple of 128 rows, >= 8 shifts)
     paired = head != "fp32" and prob["nr_psi"] % 8 == 0 and (M // 2) % 128 == 0 and T >= 8
@@ -259,14 +324,25 @@ def main():
                 "traffic_note": ("HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, rocprofv3, "
                                  "profiles/hbm_traffic_map3d_128_k4_hp2_t81_fp16.md); algorithmic: 0.31 GiB "
                                  "operands + the coarse lists") if traffic else None,
-                "avg_launch_ms": round(gemm_ms / max(gemm_calls, 1), 3), "dominant_stage": dom}
+                "avg_launch_ms": round(gemm_ms / max(gemm_calls, 1), 3), "dominant_stage": dom,
+                "coarse_full_batches": H["full_batches"]}
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
+        # the GPU box allots 16 host CPUs to its GPU, whatever nproc reports
+        nproc = os.cpu_count() or 1
+        threads = min(nproc, 16)
         ns = args.cpu_sample or 25 * threads
-        ips, cdt = cpu_baseline(prob, F, ctf, np.arange(ns), threads)
-        cpu = {"value": round(ips, 3), "unit": "images/s", "cores": threads, "kind": "port",
+        ips, cdt, ref = cpu_baseline(prob, F, ctf, np.arange(ns), threads)
+        cpu = {"value": round(ips, 3), "unit": "images/s", "cores": threads, "kind": "port", "nproc": nproc,
+               "cpu_model": cpu_model(),
                "sample": f"{ns} particles of the same workload through the FP64 oracle E-step ({cdt:.1f}s)"}
+
+        def step_sample(ns_):
+            return eng.expectation_device(
+                ns_, F.data_ptr(), F.data_ptr(), ctf.data_ptr(), hxi2.data_ptr(), power.data_ptr(), offx.data_ptr(),
+                offy.data_ptr(), norm.data_ptr(), group.data_ptr(), results=True)
+        parity = {k: parity_vs_oracle(eng, prob, k, ref, step_sample, ns) for k in arms}
     dtype = {"fp32": "fp32", "bf16": "bf16+fp32", "fp16": "fp16+fp32"}[head]
     out = {
         "metric": "E-step images/sec (3D 128^2, K=4) at >=99.5% pose agreement; mixed vs FP32",
@@ -284,6 +360,7 @@ def main():
         "pose_agreement_mixed_vs_fp32": None if pose_agree is None else round(pose_agree, 5),
         "pose_differences": None if pose_agree is None else diffs,
         "pose_agreement_note": None if "fp32" in arms else "mixed arm timed alone (profiling run): not gated",
+        "parity_vs_fp64_oracle": parity,
         "batch_stats": stats,
     }
     print(json.dumps(out), flush=True)

@@ -239,7 +239,8 @@ int rome_project_device(rome_ctx* ctx, int iclass, int size, int nrot, const dou
  * group (no counterpart in the reference, whose timers are TIMEPOINT macros, src/time.h:61-75).
  * Stages: 0 prep, 1 coarse norms, 2 coarse cross term, 3 coarse weights, 4 task build,
  * 5 fine projections, 6 fine diff2, 7 fine weights, 8 sigma2/pdf sums and slices (2D: also the
- * backprojection), 9 store sums, 10 binned 3D backprojection.
+ * backprojection), 9 store sums, 10 binned 3D backprojection, 11 split three-product recompute of the
+ * marked coarse samples (BF16 / FP16 arms; stage 3 then also holds the second weight pass).
  * rome_set_profiling(1) resets the accumulators; rome_get_profile returns the stage count. */
 int rome_set_profiling(rome_ctx* ctx, int on);
 int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, int nstages);

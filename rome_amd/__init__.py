@@ -33,7 +33,7 @@ EXPORTED = (
 COARSE_LIST, COARSE_FULL = 0, 1
 
 STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
-          "fine_weights", "post", "store", "backproject")
+          "fine_weights", "post", "store", "backproject", "coarse_recompute")
 
 
 class RomeError(RuntimeError):

@@ -140,6 +140,9 @@ struct rome_ctx {
     // mixed-precision arm: scaled BF16/FP16 copies of the coarse table and of the image operand
     DevBuf<int> fct_absmax;
     DevBuf<uint16_t> A16, W16;  // A16: every rotation, or the half table of the paired cross term
+    DevBuf<uint16_t> A16lo, W16lo;  // rounding remainders (split three-product recompute of marked samples)
+    DevBuf<double> D2c, c_dmin1;    // recomputed diff2 of the marked coarse samples; first pass's minimum
+    long long last_marked = 0;      // last pool: marked coarse samples recomputed (mixed arms)
     bool pq = false;            // the rotation table pairs psi with psi + 180 deg (conjugate projections)
     DevBuf<float> inv_scale;
     int K16() const { return (2 * Pc_pad + 63) / 64 * 64; }
@@ -202,7 +205,7 @@ struct rome_ctx {
     DevBuf<int> rc_bad;
     std::vector<double> rc_tab_h;
     // stage timing with HIP events on the engine stream (rome_set_profiling)
-    static constexpr int kStages = 11;
+    static constexpr int kStages = 12;
     static constexpr int kStageRecords = 4;  // a stage can run more than once per batch (a coarse rerun)
     bool profiling = false;
     hipEvent_t ev[kStages][kStageRecords][2] = {};
@@ -360,8 +363,10 @@ int ensure_ready(rome_ctx* ctx) {
             HIPCHK(launch_absmax(ctx->stream, ctx->FcT.p, nf, ctx->fct_absmax.p));
             const int rows = ctx->pq ? ctx->Mtot / 2 : ctx->Mpad;
             HIPCHK(ctx->A16.reserve((size_t)rows * K16));
+            HIPCHK(ctx->A16lo.reserve((size_t)rows * K16));
             HIPCHK(launch_fct_to_lowp(ctx->stream, ctx->cfg.precision, ctx->FcT.p, ctx->Pc_pad, ctx->Mpad, ctx->Mtot,
-                                      rows, ctx->pq ? ctx->PSI / 2 : 0, ctx->fct_absmax.p, ctx->A16.p, K16));
+                                      rows, ctx->pq ? ctx->PSI / 2 : 0, ctx->fct_absmax.p, ctx->A16.p, ctx->A16lo.p,
+                                      K16));
         }
         ctx->fct_dirty = false;
     }
@@ -450,6 +455,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(hipMemsetAsync(ctx->WcT.p, 0, (size_t)Kdim * Npad * sizeof(float), s));
     } else {
         HIPCHK(ctx->W16.reserve((size_t)Npad * K16));
+        HIPCHK(ctx->W16lo.reserve((size_t)Npad * K16));
         HIPCHK(ctx->inv_scale.reserve(n));
         if (Npad > Ncols)  // rows past the last column only feed discarded outputs; keep them finite
             HIPCHK(hipMemsetAsync(ctx->W16.p + (size_t)Ncols * K16, 0, (size_t)(Npad - Ncols) * K16 * 2, s));
@@ -520,6 +526,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pl.Sc = ctx->Sc.p;
         pl.fct_absmax = ctx->fct_absmax.p;
         pl.W16 = ctx->W16.p;
+        pl.W16lo = ctx->W16lo.p;
         pl.inv_scale = ctx->inv_scale.p;
         HIPCHK(launch_prep_lowp(s, prec, n, pl));
     }
@@ -555,9 +562,18 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->nblk.reserve(n));
     HIPCHK(ctx->task_off.reserve(n + 1));
     HIPCHK(ctx->blk_off.reserve(n + 1));
+    // The coarse pass marks a superset of its significant samples (mark_margin > 0) whenever their diff2 is
+    // recomputed before the fine pass: a single pass (exact recompute, the pass's weights and sums come
+    // from it) or a mixed-precision arm (split three-product recompute, significance from it).
+    const bool lowp_recompute = ctx->two_pass && prec != ROME_PREC_FP32;
+    if (lowp_recompute) wp.mark_margin = kLowpRecomputeMargin;
+    WeightParams wp_exact = wp;
+    wp_exact.mark_margin = 0.0;
     const WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p,  ctx->c_maxw.p,
                            ctx->c_amax.p, ctx->c_thr.p,   ctx->c_nsig.p,
-                           ctx->c_status.p, ctx->two_pass ? nullptr : ctx->c_tail.p};
+                           ctx->c_status.p, wp.mark_margin > 0. ? ctx->c_tail.p : nullptr};
+    const WeightOutArgs wc_exact{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p,
+                                 ctx->c_nsig.p, ctx->c_status.p, nullptr};
     const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
     long long tot[2];
     int U = 0;
@@ -566,7 +582,9 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     // weights and significance -> fine-pass task counts and the rotations they need; returns after
     // one host synchronisation.  Sparse: per-image keys of the significant samples; dense: the mask
     // (when some image has more than kSigCap significant samples).
-    auto significance = [&](const ListArgs* L, bool dense) -> int {
+    // from_d2c: the second significance pass of a mixed arm, over the recomputed marked samples (D2c, one
+    // block per sample in the first pass's task lists) plus the mass the first pass left unmarked.
+    auto significance = [&](const ListArgs* L, bool dense, bool from_d2c = false) -> int {
         SigArgs sg{nullptr, ctx->TW, nullptr, nullptr, 0};
         if (dense) {
             HIPCHK(ctx->sigmask.reserve((size_t)n * M * ctx->TW));
@@ -580,7 +598,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         }
         HIPCHK(hipMemsetAsync(ctx->used.p, 0, M, s));
         stage_begin(ctx, 3);
-        if (L)
+        if (from_d2c)
+            HIPCHK(launch_weights_fine(s, n, wp_exact, ctx->D2c.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p,
+                                       ctx->blk_off.p, 1, wc_exact, cand, TailArgs{ctx->c_tail.p, ctx->c_dmin1.p}, sg));
+        else if (L)
             HIPCHK(launch_weights_list(s, n, wp, *L, ctx->C2c.p, ctx->emin_key.p, wc, sg, cand));
         else
             HIPCHK(launch_weights_coarse(s, n, wp, ctx->E.p, ctx->C2c.p, ctx->emin_key.p, wc, sg, cand));
@@ -670,6 +691,45 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         ctx->last_coarse_full = 2;  // a list overflowed: the whole batch again with the full pose term
         rc = coarse(false);
         if (rc) return rc;
+    }
+    ctx->last_marked = 0;
+    if (lowp_recompute) {
+        // tasks of the marked samples -> split three-product diff2 -> exact weights and significance
+        for (int i = 0; i < n; ++i)
+            if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: no coarse sample marked", i);
+        const long long mt = tot[0], mb = tot[1];
+        ctx->last_marked = mb;
+        HIPCHK(ctx->task_img.grow(mt));
+        HIPCHK(ctx->task_m.grow(mt));
+        HIPCHK(ctx->task_nt.grow(mt));
+        HIPCHK(ctx->task_blk0.grow(mt));
+        HIPCHK(ctx->blk_task.grow(mb));
+        HIPCHK(ctx->blk_t.grow(mb));
+        HIPCHK(ctx->D2c.grow(mb));
+        HIPCHK(ctx->c_dmin1.reserve(n));
+        const SigArgs sg1 = sig_dense ? SigArgs{ctx->sigmask.p, ctx->TW, nullptr, nullptr, 0}
+                                      : SigArgs{nullptr, ctx->TW, ctx->sig_keys.p, ctx->sig_cnt.p, ctx->sig_cap};
+        HIPCHK(launch_task_emit(s, n, sg1, M, T, ctx->task_off.p, ctx->blk_off.p, ctx->task_img.p, ctx->task_m.p,
+                                ctx->task_blk0.p, ctx->task_nt.p, ctx->blk_task.p, ctx->blk_t.p));
+        RecomputeArgs ra{ctx->task_off.p, ctx->task_m.p, ctx->task_nt.p, ctx->task_blk0.p, ctx->blk_t.p,
+                         ctx->A16.p,      ctx->A16lo.p,  ctx->pq ? ctx->PSI / 2 : 0,
+                         ctx->W16.p,      ctx->W16lo.p,  ctx->inv_scale.p, ctx->At.p, ctx->C2c.p,
+                         K16,             ctx->Mpad,     T,               ctx->D2c.p};
+        stage_begin(ctx, 11);
+        HIPCHK(launch_recompute_split(s, prec, n, ra));
+        stage_end(ctx, 11);
+        HIPCHK(hipMemcpyAsync(ctx->c_dmin1.p, ctx->c_dmin.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
+        rc = significance(nullptr, dense, true);
+        if (rc) return rc;
+        if (!dense) {
+            const int mx = *std::max_element(scount.begin(), scount.end());
+            if (mx > ctx->sig_cap) {
+                ctx->sig_dense_left = 8;
+                rc = significance(nullptr, true, true);
+                if (rc) return rc;
+            }
+        }
     }
     for (int i = 0; i < n; ++i)
         if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);

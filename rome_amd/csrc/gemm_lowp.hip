@@ -426,7 +426,7 @@ DEV float table_scale(const int* absmax_bits) {
 // psih > 0: the half table of the paired cross term, row h = rotation (h / psih) * 2 psih + h % psih.
 template <typename T>
 __global__ void k_fct_to_lowp(const float* __restrict__ FcT, int Pc_pad, int Mpad, int Mtot, int rows, int psih,
-                              const int* __restrict__ absmax_bits, T* __restrict__ A16, int K16) {
+                              const int* __restrict__ absmax_bits, T* __restrict__ A16, T* __restrict__ A16lo, int K16) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)rows * K16) return;
     const float sF = table_scale(absmax_bits);
@@ -434,7 +434,9 @@ __global__ void k_fct_to_lowp(const float* __restrict__ FcT, int Pc_pad, int Mpa
     const int m = psih > 0 ? (row / psih) * 2 * psih + row % psih : row;
     const int kf = k < Pc_pad ? k : (k >= KH && k < KH + Pc_pad ? Pc_pad + (k - KH) : -1);
     const float v = (m < Mtot && kf >= 0) ? FcT[(size_t)kf * Mpad + m] * sF : 0.f;
-    A16[idx] = static_cast<T>(v);
+    const T hi = static_cast<T>(v);
+    A16[idx] = hi;
+    if (A16lo) A16lo[idx] = static_cast<T>(v - static_cast<float>(hi));  // v = hi + lo to 2x the format's bits
 }
 
 // max |x| over an array as float bits (non-negative floats order like their bit patterns); *out starts at 0
@@ -452,6 +454,7 @@ __global__ void k_absmax(const float* __restrict__ x, size_t n, int* __restrict_
 template <typename T>
 __global__ void k_prep_lowp(PrepLowpArgs a) {
     __shared__ float red[16];
+    T* Wlo = a.W16lo ? reinterpret_cast<T*>(a.W16lo) + (size_t)blockIdx.x * a.T * a.K16 : nullptr;
     const int img = blockIdx.x;
     const int g = a.group[img];
     const double* s2 = a.sigma2 + (size_t)g * a.ori_Fsize;
@@ -474,8 +477,14 @@ __global__ void k_prep_lowp(PrepLowpArgs a) {
         const float mc = (float)(1.0 / (a.sigma2_fudge * s2[a.c_ires[p]])) * (a.ctf_in_diff ? fc[nf] : 1.f) * s;
         for (int t = 0; t < a.T; ++t) {
             const float2 XS = cmul(X, a.Sc[(size_t)t * a.Pc + p]);
-            W[(size_t)t * a.K16 + p] = static_cast<T>(mc * XS.x);
-            W[(size_t)t * a.K16 + a.K16 / 2 + p] = static_cast<T>(mc * XS.y);
+            const float vr = mc * XS.x, vi = mc * XS.y;
+            const T hr = static_cast<T>(vr), hi = static_cast<T>(vi);
+            W[(size_t)t * a.K16 + p] = hr;
+            W[(size_t)t * a.K16 + a.K16 / 2 + p] = hi;
+            if (Wlo) {
+                Wlo[(size_t)t * a.K16 + p] = static_cast<T>(vr - static_cast<float>(hr));
+                Wlo[(size_t)t * a.K16 + a.K16 / 2 + p] = static_cast<T>(vi - static_cast<float>(hi));
+            }
         }
     }
     // padding columns: [Pc, K16/2), [K16/2 + Pc, K16)
@@ -485,6 +494,99 @@ __global__ void k_prep_lowp(PrepLowpArgs a) {
         const int d = a.K16 / 2 - a.Pc;
         const int k = j < d ? a.Pc + j : a.K16 / 2 + a.Pc + (j - d);
         W[(size_t)t * a.K16 + k] = static_cast<T>(0.f);
+        if (Wlo) Wlo[(size_t)t * a.K16 + k] = static_cast<T>(0.f);
+    }
+}
+
+// Split three-product recompute of the marked coarse samples of the mixed arms (one block per image).
+// Both operands are held as hi + lo in the arm's format (hi = the operand the coarse GEMM used, lo = the
+// rounding of the rest), and C = A_hi B_hi + A_hi B_lo + A_lo B_hi (the A_lo B_lo term is below the
+// format's squared precision): 2^-16 relative per product in BF16 instead of 2^-8, on MFMA.  The
+// image's tasks (rotations m, sorted) go 32 at a time: rows = those rotations (half-table rows with the
+// sign of the mirrored Im part in the paired layout), columns = every translation in tiles of 32 (one
+// wave per tile); the epilogue keeps the (m, t) of the image's marked samples: D2[b] = At - C / (sF s_img)
+// + the image constant, for block b of the task list.
+template <typename T>
+__global__ __launch_bounds__(256) void k_recompute_split(RecomputeArgs a) {
+    typedef typename Lowp<T>::v8 v8;
+    __shared__ short bidx[32][kMaxT];  // tile row x translation -> block index - tile base (-1: unmarked)
+    __shared__ int rowm[32];
+    __shared__ float rowsg[32];
+    const int img = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long q0 = a.task_off[img], q1 = a.task_off[img + 1];
+    const int KH = a.K16 / 2, Tn = a.T, ntt = (Tn + 31) / 32;
+    const T* Wh = reinterpret_cast<const T*>(a.Whi) + (size_t)img * Tn * a.K16;
+    const T* Wl = reinterpret_cast<const T*>(a.Wlo) + (size_t)img * Tn * a.K16;
+    const T* Ahi = reinterpret_cast<const T*>(a.Ahi);
+    const T* Alo = reinterpret_cast<const T*>(a.Alo);
+    const float inv = a.inv_scale[img];
+    const float* At = a.At + (size_t)img * a.Mpad;
+    const double cimg = a.C2c[img];
+    const v8 zero = {};
+    for (long long qt = q0; qt < q1; qt += 32) {
+        const int nr = (int)min(32LL, q1 - qt);
+        const long long base = a.task_blk0[qt];
+        for (int i = tid; i < 32 * kMaxT; i += 256) (&bidx[0][0])[i] = -1;
+        if (tid < 32) {
+            const int m = tid < nr ? a.task_m[qt + tid] : -1;
+            // paired layout: the rotation with psi >= 180 deg has the conjugate of psi - 180's projection
+            const bool mirrored = m >= 0 && a.psih > 0 && m % (2 * a.psih) >= a.psih;
+            rowm[tid] = m;
+            rowsg[tid] = mirrored ? -1.f : 1.f;
+        }
+        __syncthreads();
+        for (int r = wave; r < nr; r += 4) {
+            const long long b0 = a.task_blk0[qt + r];
+            const int nt = a.task_nt[qt + r];
+            for (int j = lane; j < nt; j += 64) bidx[r][a.blk_t[b0 + j]] = (short)(b0 + j - base);
+        }
+        __syncthreads();
+        const int r = lane & 31;
+        const int m = rowm[r];
+        int h = m;  // the rotation's row of the (half) table
+        if (m >= 0 && a.psih > 0) {
+            const int P2 = 2 * a.psih, qd = m / P2, ps = m - qd * P2;
+            h = qd * a.psih + (ps >= a.psih ? ps - a.psih : ps);
+        }
+        const T* ah_row = Ahi + (size_t)(m >= 0 ? h : 0) * a.K16 + (lane >> 5) * 8;
+        const T* al_row = Alo + (size_t)(m >= 0 ? h : 0) * a.K16 + (lane >> 5) * 8;
+        for (int tt = wave; tt < ntt; tt += 4) {
+            const int t = tt * 32 + r;
+            const bool tok = t < Tn;
+            const T* bh_row = Wh + (size_t)(tok ? t : 0) * a.K16 + (lane >> 5) * 8;
+            const T* bl_row = Wl + (size_t)(tok ? t : 0) * a.K16 + (lane >> 5) * 8;
+            f16v P = {}, Q = {};
+            for (int k0 = 0; k0 < KH; k0 += 16) {
+                const v8 ah = m >= 0 ? *reinterpret_cast<const v8*>(ah_row + k0) : zero;
+                const v8 al = m >= 0 ? *reinterpret_cast<const v8*>(al_row + k0) : zero;
+                const v8 bh = tok ? *reinterpret_cast<const v8*>(bh_row + k0) : zero;
+                const v8 bl = tok ? *reinterpret_cast<const v8*>(bl_row + k0) : zero;
+                P = Lowp<T>::mfma(ah, bh, P);
+                P = Lowp<T>::mfma(ah, bl, P);
+                P = Lowp<T>::mfma(al, bh, P);
+            }
+            for (int k0 = KH; k0 < a.K16; k0 += 16) {
+                const v8 ah = m >= 0 ? *reinterpret_cast<const v8*>(ah_row + k0) : zero;
+                const v8 al = m >= 0 ? *reinterpret_cast<const v8*>(al_row + k0) : zero;
+                const v8 bh = tok ? *reinterpret_cast<const v8*>(bh_row + k0) : zero;
+                const v8 bl = tok ? *reinterpret_cast<const v8*>(bl_row + k0) : zero;
+                Q = Lowp<T>::mfma(ah, bh, Q);
+                Q = Lowp<T>::mfma(ah, bl, Q);
+                Q = Lowp<T>::mfma(al, bh, Q);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int row = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                const int tc = tt * 32 + (lane & 31);
+                if (row >= nr || tc >= Tn) continue;
+                const int rb = bidx[row][tc];
+                if (rb < 0) continue;
+                const float c = P[j] + rowsg[row] * Q[j];
+                const float e = __builtin_fmaf(-c, inv, At[rowm[row]]);
+                a.D2[base + rb] = (double)e + cimg;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -526,15 +628,25 @@ hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a)
 }
 
 hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Pc_pad, int Mpad, int Mtot,
-                              int rows, int psih, const int* fct_absmax, void* A16, int K16) {
+                              int rows, int psih, const int* fct_absmax, void* A16, void* A16lo, int K16) {
     if (K16 % 2 || 2 * Pc_pad > K16) return hipErrorInvalidValue;
     const size_t tot = (size_t)rows * K16;
     const unsigned blocks = (unsigned)((tot + 255) / 256);
     if (precision == 1)
-        k_fct_to_lowp<__bf16><<<blocks, 256, 0, s>>>(FcT, Pc_pad, Mpad, Mtot, rows, psih, fct_absmax, (__bf16*)A16, K16);
+        k_fct_to_lowp<__bf16><<<blocks, 256, 0, s>>>(FcT, Pc_pad, Mpad, Mtot, rows, psih, fct_absmax, (__bf16*)A16,
+                                                      (__bf16*)A16lo, K16);
     else if (precision == 2)
         k_fct_to_lowp<_Float16><<<blocks, 256, 0, s>>>(FcT, Pc_pad, Mpad, Mtot, rows, psih, fct_absmax, (_Float16*)A16,
-                                                        K16);
+                                                        (_Float16*)A16lo, K16);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_recompute_split(hipStream_t s, int precision, int n, const RecomputeArgs& a) {
+    if (n <= 0) return hipSuccess;
+    if (a.T > kMaxT || a.K16 % 32) return hipErrorInvalidValue;  // bidx rows hold kMaxT translations; KH % 16 == 0
+    if (precision == 1) k_recompute_split<__bf16><<<n, 256, 0, s>>>(a);
+    else if (precision == 2) k_recompute_split<_Float16><<<n, 256, 0, s>>>(a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

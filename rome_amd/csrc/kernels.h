@@ -56,6 +56,13 @@ constexpr double kWeightWindow = 64.0;
 // so the exact significant set lies inside the marked one.
 constexpr double kRecomputeMargin = 2.0;
 
+// Mixed arms: the BF16 / FP16 cross term marks the samples within this many nats of its significance
+// threshold for the split three-product recompute, which then decides significance.  The rounding of the
+// low-precision operands moves a diff2 by about 0.04 nat (BF16, 2^-8 per operand over a few hundred
+// products of a 128^2 image; FP16 an eighth of that), so 2 nats keep the exact significant set inside
+// the marked one by a wide margin.
+constexpr double kLowpRecomputeMargin = 2.0;
+
 // Coarse list output of the cross-term GEMMs (instead of the full pose term E): per image the
 // entries e = t*Mtot + m that can lie within kWeightWindow of the image's minimum u, with their E
 // value.  With v = E - ln pdf_direction(m) and D = max - min of ln pdf_offset over the image's
@@ -127,7 +134,26 @@ struct PrepLowpArgs {
     const float2* Sc;
     const int* fct_absmax;    // float bits of max |FcT| (sets the table's scale sF)
     void* W16;                // [Npad][K16]
+    void* W16lo;              // [Npad][K16] rounding remainder of W16 (split recompute), or null
     float* inv_scale;         // [n]
+};
+
+// Split three-product recompute of the marked coarse samples (mixed arms): task lists of the marked
+// samples as the fine pass builds them (tasks = (image, rotation m), blocks = (task, t)), operands hi / lo.
+struct RecomputeArgs {
+    const long long* task_off;  // [n + 1]
+    const int* task_m;
+    const int* task_nt;
+    const long long* task_blk0;
+    const int* blk_t;
+    const void *Ahi, *Alo;      // [rows][K16] coarse table (half table when psih > 0)
+    int psih;
+    const void *Whi, *Wlo;      // [n*T][K16] image operand
+    const float* inv_scale;     // [n]
+    const float* At;            // [n][Mpad]
+    const double* C2c;          // [n]
+    int K16, Mpad, T;
+    double* D2;                 // [blocks] diff2 of the marked samples
 };
 
 struct WeightParams {
@@ -357,7 +383,8 @@ hipError_t launch_weights_list(hipStream_t s, int n, const WeightParams& wp, con
 hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float* reach);
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
-                               const WeightOutArgs& wo, const CandArgs& cand, TailArgs tail = TailArgs{nullptr, nullptr});
+                               const WeightOutArgs& wo, const CandArgs& cand, TailArgs tail = TailArgs{nullptr, nullptr},
+                               SigArgs sig = SigArgs{nullptr, 1, nullptr, nullptr, 0});
 // Task lists of the fine pass from the significant samples: per image the number of tasks
 // (rotations with any significant translation) and blocks (significant samples), their exclusive
 // scans (task_off, blk_off [n + 1]), used[m] = 1 for every rotation some image needs (zero before)
@@ -444,7 +471,8 @@ hipError_t launch_absmax(hipStream_t s, const float* x, size_t n, int* out);
 // A16 rows from FcT (Re at k = p, Im at k = K16/2 + p): rows = Mpad (row = rotation) with psih = 0, or
 // rows = Mtot/2 of the paired cross term (row h = rotation (h / psih) 2 psih + h % psih) with psih > 0
 hipError_t launch_fct_to_lowp(hipStream_t s, int precision, const float* FcT, int Pc_pad, int Mpad, int Mtot,
-                              int rows, int psih, const int* fct_absmax, void* A16, int K16);
+                              int rows, int psih, const int* fct_absmax, void* A16, void* A16lo, int K16);
+hipError_t launch_recompute_split(hipStream_t s, int precision, int n, const RecomputeArgs& a);
 hipError_t launch_prep_lowp(hipStream_t s, int precision, int n, const PrepLowpArgs& a);
 hipError_t launch_gemm_lowp(hipStream_t s, int precision, const LowpGemmArgs& a);
 

@@ -521,7 +521,7 @@ __global__ __launch_bounds__(1024, 8) void k_weights_fine(WeightParams wp, const
                                                        const int* __restrict__ blk_task, const int* __restrict__ blk_t,
                                                        const int* __restrict__ task_m,
                                                        const long long* __restrict__ img_blk, int log2NE, WeightOut wo,
-                                                       Cand cand, TailArgs tail) {
+                                                       Cand cand, TailArgs tail, SigArgs sig) {
     __shared__ WShared sh;
     const int img = blockIdx.x;
     const long long b0 = img_blk[img];
@@ -536,8 +536,7 @@ __global__ __launch_bounds__(1024, 8) void k_weights_fine(WeightParams wp, const
     };
     const AllPre pre{count};
     auto key = [](size_t e, int, int) { return (long long)e; };
-    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, SigArgs{nullptr, 1, nullptr, nullptr, 0}, cand, nullptr,
-                tail);
+    weight_pass(img, count, 0.0, wp, get, pre, key, wo, sh, sig, cand, nullptr, tail);
 }
 
 }  // namespace
@@ -569,13 +568,15 @@ hipError_t launch_list_reach(hipStream_t s, int n, const WeightParams& wp, float
 
 hipError_t launch_weights_fine(hipStream_t s, int n, const WeightParams& wp, const double* D2, const int* blk_task,
                                const int* blk_t, const int* task_m, const long long* img_blk, int NE,
-                               const WeightOutArgs& wo, const CandArgs& c, TailArgs tail) {
+                               const WeightOutArgs& wo, const CandArgs& c, TailArgs tail, SigArgs sig) {
     int log2NE = 0;
     while ((1 << log2NE) < NE) ++log2NE;
     if ((1 << log2NE) != NE) return hipErrorInvalidValue;
+    if ((sig.keys || sig.mask) && NE != 1) return hipErrorInvalidValue;  // marking keys m*T + t: one entry per block
+    if (sig.keys && ((long long)wp.Mtot * wp.T > 0x7fffffffLL || sig.cap > kSigCap)) return hipErrorInvalidValue;
     WeightOut o{wo.dmin, wo.sumw, wo.maxw, wo.amax, wo.thr, wo.nsig, wo.status, wo.tail};
     k_weights_fine<<<n, 1024, 0, s>>>(wp, D2, blk_task, blk_t, task_m, img_blk, log2NE, o, Cand{c.w, c.du, c.e, c.cap},
-                                      tail);
+                                      tail, sig);
     return hipGetLastError();
 }
 
