@@ -305,6 +305,72 @@ int rome_maximization_other(int ref_dim, int ori_size, int nr_classes, int nr_gr
  * CTF correction, refs_are_ctf_corrected = 1). */
 int rome_update_model(rome_ctx* ctx, int install, rome_model* out);
 
+/* Back-projection of whole images at given poses into class iclass's backprojector, the loop of the
+ * offline reconstruction apps/rome_reconstruct.cpp:104-330 (MyBackProjector::backproject,
+ * src/reconstruct.cpp:977-1164): each transform (rome_images layout at current_size, e.g. the fnomask of
+ * rome_prepare_images) is shifted by its offset with the tabulated phase of shiftImageInFourierTransform,
+ * multiplied by its CTF image (weights CTF^2; ones without do_ctf_correction or with fctf NULL) and
+ * scattered into the FP32 grids.  euler [n][3] (rot, tilt, psi deg; 2D uses psi), shift [n][2] pixels.
+ * rome_reconstruct then gives the map (no MAP term, as the offline tool). */
+int rome_backproject_images(rome_ctx* ctx, int iclass, int n, const double* fre, const double* fim, const double* fctf,
+                            const double* euler, const double* shift);
+
+/* ---- Learned pose proposal (no counterpart in the reference, 3D two-pass only) ----
+ * A network (2 * nr_coef low-resolution coefficients of the masked image -> 3 x hidden GELU -> K x C
+ * cell logits + 2 shift outputs; C = the HEALPix directions one order finer than the coarse sampling x
+ * nr_psi_bins in-plane bins; BF16 products, FP32 accumulation and parameters) learns the E-step's own
+ * posteriors and then proposes, per image, the cells whose coarse rotations get exact diff2: the top_k
+ * cells, their neighbouring coarse directions and in-plane bins.  It stands in for the exhaustive coarse
+ * pass of getAllSquaredDifferencesCoarse (src/map3d_optimizer.cpp:1195-1259), whose significance
+ * (src/map3d_optimizer.cpp:1812-1931) stays the judge: an image whose predicted captured mass is below
+ * min_mass takes every rotation, and audit_fraction of every pool also runs the exhaustive pass; a pool
+ * where an audited image's captured posterior mass is below min_audit_mass reruns exhaustively.  Needs
+ * precision BF16 or FP16 (the proposed rotations' diff2 are computed as split three-product sums). */
+typedef struct {
+    int hidden;            /* hidden width, multiple of 32 (1024) */
+    int nr_coef;           /* low-resolution coefficients of the input (812) */
+    int nr_psi_bins;       /* in-plane bins per cell; divides nr_psi (4) */
+    int top_k;             /* cells kept per image before the neighbours (64) */
+    double min_mass;       /* predicted captured mass below which an image takes every rotation (0.99) */
+    double audit_fraction; /* share of each pool that also runs the exhaustive coarse pass (0.02) */
+    double min_audit_mass; /* audited captured mass below which the pool reruns exhaustively (0.99) */
+    int target_cells;      /* cells per training target (32) */
+    double shift_weight;   /* weight of the squared shift error in the loss (1.0) */
+    int max_train;         /* capacity of the training set in images */
+    unsigned long long seed;
+} rome_proposal_config;
+
+enum { ROME_PROPOSAL_OFF = 0, ROME_PROPOSAL_COLLECT = 1, ROME_PROPOSAL_USE = 2 };
+
+/* (Re)creates the network with fresh parameters and an empty training set (needs rome_configure and
+ * rome_set_sampling).  Mode COLLECT: every rome_expectation pool (exhaustive) adds its images and their
+ * posteriors over cells to the training set; USE: pools run on the proposal; OFF: exhaustive. */
+int rome_proposal_configure(rome_ctx* ctx, const rome_proposal_config* cfg);
+int rome_proposal_mode(rome_ctx* ctx, int mode);
+/* Adam over the training set: epochs x (training images / batch) steps of `batch` images (a multiple of
+ * 32), shuffled per epoch; loss_per_epoch [epochs] (may be NULL) receives each epoch's mean loss. */
+int rome_proposal_train(rome_ctx* ctx, int epochs, int batch, double lr, double* loss_per_epoch);
+/* out8: pools, images, mean candidate rotations per image, images that took every rotation, audited
+ * images, lowest audited captured mass, pools rerun exhaustively, training images. */
+int rome_proposal_stats(rome_ctx* ctx, double* out8);
+
+/* Parameters in host FP32, row-major: W1 [hidden][2 nr_coef], W2, W3 [hidden][hidden],
+ * W4 [K C + 2][hidden], biases [rows]; also the layout of gradients. */
+typedef struct {
+    float *W1, *b1, *W2, *b2, *W3, *b3, *W4, *b4;
+} rome_proposal_params;
+int rome_proposal_get_params(rome_ctx* ctx, rome_proposal_params* out);
+int rome_proposal_set_params(rome_ctx* ctx, const rome_proposal_params* in);
+/* Parity hooks: the network's outputs z [n][K C + 2] for host features x [n][2 nr_coef], and the loss and
+ * gradients of one batch (n a multiple of 32) with dense targets [n][K C] (at most target_cells non-zero
+ * per row) and shift targets [n][2]. */
+int rome_proposal_forward(rome_ctx* ctx, int n, const float* x, float* z);
+int rome_proposal_gradients(rome_ctx* ctx, int n, const float* x, const float* target, const float* shift,
+                            double* loss, rome_proposal_params* grads);
+/* Features of the last pool's images [n][2 nr_coef] and their training targets (dense [n][K C], shift
+ * [n][2]) as the training set holds them, for rows row0 .. row0 + n - 1 of the set. */
+int rome_proposal_training_rows(rome_ctx* ctx, int row0, int n, float* x, float* target, float* shift);
+
 /* ---- host sampling helpers (HealpixSampler, C1) ---- */
 int rome_sampling_pose3d(int healpix_order, int oversampling, double perturbation, double* out);
 int rome_sampling_pose2d(double psi_step, int oversampling, double perturbation, double* out);

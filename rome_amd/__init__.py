@@ -27,13 +27,15 @@ EXPORTED = (
     "rome_healpix_pix2ang", "rome_project_device", "rome_set_profiling", "rome_get_profile",
     "rome_get_batch_stats", "rome_maximization_other", "rome_update_model", "rome_fourier_map",
     "rome_reconstruct", "rome_set_coarse_mode", "rome_get_coarse_stats", "rome_prepare_images",
-    "rome_set_significance_cap", "rome_get_significance_stats",
+    "rome_set_significance_cap", "rome_get_significance_stats", "rome_proposal_configure", "rome_proposal_mode",
+    "rome_proposal_train", "rome_proposal_stats", "rome_proposal_get_params", "rome_proposal_set_params",
+    "rome_proposal_forward", "rome_proposal_gradients", "rome_proposal_training_rows", "rome_backproject_images",
 )
 
 COARSE_LIST, COARSE_FULL = 0, 1
 
 STAGES = ("prep", "coarse_norms", "coarse_cross", "coarse_weights", "task_build", "fine_project", "fine_diff2",
-          "fine_weights", "post", "store", "backproject", "coarse_recompute")
+          "fine_weights", "post", "store", "backproject", "coarse_recompute", "proposal")
 
 
 class RomeError(RuntimeError):
@@ -125,6 +127,23 @@ class rome_prepared(C.Structure):
 class rome_prepared_device(C.Structure):
     _fields_ = [("fmask", C.c_void_p), ("fnomask", C.c_void_p), ("fctf", C.c_void_p), ("highres_Xi2", C.c_void_p),
                 ("power_img", C.c_void_p), ("old_offx", C.c_void_p), ("old_offy", C.c_void_p)]
+
+
+class rome_proposal_config(C.Structure):
+    _fields_ = [("hidden", C.c_int), ("nr_coef", C.c_int), ("nr_psi_bins", C.c_int), ("top_k", C.c_int),
+                ("min_mass", C.c_double), ("audit_fraction", C.c_double), ("min_audit_mass", C.c_double),
+                ("target_cells", C.c_int), ("shift_weight", C.c_double), ("max_train", C.c_int),
+                ("seed", C.c_ulonglong)]
+
+
+PROPOSAL_PARAMS = ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4")
+
+
+class rome_proposal_params(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_float)) for n in PROPOSAL_PARAMS]
+
+
+PROPOSAL_MODES = {"off": 0, "collect": 1, "use": 2}
 
 
 def lib():
@@ -411,6 +430,86 @@ class Engine:
         self._check(lib().rome_prepare_images(self._h, C.byref(raw), C.byref(prm), C.byref(ho),
                                               C.byref(do) if do is not None else None), "rome_prepare_images")
         return out
+
+    def backproject_images(self, iclass, images, euler, shift):
+        """rome_backproject_images: transforms (dict with fnomask_re / fnomask_im / fctf [n][F2], as
+        prepare_images returns them) at poses euler [n][3] and offsets shift [n][2] into class iclass's grid."""
+        re_, im_ = _f64(images["fnomask_re"]), _f64(images["fnomask_im"])
+        ctf = _f64(images["fctf"]) if images.get("fctf") is not None else None
+        e, sh = _f64(euler), _f64(shift)
+        self._check(lib().rome_backproject_images(self._h, int(iclass), re_.shape[0], _p(re_), _p(im_), _p(ctf),
+                                                  _p(e), _p(sh)), "rome_backproject_images")
+
+    # ---- learned pose proposal
+    def proposal_configure(self, hidden=1024, nr_coef=812, nr_psi_bins=4, top_k=64, min_mass=0.99,
+                           audit_fraction=0.02, min_audit_mass=0.99, target_cells=32, shift_weight=1.0,
+                           max_train=20000, seed=1):
+        cfg = rome_proposal_config(hidden, nr_coef, nr_psi_bins, top_k, min_mass, audit_fraction, min_audit_mass,
+                                   target_cells, shift_weight, max_train, seed)
+        self._check(lib().rome_proposal_configure(self._h, C.byref(cfg)), "rome_proposal_configure")
+        self.prop = dict(hidden=hidden, nr_coef=nr_coef, ncell=self.K * 4 * self.D * nr_psi_bins,
+                         target_cells=target_cells)
+
+    def proposal_mode(self, mode):
+        self._check(lib().rome_proposal_mode(self._h, PROPOSAL_MODES.get(mode, mode)), "rome_proposal_mode")
+
+    def proposal_train(self, epochs, batch=256, lr=1e-3):
+        out = np.zeros(max(epochs, 1))
+        self._check(lib().rome_proposal_train(self._h, int(epochs), int(batch), C.c_double(lr), _p(out)),
+                    "rome_proposal_train")
+        return out[:epochs]
+
+    def proposal_stats(self):
+        out = np.zeros(8)
+        self._check(lib().rome_proposal_stats(self._h, _p(out)), "rome_proposal_stats")
+        return dict(zip(("pools", "images", "cand_rot_per_image", "images_all_rotations", "audited",
+                         "min_audit_mass", "fallback_pools", "train_images"), out.tolist()))
+
+    def _prop_shapes(self):
+        H, Fin, O = self.prop["hidden"], 2 * self.prop["nr_coef"], self.prop["ncell"] + 2
+        return {"W1": (H, Fin), "b1": (H,), "W2": (H, H), "b2": (H,), "W3": (H, H), "b3": (H,), "W4": (O, H),
+                "b4": (O,)}
+
+    def _prop_struct(self, arrs):
+        return rome_proposal_params(*[arrs[k].ctypes.data_as(C.POINTER(C.c_float)) for k in PROPOSAL_PARAMS])
+
+    def proposal_get_params(self):
+        arrs = {k: np.zeros(v, np.float32) for k, v in self._prop_shapes().items()}
+        self._check(lib().rome_proposal_get_params(self._h, C.byref(self._prop_struct(arrs))), "rome_proposal_get_params")
+        return arrs
+
+    def proposal_set_params(self, params):
+        arrs = {k: np.ascontiguousarray(params[k], dtype=np.float32).reshape(v) for k, v in self._prop_shapes().items()}
+        self._check(lib().rome_proposal_set_params(self._h, C.byref(self._prop_struct(arrs))), "rome_proposal_set_params")
+
+    def proposal_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        z = np.zeros((x.shape[0], self.prop["ncell"] + 2), np.float32)
+        self._check(lib().rome_proposal_forward(self._h, x.shape[0], x.ctypes.data_as(C.POINTER(C.c_float)),
+                                                z.ctypes.data_as(C.POINTER(C.c_float))), "rome_proposal_forward")
+        return z
+
+    def proposal_gradients(self, x, target, shift):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        sh = np.ascontiguousarray(shift, dtype=np.float32)
+        g = {k: np.zeros(v, np.float32) for k, v in self._prop_shapes().items()}
+        loss = C.c_double()
+        fp = C.POINTER(C.c_float)
+        self._check(lib().rome_proposal_gradients(self._h, x.shape[0], x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                                  sh.ctypes.data_as(fp), C.byref(loss), C.byref(self._prop_struct(g))),
+                    "rome_proposal_gradients")
+        return loss.value, g
+
+    def proposal_training_rows(self, row0, n):
+        x = np.zeros((n, 2 * self.prop["nr_coef"]), np.float32)
+        t = np.zeros((n, self.prop["ncell"]), np.float32)
+        sh = np.zeros((n, 2), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(lib().rome_proposal_training_rows(self._h, int(row0), int(n), x.ctypes.data_as(fp),
+                                                      t.ctypes.data_as(fp), sh.ctypes.data_as(fp)),
+                    "rome_proposal_training_rows")
+        return x, t, sh
 
     def set_coarse_mode(self, mode, list_cap=0):
         """ROME_COARSE_LIST (keep only the entries within reach of the weights) or ROME_COARSE_FULL

@@ -18,6 +18,7 @@
 
 #include "../../include/rome_gpu.h"
 #include "kernels.h"
+#include "proposal.h"
 #include "rome_common.h"
 
 namespace rome {
@@ -83,6 +84,40 @@ struct DevBuf {
 
 struct Error {
     int code;
+};
+
+// State of the learned pose proposal (rome_proposal_*): parameters (FP32 masters, BF16 copies and their
+// transposes for the backward products, Adam moments, gradients), the input and neighbour tables, the
+// training set, activations of the current batch, and the per-pool candidate and audit buffers.
+struct ProposalState {
+    bool ready = false;
+    rome_proposal_config cfg{};
+    int mode = ROME_PROPOSAL_OFF;
+    int K = 0, D = 0, PSI = 0, cur = 0;  // shape the network was built for
+    int ncoef = 0, Fin = 0, Fin_pad = 0, H = 0, ncell = 0, O = 0, O_pad = 0, C = 0, nbins = 4, gwords = 0;
+    int rows[4] = {}, cols[4] = {};        // W_l [rows][cols] (W1's columns padded to Fin_pad)
+    DevBuf<float> w[8], m[8], v[8], g[8];  // 2l: W_l+1, 2l+1: b_l+1
+    DevBuf<uint16_t> wq[4], wqT[4];        // BF16 copies of W_l, and of W_l^T (l >= 2)
+    int adam_t = 0;
+    DevBuf<int> coef_n, coef_ires, nbr;
+    int ntrain = 0;
+    DevBuf<uint16_t> tx;  // [max_train][Fin_pad]
+    DevBuf<int> tcell, tcnt;
+    DevBuf<float> tprob, tshift;
+    DevBuf<uint16_t> x, h[3], xT, hT[3], dzq, dzqT, daq, daqT;
+    DevBuf<float> a[3], z, dz, dh;
+    DevBuf<int> bidx, bcell, bcnt;
+    DevBuf<float> bprob, bshift, loss;
+    DevBuf<unsigned> gmask;
+    DevBuf<int> cand_m, cand_cnt, cand_all;
+    DevBuf<float> pmass;
+    DevBuf<float> E_audit;
+    DevBuf<int> audit_emin;
+    DevBuf<double> audit_mass;
+    long long audit_pos = 0;
+    long long st_pools = 0, st_images = 0, st_all = 0, st_audited = 0, st_fallback = 0;
+    double st_cand = 0., st_min_mass = 1.;
+    unsigned long long rng = 0;
 };
 
 }  // namespace rome
@@ -205,7 +240,7 @@ struct rome_ctx {
     DevBuf<int> rc_bad;
     std::vector<double> rc_tab_h;
     // stage timing with HIP events on the engine stream (rome_set_profiling)
-    static constexpr int kStages = 12;
+    static constexpr int kStages = 13;
     static constexpr int kStageRecords = 4;  // a stage can run more than once per batch (a coarse rerun)
     bool profiling = false;
     hipEvent_t ev[kStages][kStageRecords][2] = {};
@@ -213,6 +248,7 @@ struct rome_ctx {
     double stage_ms[kStages] = {};
     long long stage_calls[kStages] = {};
     long long last_U = 0;
+    ProposalState prop;
     // batched read-backs / uploads (pack kernel + pinned staging)
     DevBuf<unsigned char> rb_dev;
     PinnedBuf rb_host, up_host;
@@ -437,6 +473,171 @@ WeightParams weight_params(rome_ctx* ctx) {
     return wp;
 }
 
+// ------------------------------------------------------------------ learned pose proposal (host side)
+unsigned long long splitmix(unsigned long long& st) {
+    unsigned long long z = (st += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double uniform01(unsigned long long& st) { return ((splitmix(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+double gauss(unsigned long long& st) {
+    const double u = uniform01(st), v = uniform01(st);
+    return std::sqrt(-2.0 * std::log(u)) * std::cos(2.0 * 3.14159265358979323846 * v);
+}
+
+// Activation buffers for `rows` images.
+int prop_acts(rome_ctx* ctx, int rows) {
+    ProposalState& P = ctx->prop;
+    const size_t r = (size_t)std::max(rows, 1), r32 = (r + 31) / 32 * 32;
+    HIPCHK(P.x.grow(r * P.Fin_pad));
+    HIPCHK(P.xT.grow((size_t)P.Fin_pad * r32));
+    for (int l = 0; l < 3; ++l) {
+        HIPCHK(P.a[l].grow(r * P.H));
+        HIPCHK(P.h[l].grow(r * P.H));
+        HIPCHK(P.hT[l].grow((size_t)P.H * r32));
+    }
+    HIPCHK(P.z.grow(r * P.O_pad));
+    HIPCHK(P.dz.grow(r * P.O_pad));
+    HIPCHK(P.dzq.grow(r * P.O_pad));
+    HIPCHK(P.dzqT.grow((size_t)P.O_pad * r32));
+    HIPCHK(P.dh.grow(r * P.H));
+    HIPCHK(P.daq.grow(r * P.H));
+    HIPCHK(P.daqT.grow((size_t)P.H * r32));
+    const size_t tc = (size_t)P.cfg.target_cells;
+    HIPCHK(P.bcell.grow(r * tc));
+    HIPCHK(P.bprob.grow(r * tc));
+    HIPCHK(P.bcnt.grow(r));
+    HIPCHK(P.bshift.grow(2 * r));
+    return ROME_OK;
+}
+
+// BF16 copies of the weight matrices and the transposes the backward products read.
+int prop_refresh(rome_ctx* ctx) {
+    ProposalState& P = ctx->prop;
+    hipStream_t s = ctx->stream;
+    for (int l = 0; l < 4; ++l) {
+        HIPCHK(mlp_to_bf16(s, P.w[2 * l].p, P.wq[l].p, (size_t)P.rows[l] * P.cols[l]));
+        if (l > 0) HIPCHK(mlp_transpose(s, P.wq[l].p, false, P.rows[l], P.cols[l], P.cols[l], P.wqT[l].p, l == 3 ? P.O_pad : P.rows[l]));
+    }
+    return ROME_OK;
+}
+
+// Forward pass of `rows` images whose features are in P.x: pre-activations a, activations h, outputs z.
+int prop_forward(rome_ctx* ctx, int rows) {
+    ProposalState& P = ctx->prop;
+    hipStream_t s = ctx->stream;
+    const uint16_t* in = P.x.p;
+    int ldin = P.Fin_pad, kin = P.Fin_pad;
+    for (int l = 0; l < 3; ++l) {
+        HIPCHK(mlp_gemm(s, in, ldin, P.wq[l].p, P.cols[l], P.a[l].p, P.H, P.w[2 * l + 1].p, rows, P.H, kin));
+        HIPCHK(mlp_gelu_fwd(s, P.a[l].p, P.H, P.h[l].p, P.H, rows, P.H));
+        in = P.h[l].p;
+        ldin = kin = P.H;
+    }
+    HIPCHK(mlp_gemm(s, in, P.H, P.wq[3].p, P.H, P.z.p, P.O_pad, P.w[7].p, rows, P.O, P.H));
+    return ROME_OK;
+}
+
+// Loss and gradients of a batch of B images (features in P.x, forward done, targets in P.b*); the loss
+// is added to P.loss.
+int prop_backward(rome_ctx* ctx, int B) {
+    ProposalState& P = ctx->prop;
+    hipStream_t s = ctx->stream;
+    HIPCHK(mlp_softmax_ce(s, P.z.p, P.O_pad, P.ncell, B, P.bcell.p, P.bprob.p, P.bcnt.p, P.cfg.target_cells,
+                          P.bshift.p, (float)P.cfg.shift_weight, P.dz.p, P.dzq.p, P.O_pad, P.loss.p));
+    // layer 4: dW4 = dz^T h3, db4 = sum dz, dh3 = dz W4
+    HIPCHK(mlp_transpose(s, P.dzq.p, false, B, P.O_pad, P.O_pad, P.dzqT.p, B));
+    HIPCHK(mlp_transpose(s, P.h[2].p, false, B, P.H, P.H, P.hT[2].p, B));
+    HIPCHK(mlp_gemm(s, P.dzqT.p, B, P.hT[2].p, B, P.g[6].p, P.H, nullptr, P.O, P.H, B));
+    HIPCHK(mlp_colsum(s, P.dz.p, B, P.O, P.O_pad, P.g[7].p));
+    HIPCHK(mlp_gemm(s, P.dzq.p, P.O_pad, P.wqT[3].p, P.O_pad, P.dh.p, P.H, nullptr, B, P.H, P.O_pad));
+    for (int l = 2; l >= 0; --l) {
+        // da = dh * gelu'(a); dW = da^T input; db = sum da; dh(prev) = da W
+        HIPCHK(mlp_gelu_bwd(s, P.dh.p, P.a[l].p, P.daq.p, (size_t)B * P.H));
+        HIPCHK(mlp_transpose(s, P.daq.p, false, B, P.H, P.H, P.daqT.p, B));
+        const uint16_t* inT;
+        int ncol;
+        if (l == 0) {
+            HIPCHK(mlp_transpose(s, P.x.p, false, B, P.Fin_pad, P.Fin_pad, P.xT.p, B));
+            inT = P.xT.p;
+            ncol = P.Fin_pad;
+        } else {
+            HIPCHK(mlp_transpose(s, P.h[l - 1].p, false, B, P.H, P.H, P.hT[l - 1].p, B));
+            inT = P.hT[l - 1].p;
+            ncol = P.H;
+        }
+        HIPCHK(mlp_gemm(s, P.daqT.p, B, inT, B, P.g[2 * l].p, ncol, nullptr, P.H, ncol, B));
+        HIPCHK(mlp_colsum(s, P.dh.p, B, P.H, P.H, P.g[2 * l + 1].p));
+        if (l > 0) HIPCHK(mlp_gemm(s, P.daq.p, P.H, P.wqT[l].p, P.H, P.dh.p, P.H, nullptr, B, P.H, P.H));
+    }
+    return ROME_OK;
+}
+
+int prop_adam(rome_ctx* ctx, double lr) {
+    ProposalState& P = ctx->prop;
+    hipStream_t s = ctx->stream;
+    ++P.adam_t;
+    const double b1 = 0.9, b2 = 0.999;
+    const float bc1 = (float)(1.0 / (1.0 - std::pow(b1, P.adam_t))), bc2 = (float)(1.0 / (1.0 - std::pow(b2, P.adam_t)));
+    for (int q = 0; q < 8; ++q) {
+        const int l = q / 2;
+        const size_t n = (q & 1) ? (size_t)P.rows[l] : (size_t)P.rows[l] * P.cols[l];
+        HIPCHK(mlp_adam(s, P.w[q].p, P.m[q].p, P.v[q].p, P.g[q].p, (q & 1) ? nullptr : P.wq[l].p, n, (float)lr,
+                        (float)b1, (float)b2, 1e-8f, bc1, bc2));
+    }
+    for (int l = 1; l < 4; ++l)
+        HIPCHK(mlp_transpose(s, P.wq[l].p, false, P.rows[l], P.cols[l], P.cols[l], P.wqT[l].p, l == 3 ? P.O_pad : P.rows[l]));
+    return ROME_OK;
+}
+
+// The last pool's images (fine pass done) -> training rows: features and posterior targets.
+int prop_collect(rome_ctx* ctx, int n) {
+    ProposalState& P = ctx->prop;
+    const int rows = std::min(n, P.cfg.max_train - P.ntrain);
+    if (rows <= 0) return ROME_OK;
+    hipStream_t s = ctx->stream;
+    HIPCHK(mlp_features(s, rows, ctx->in.fmask, ctx->F2f, P.coef_n.p, P.coef_ires.p, P.ncoef, ctx->sigma2.p,
+                        ctx->in.group, ctx->oriF, P.tx.p + (size_t)P.ntrain * P.Fin_pad, P.Fin_pad));
+    ProposalTargetArgs ta;
+    ta.K = ctx->K;
+    ta.C = P.C;
+    ta.nbins = P.nbins;
+    ta.R = ctx->R;
+    ta.PSI = ctx->PSI;
+    ta.NOR = ctx->NORf;
+    ta.NOT = ctx->NOTf;
+    ta.tcap = P.cfg.target_cells;
+    ta.img_blk = ctx->blk_off.p;
+    ta.blk_task = ctx->blk_task.p;
+    ta.blk_t = ctx->blk_t.p;
+    ta.task_m = ctx->task_m.p;
+    ta.D2 = ctx->D2.p;
+    ta.dmin = ctx->f_dmin.p;
+    ta.sumw = ctx->f_sumw.p;
+    ta.pdf_m = ctx->pdf_m.p;
+    ta.trans_x = ctx->d_tx.p;
+    ta.trans_y = ctx->d_ty.p;
+    ta.old_offx = ctx->in.offx;
+    ta.old_offy = ctx->in.offy;
+    ta.prior_x = ctx->prior_x.p;
+    ta.prior_y = ctx->prior_y.p;
+    ta.sigma2_offset = ctx->cfg.sigma2_offset;
+    ta.shift_x = ctx->d_sxf.p;
+    ta.shift_y = ctx->d_syf.p;
+    double rng = 1.0;
+    for (size_t t = 0; t < ctx->tx.size(); ++t) rng = std::max(rng, std::hypot(ctx->tx[t], ctx->ty[t]));
+    ta.shift_scale = (float)rng;
+    ta.row0 = P.ntrain;
+    ta.tcell = P.tcell.p;
+    ta.tprob = P.tprob.p;
+    ta.tcnt = P.tcnt.p;
+    ta.tshift = P.tshift.p;
+    HIPCHK(proposal_targets(s, rows, ta));
+    P.ntrain += rows;
+    return ROME_OK;
+}
+
 // The E-step on the device inputs ctx->in points to.
 int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     int rc = ensure_ready(ctx);
@@ -454,11 +655,12 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(ctx->WcT.reserve((size_t)Kdim * Npad));
         HIPCHK(hipMemsetAsync(ctx->WcT.p, 0, (size_t)Kdim * Npad * sizeof(float), s));
     } else {
-        HIPCHK(ctx->W16.reserve((size_t)Npad * K16));
-        HIPCHK(ctx->W16lo.reserve((size_t)Npad * K16));
+        // 256 rows of slack: the audit slice of a learned-proposal pool reads whole 256-row tiles from its start
+        HIPCHK(ctx->W16.reserve((size_t)(Npad + 256) * K16));
+        HIPCHK(ctx->W16lo.reserve((size_t)(Npad + 256) * K16));
         HIPCHK(ctx->inv_scale.reserve(n));
-        if (Npad > Ncols)  // rows past the last column only feed discarded outputs; keep them finite
-            HIPCHK(hipMemsetAsync(ctx->W16.p + (size_t)Ncols * K16, 0, (size_t)(Npad - Ncols) * K16 * 2, s));
+        // rows past the last column only feed discarded outputs; keep them finite
+        HIPCHK(hipMemsetAsync(ctx->W16.p + (size_t)Ncols * K16, 0, (size_t)(Npad + 256 - Ncols) * K16 * 2, s));
     }
     const int ldn = (n + 127) / 128 * 128;  // images of the A-term operand, padded with zero columns
     HIPCHK(ctx->MC2c.reserve((size_t)ctx->Kq * ldn));
@@ -565,13 +767,19 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     // The coarse pass marks a superset of its significant samples (mark_margin > 0) whenever their diff2 is
     // recomputed before the fine pass: a single pass (exact recompute, the pass's weights and sums come
     // from it) or a mixed-precision arm (split three-product recompute, significance from it).
-    const bool lowp_recompute = ctx->two_pass && prec != ROME_PREC_FP32;
+    // A pool on the learned proposal computes its candidates' diff2 as split three-product sums already;
+    // it needs the recompute only when it falls back to the exhaustive low-precision pass.
+    ProposalState& P = ctx->prop;
+    const bool learned = P.ready && P.mode == ROME_PROPOSAL_USE && prec != ROME_PREC_FP32 && ctx->two_pass &&
+                         ctx->is3d && ctx->coarse_mode == ROME_COARSE_LIST && P.K == ctx->K && P.D == ctx->D &&
+                         P.PSI == ctx->PSI && P.cur == c.current_size;
+    bool lowp_recompute = ctx->two_pass && prec != ROME_PREC_FP32 && !learned;
     if (lowp_recompute) wp.mark_margin = kLowpRecomputeMargin;
     WeightParams wp_exact = wp;
     wp_exact.mark_margin = 0.0;
-    const WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p,  ctx->c_maxw.p,
-                           ctx->c_amax.p, ctx->c_thr.p,   ctx->c_nsig.p,
-                           ctx->c_status.p, wp.mark_margin > 0. ? ctx->c_tail.p : nullptr};
+    WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p,  ctx->c_maxw.p,
+                     ctx->c_amax.p, ctx->c_thr.p,   ctx->c_nsig.p,
+                     ctx->c_status.p, wp.mark_margin > 0. ? ctx->c_tail.p : nullptr};
     const WeightOutArgs wc_exact{ctx->c_dmin.p, ctx->c_sumw.p, ctx->c_maxw.p, ctx->c_amax.p, ctx->c_thr.p,
                                  ctx->c_nsig.p, ctx->c_status.p, nullptr};
     const CandArgs cand{ctx->cand_w.p, ctx->cand_du.p, ctx->cand_e.p, rome_ctx::kCandCap};
@@ -579,6 +787,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     int U = 0;
     std::vector<int> cstatus(n), lcount, scount(n);
     bool sig_dense = false;
+    std::vector<Seg> extra_rb;  // further device arrays read back with the significance results
     // weights and significance -> fine-pass task counts and the rotations they need; returns after
     // one host synchronisation.  Sparse: per-image keys of the significant samples; dense: the mask
     // (when some image has more than kSigCap significant samples).
@@ -612,33 +821,54 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         stage_end(ctx, 4);
         if (L) lcount.resize(n);
         sig_dense = dense;
-        return readback(ctx, {{ctx->task_off.p + n, &tot[0], sizeof(long long)},
+        std::vector<Seg> segs{{ctx->task_off.p + n, &tot[0], sizeof(long long)},
                               {ctx->blk_off.p + n, &tot[1], sizeof(long long)},
                               {ctx->ucount.p, &U, sizeof(int)},
                               {ctx->c_status.p, cstatus.data(), n * sizeof(int)},
                               {ctx->list_cnt.p, L ? lcount.data() : nullptr, n * sizeof(int)},
-                              {ctx->sig_cnt.p, dense ? nullptr : scount.data(), n * sizeof(int)}});
+                              {ctx->sig_cnt.p, dense ? nullptr : scount.data(), n * sizeof(int)}};
+        segs.insert(segs.end(), extra_rb.begin(), extra_rb.end());
+        extra_rb.clear();
+        return readback(ctx, segs);
     };
     // cross term -> list of the entries within reach of the weights (or the whole pose term E) ->
     // weights and significance -> fine-pass tasks; returns after one host synchronisation
+    // the coarse list's buffers: every entry when the pool's lists fit in 1 GiB, else a quarter of them (a
+    // list past its cap sends the pool through the full pose term)
+    auto list_setup = [&](ListArgs& L) -> int {
+        const size_t per = (size_t)M * T;
+        const int cap = ctx->list_cap > 0                       ? ctx->list_cap
+                        : (size_t)n * per * 8 <= ((size_t)1 << 30) ? (int)per
+                                                                  : (int)std::max<size_t>(65536, (per + 3) / 4);
+        HIPCHK(ctx->vbound.reserve(n));
+        HIPCHK(ctx->list_cnt.reserve(n));
+        HIPCHK(ctx->list_ent.reserve((size_t)n * cap));
+        HIPCHK(ctx->list_val.reserve((size_t)n * cap));
+        HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->vbound.p, 0x7fffffff, n, s));
+        HIPCHK(hipMemsetAsync(ctx->list_cnt.p, 0, (size_t)n * sizeof(int), s));
+        L = ListArgs{ctx->lpd_row.p, ctx->reach.p, ctx->vbound.p, ctx->list_ent.p, ctx->list_val.p, ctx->list_cnt.p, cap};
+        return ROME_OK;
+    };
+    // weights and significance of the coarse pass (keys, or the mask past their cap)
+    auto finish = [&](const ListArgs* L) -> int {
+        const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
+        int r = significance(L, dense);
+        ctx->last_sig_form = dense ? 1 : 0;
+        ctx->last_sig_max = dense ? -1 : *std::max_element(scount.begin(), scount.end());
+        if (r || dense) return r;
+        if (ctx->last_sig_max > ctx->sig_cap) {
+            ctx->sig_dense_left = 8;  // more significant samples than the keys hold: the mask, for a while
+            ctx->last_sig_form = 2;
+            return significance(L, true);
+        }
+        return 0;
+    };
     auto coarse = [&](bool list) -> int {
         HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
         ListArgs L{};
         if (list) {
-            // every entry when the pool's lists fit in 1 GiB, else a quarter of them (a list past its
-            // cap sends the pool through the full pose term)
-            const size_t per = (size_t)M * T;
-            const int cap = ctx->list_cap > 0                       ? ctx->list_cap
-                            : (size_t)n * per * 8 <= ((size_t)1 << 30) ? (int)per
-                                                                      : (int)std::max<size_t>(65536, (per + 3) / 4);
-            HIPCHK(ctx->vbound.reserve(n));
-            HIPCHK(ctx->list_cnt.reserve(n));
-            HIPCHK(ctx->list_ent.reserve((size_t)n * cap));
-            HIPCHK(ctx->list_val.reserve((size_t)n * cap));
-            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->vbound.p, 0x7fffffff, n, s));
-            HIPCHK(hipMemsetAsync(ctx->list_cnt.p, 0, (size_t)n * sizeof(int), s));
-            L = ListArgs{ctx->lpd_row.p, ctx->reach.p, ctx->vbound.p, ctx->list_ent.p, ctx->list_val.p,
-                         ctx->list_cnt.p, cap};
+            const int lr = list_setup(L);
+            if (lr) return lr;
         } else {
             HIPCHK(ctx->E.reserve((size_t)n * M * T));
         }
@@ -663,22 +893,104 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             HIPCHK(launch_gemm_lowp(s, prec, la));
         }
         stage_end(ctx, 2);
-        const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
-        int r = significance(list ? &L : nullptr, dense);
-        ctx->last_sig_form = dense ? 1 : 0;
-        ctx->last_sig_max = dense ? -1 : *std::max_element(scount.begin(), scount.end());
-        if (r || dense) return r;
-        if (ctx->last_sig_max > ctx->sig_cap) {
-            ctx->sig_dense_left = 8;  // more significant samples than the keys hold: the mask, for a while
-            ctx->last_sig_form = 2;
-            return significance(list ? &L : nullptr, true);
+        return finish(list ? &L : nullptr);
+    };
+    // Learned proposal: network -> candidate rotations per image -> their diff2 (split three-product) into
+    // the coarse list; the audit slice also runs the exhaustive pass (full pose term) for its captured mass.
+    int na = 0;
+    std::vector<double> audit_h;
+    std::vector<int> cand_h, all_h;
+    auto coarse_learned = [&]() -> int {
+        HIPCHK(P.gmask.reserve((size_t)n * P.gwords));
+        HIPCHK(P.cand_m.reserve((size_t)n * M));
+        HIPCHK(P.cand_cnt.reserve(n));
+        HIPCHK(P.cand_all.reserve(n));
+        HIPCHK(P.pmass.reserve(n));
+        int pr = prop_acts(ctx, n);
+        if (pr) return pr;
+        stage_begin(ctx, 12);
+        HIPCHK(mlp_features(s, n, ctx->in.fmask, ctx->F2f, P.coef_n.p, P.coef_ires.p, P.ncoef, ctx->sigma2.p,
+                            ctx->in.group, ctx->oriF, P.x.p, P.Fin_pad));
+        pr = prop_forward(ctx, n);
+        if (pr) return pr;
+        CandidateArgs ca{P.z.p,     P.O_pad,   ctx->K,       P.C,          ctx->D,     P.nbins,
+                         ctx->PSI,  P.cfg.top_k, (float)P.cfg.min_mass, P.nbr.p, P.gmask.p, P.gwords,
+                         P.cand_m.p, M,         P.cand_cnt.p, P.pmass.p,   P.cand_all.p};
+        HIPCHK(select_candidates(s, n, ca));
+        na = P.cfg.audit_fraction > 0. ? std::min(n, (int)std::ceil(P.cfg.audit_fraction * n)) : 0;
+        const int a0 = na ? (int)(P.audit_pos % (long long)(n - na + 1)) : 0;
+        P.audit_pos += na;
+        if (na) {
+            HIPCHK(P.E_audit.reserve((size_t)na * M * T));
+            HIPCHK(P.audit_emin.reserve(na));
+            HIPCHK(P.audit_mass.reserve(na));
+            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)P.audit_emin.p, 0x7fffffff, na, s));
+            const int ncol = na * T, npad = (ncol + 255) / 256 * 256;
+            LowpGemmArgs la{ctx->A16.p, ctx->W16.p + (size_t)a0 * T * K16, ctx->At.p + (size_t)a0 * ctx->Mpad,
+                            ctx->inv_scale.p + a0, P.E_audit.p, ctx->valid.p, P.audit_emin.p, K16, ctx->Mpad, npad,
+                            M, ncol, T, gemm_sub_band(ctx->Mpad / 256, (size_t)256 * K16 * 2, (size_t)5 << 19),
+                            ListArgs{}, 0, 0};
+            if (ctx->pq) {
+                la.Mh = M / 2;
+                la.psih = ctx->PSI / 2;
+                la.sband = gemm_sub_band(la.Mh / 128, (size_t)128 * K16 * 2, (size_t)5 << 19);
+                la.list.lpd = ctx->lpd_row.p;
+            }
+            HIPCHK(launch_gemm_lowp(s, prec, la));
+            AuditArgs aa{P.E_audit.p, a0, T, M, P.nbins, P.gwords, P.gmask.p, wp, P.audit_mass.p};
+            HIPCHK(audit_mass(s, na, aa));
         }
-        return 0;
+        stage_end(ctx, 12);
+        ListArgs L{};
+        pr = list_setup(L);
+        if (pr) return pr;
+        HIPCHK(ctx->reach.reserve(n));
+        CandCoarseArgs cc{P.cand_m.p,  P.cand_cnt.p, M,          ctx->A16.p, ctx->A16lo.p, ctx->pq ? ctx->PSI / 2 : 0,
+                          ctx->W16.p,  ctx->W16lo.p, ctx->inv_scale.p, ctx->At.p, ctx->reach.p, K16,
+                          ctx->Mpad,   M,            T,          L,          ctx->emin_key.p};
+        stage_begin(ctx, 2);
+        HIPCHK(cand_coarse(s, prec, n, cc));
+        stage_end(ctx, 2);
+        audit_h.assign(na, 0.);
+        cand_h.assign(n, 0);
+        all_h.assign(n, 0);
+        extra_rb = {{P.audit_mass.p, na ? audit_h.data() : nullptr, na * sizeof(double)},
+                    {P.cand_cnt.p, cand_h.data(), n * sizeof(int)},
+                    {P.cand_all.p, all_h.data(), n * sizeof(int)}};
+        return finish(&L);
     };
     // the single pass (no oversampling) takes its significant diff2 back from the full pose term
     const bool use_list = ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass;
-    rc = coarse(use_list);
-    if (rc) return rc;
+    bool fell_back = false;
+    if (learned) {
+        rc = coarse_learned();
+        if (rc) return rc;
+        double mn = 1.0;
+        for (double v : audit_h) mn = std::min(mn, v);
+        P.st_pools += 1;
+        P.st_images += n;
+        for (int i = 0; i < n; ++i) {
+            P.st_cand += cand_h[i];
+            P.st_all += all_h[i];
+        }
+        P.st_audited += na;
+        P.st_min_mass = std::min(P.st_min_mass, mn);
+        if (mn < P.cfg.min_audit_mass) fell_back = true;
+    }
+    // a pool off the proposal (or one the audit rejected): the exhaustive low-precision pass + recompute
+    auto exhaustive_setup = [&]() {
+        if (prec != ROME_PREC_FP32 && ctx->two_pass && !lowp_recompute) {
+            lowp_recompute = true;
+            wp.mark_margin = kLowpRecomputeMargin;
+            wc.tail = ctx->c_tail.p;
+        }
+    };
+    if (!learned || fell_back) {
+        if (fell_back) P.st_fallback += 1;
+        exhaustive_setup();
+        rc = coarse(use_list);
+        if (rc) return rc;
+    }
     if (ctx->sig_dense_left > 0) --ctx->sig_dense_left;
     ctx->last_list_max = 0;
     ctx->last_list_total = 0;
@@ -689,6 +1001,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     ctx->last_coarse_full = use_list ? 0 : 1;
     if (use_list && std::any_of(cstatus.begin(), cstatus.end(), [](int v) { return (v & 8) != 0; })) {
         ctx->last_coarse_full = 2;  // a list overflowed: the whole batch again with the full pose term
+        exhaustive_setup();
         rc = coarse(false);
         if (rc) return rc;
     }
@@ -785,6 +1098,11 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE, wf,
                                cand, tail));
     stage_end(ctx, 7);
+    if (P.ready && P.mode == ROME_PROPOSAL_COLLECT && ctx->two_pass && ctx->is3d && P.K == ctx->K && P.D == ctx->D &&
+        P.PSI == ctx->PSI && P.cur == c.current_size) {
+        rc = prop_collect(ctx, n);  // the pool's posteriors become training targets
+        if (rc) return rc;
+    }
     const double* thr_p = ctx->f_thr.p;
     const double* sum_p = ctx->f_sumw.p;
     const double* dmin_p = ctx->f_dmin.p;
@@ -1952,6 +2270,395 @@ int rome_prepare_images(rome_ctx* ctx, const rome_raw_images* in, const rome_pre
             }
         }
         HIPCHK(hipStreamSynchronize(s));  // the staging buffers are reused by the next chunk
+    }
+    return ROME_OK;
+}
+
+// ------------------------------------------------------------------ learned pose proposal (C-ABI)
+int rome_proposal_configure(rome_ctx* ctx, const rome_proposal_config* cfg) {
+    if (!ctx || !cfg) return ROME_ERR_ARG;
+    if (!ctx->configured || !ctx->have_sampling)
+        return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure needs rome_configure and rome_set_sampling");
+    if (!ctx->is3d || !ctx->two_pass)
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "the learned proposal needs 3D with adaptive oversampling");
+    const int nb = cfg->nr_psi_bins;
+    if (cfg->hidden <= 0 || cfg->hidden % 32 || cfg->nr_coef <= 0 || nb <= 0 || ctx->PSI % nb || cfg->top_k <= 0 ||
+        cfg->target_cells <= 0 || cfg->target_cells > 1024 || cfg->max_train < 0 || cfg->audit_fraction < 0 ||
+        cfg->audit_fraction > 1)
+        return fail(ctx, ROME_ERR_ARG, "bad proposal configuration");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ProposalState& P = ctx->prop;
+    P.ready = false;
+    P.cfg = *cfg;
+    P.mode = ROME_PROPOSAL_OFF;
+    P.K = ctx->K;
+    P.D = ctx->D;
+    P.PSI = ctx->PSI;
+    P.cur = ctx->cfg.current_size;
+    P.nbins = nb;
+    P.C = 4 * ctx->D * nb;  // directions one HEALPix order finer (4 NEST children per coarse direction)
+    P.ncell = ctx->K * P.C;
+    P.O = P.ncell + 2;
+    P.O_pad = (P.O + 31) / 32 * 32;
+    P.ncoef = cfg->nr_coef;
+    P.Fin = 2 * P.ncoef;
+    P.Fin_pad = (P.Fin + 31) / 32 * 32;
+    P.H = cfg->hidden;
+    P.gwords = (ctx->K * ctx->D * nb + 31) / 32;
+    if (P.gwords > kMaxGroupWords || (size_t)P.ncell * sizeof(float) > 64 * 1024)
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "too many proposal cells (%d)", P.ncell);
+    // input coefficients: the lowest frequencies of the stored half-transform (x >= 0, the x = 0 column
+    // for y >= 0 only: its y < 0 half holds their conjugates), by |k| then by index
+    const int cur = ctx->cfg.current_size, Fs = cur / 2 + 1;
+    std::vector<std::pair<long long, int>> px;
+    for (int i = 0; i < cur; ++i)
+        for (int j = 0; j < Fs; ++j) {
+            const int y = i < Fs ? i : i - cur;
+            if (j == 0 && y < 0) continue;
+            px.push_back({(long long)(j * j + y * y) * (1LL << 32) + i * Fs + j, i * Fs + j});
+        }
+    if ((int)px.size() < P.ncoef) return fail(ctx, ROME_ERR_ARG, "nr_coef exceeds the image's coefficients");
+    std::sort(px.begin(), px.end());
+    std::vector<int> cn(P.ncoef), ci(P.ncoef);
+    for (int q = 0; q < P.ncoef; ++q) {
+        cn[q] = px[q].second;
+        const int i = cn[q] / Fs, j = cn[q] % Fs, y = i < Fs ? i : i - cur;
+        ci[q] = std::min((int)std::lround(std::sqrt((double)(j * j + y * y))), ctx->oriF - 1);
+    }
+    HIPCHK(P.coef_n.upload(cn.data(), cn.size(), ctx->stream));
+    HIPCHK(P.coef_ires.upload(ci.data(), ci.size(), ctx->stream));
+    // the 8 nearest coarse directions of each (angles between the HEALPix pixel centres)
+    int order = 0;
+    while ((12 << (2 * order)) < ctx->D) ++order;
+    if ((12 << (2 * order)) != ctx->D) return fail(ctx, ROME_ERR_UNSUPPORTED, "nr_dir is not a HEALPix grid");
+    std::vector<double> ux(ctx->D), uy(ctx->D), uz(ctx->D);
+    for (int d = 0; d < ctx->D; ++d) {
+        double z, phi;
+        healpix_pix2ang(order, d, z, phi);
+        const double st = std::sqrt(std::max(0.0, 1.0 - z * z));
+        ux[d] = st * std::cos(phi);
+        uy[d] = st * std::sin(phi);
+        uz[d] = z;
+    }
+    std::vector<int> nbr((size_t)ctx->D * 8, -1);
+    for (int d = 0; d < ctx->D; ++d) {
+        std::vector<std::pair<double, int>> c;
+        for (int e = 0; e < ctx->D; ++e)
+            if (e != d) c.push_back({-(ux[d] * ux[e] + uy[d] * uy[e] + uz[d] * uz[e]), e});
+        std::partial_sort(c.begin(), c.begin() + std::min<size_t>(8, c.size()), c.end());
+        for (int q = 0; q < 8 && q < (int)c.size(); ++q) nbr[(size_t)d * 8 + q] = c[q].second;
+    }
+    HIPCHK(P.nbr.upload(nbr.data(), nbr.size(), ctx->stream));
+    // parameters: W ~ N(0, 1 / fan_in) (the output layer at a tenth, for near-uniform first proposals), b = 0
+    const int rows[4] = {P.H, P.H, P.H, P.O}, cols[4] = {P.Fin_pad, P.H, P.H, P.H}, fan[4] = {P.Fin, P.H, P.H, P.H};
+    P.rng = cfg->seed ^ 0x5DEECE66DULL;
+    for (int l = 0; l < 4; ++l) {
+        P.rows[l] = rows[l];
+        P.cols[l] = cols[l];
+        const size_t nw = (size_t)rows[l] * cols[l];
+        std::vector<float> h(nw, 0.f);
+        const double sd = (l == 3 ? 0.1 : 1.0) / std::sqrt((double)fan[l]);
+        for (int r = 0; r < rows[l]; ++r)
+            for (int q = 0; q < fan[l]; ++q) h[(size_t)r * cols[l] + q] = (float)(sd * gauss(P.rng));
+        HIPCHK(P.w[2 * l].upload(h.data(), nw, ctx->stream));
+        std::vector<float> b(rows[l], 0.f);
+        HIPCHK(P.w[2 * l + 1].upload(b.data(), b.size(), ctx->stream));
+        for (int q = 2 * l; q <= 2 * l + 1; ++q) {
+            const size_t nq = (q & 1) ? (size_t)rows[l] : nw;
+            HIPCHK(P.m[q].reserve(nq));
+            HIPCHK(P.v[q].reserve(nq));
+            HIPCHK(P.g[q].reserve(nq));
+            HIPCHK(hipMemsetAsync(P.m[q].p, 0, nq * sizeof(float), ctx->stream));
+            HIPCHK(hipMemsetAsync(P.v[q].p, 0, nq * sizeof(float), ctx->stream));
+        }
+        HIPCHK(P.wq[l].reserve(nw));
+        if (l > 0) {
+            const size_t nt = (size_t)cols[l] * (l == 3 ? P.O_pad : rows[l]);
+            HIPCHK(P.wqT[l].reserve(nt));
+            HIPCHK(hipMemsetAsync(P.wqT[l].p, 0, nt * 2, ctx->stream));
+        }
+    }
+    P.adam_t = 0;
+    const size_t mt = (size_t)std::max(cfg->max_train, 1), tc = (size_t)cfg->target_cells;
+    HIPCHK(P.tx.reserve(mt * P.Fin_pad));
+    HIPCHK(P.tcell.reserve(mt * tc));
+    HIPCHK(P.tprob.reserve(mt * tc));
+    HIPCHK(P.tcnt.reserve(mt));
+    HIPCHK(P.tshift.reserve(2 * mt));
+    HIPCHK(P.loss.reserve(1));
+    P.ntrain = 0;
+    int r = prop_refresh(ctx);
+    if (r) return r;
+    P.audit_pos = 0;
+    P.st_pools = P.st_images = P.st_all = P.st_audited = P.st_fallback = 0;
+    P.st_cand = 0.;
+    P.st_min_mass = 1.;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    P.ready = true;
+    return ROME_OK;
+}
+
+int rome_proposal_mode(rome_ctx* ctx, int mode) {
+    if (!ctx || mode < ROME_PROPOSAL_OFF || mode > ROME_PROPOSAL_USE) return ROME_ERR_ARG;
+    if (mode != ROME_PROPOSAL_OFF && !ctx->prop.ready)
+        return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    if (mode == ROME_PROPOSAL_USE && ctx->cfg.precision == ROME_PREC_FP32)
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "the learned proposal needs precision BF16 or FP16");
+    ctx->prop.mode = mode;
+    return ROME_OK;
+}
+
+int rome_proposal_train(rome_ctx* ctx, int epochs, int batch, double lr, double* loss_per_epoch) {
+    if (!ctx || epochs < 0 || batch <= 0 || batch % 32 || !(lr > 0)) return ROME_ERR_ARG;
+    ProposalState& P = ctx->prop;
+    if (!P.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    if (P.ntrain < batch) return fail(ctx, ROME_ERR_ARG, "training set (%d images) smaller than a batch", P.ntrain);
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    int rc = prop_acts(ctx, batch);
+    if (rc) return rc;
+    HIPCHK(P.bidx.reserve(P.ntrain));
+    std::vector<int> perm(P.ntrain);
+    const int nb = P.ntrain / batch;
+    for (int e = 0; e < epochs; ++e) {
+        for (int i = 0; i < P.ntrain; ++i) perm[i] = i;
+        for (int i = P.ntrain - 1; i > 0; --i) std::swap(perm[i], perm[(int)(splitmix(P.rng) % (unsigned long long)(i + 1))]);
+        HIPCHK(hipMemcpyAsync(P.bidx.p, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(P.loss.p, 0, sizeof(float), s));
+        for (int b = 0; b < nb; ++b) {
+            const int* idx = P.bidx.p + (size_t)b * batch;
+            HIPCHK(mlp_gather_rows(s, P.tx.p, idx, batch, P.Fin_pad, P.x.p));
+            HIPCHK(mlp_gather_targets(s, idx, batch, P.cfg.target_cells, P.tcell.p, P.tprob.p, P.tcnt.p, P.tshift.p,
+                                      P.bcell.p, P.bprob.p, P.bcnt.p, P.bshift.p));
+            if ((rc = prop_forward(ctx, batch)) || (rc = prop_backward(ctx, batch)) || (rc = prop_adam(ctx, lr))) return rc;
+        }
+        float l = 0.f;
+        HIPCHK(hipMemcpyAsync(&l, P.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));  // also keeps the host permutation alive until its copy is done
+        if (loss_per_epoch) loss_per_epoch[e] = nb ? l / nb : 0.0;
+    }
+    return ROME_OK;
+}
+
+int rome_proposal_stats(rome_ctx* ctx, double* out) {
+    if (!ctx || !out) return ROME_ERR_ARG;
+    const ProposalState& P = ctx->prop;
+    out[0] = (double)P.st_pools;
+    out[1] = (double)P.st_images;
+    out[2] = P.st_images ? P.st_cand / (double)P.st_images : 0.;
+    out[3] = (double)P.st_all;
+    out[4] = (double)P.st_audited;
+    out[5] = P.st_min_mass;
+    out[6] = (double)P.st_fallback;
+    out[7] = (double)P.ntrain;
+    return ROME_OK;
+}
+
+int rome_proposal_get_params(rome_ctx* ctx, rome_proposal_params* o) {
+    if (!ctx || !o) return ROME_ERR_ARG;
+    ProposalState& P = ctx->prop;
+    if (!P.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    float* dst[8] = {o->W1, o->b1, o->W2, o->b2, o->W3, o->b3, o->W4, o->b4};
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < 8; ++q) {
+        if (!dst[q]) continue;
+        const int l = q / 2;
+        if (q & 1) {
+            HIPCHK(hipMemcpy(dst[q], P.w[q].p, (size_t)P.rows[l] * sizeof(float), hipMemcpyDeviceToHost));
+        } else {
+            const int c = l == 0 ? P.Fin : P.cols[l];
+            HIPCHK(hipMemcpy2D(dst[q], c * sizeof(float), P.w[q].p, P.cols[l] * sizeof(float), c * sizeof(float),
+                               P.rows[l], hipMemcpyDeviceToHost));
+        }
+    }
+    return ROME_OK;
+}
+
+int rome_proposal_set_params(rome_ctx* ctx, const rome_proposal_params* in) {
+    if (!ctx || !in) return ROME_ERR_ARG;
+    ProposalState& P = ctx->prop;
+    if (!P.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    const float* src[8] = {in->W1, in->b1, in->W2, in->b2, in->W3, in->b3, in->W4, in->b4};
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < 8; ++q) {
+        if (!src[q]) return fail(ctx, ROME_ERR_ARG, "parameter %d missing", q);
+        const int l = q / 2;
+        if (q & 1) {
+            HIPCHK(hipMemcpy(P.w[q].p, src[q], (size_t)P.rows[l] * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            const int c = l == 0 ? P.Fin : P.cols[l];
+            if (l == 0) HIPCHK(hipMemset(P.w[q].p, 0, (size_t)P.rows[l] * P.cols[l] * sizeof(float)));
+            HIPCHK(hipMemcpy2D(P.w[q].p, P.cols[l] * sizeof(float), src[q], c * sizeof(float), c * sizeof(float),
+                               P.rows[l], hipMemcpyHostToDevice));
+        }
+    }
+    int r = prop_refresh(ctx);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ROME_OK;
+}
+
+namespace {
+// host features [n][Fin] -> P.x (BF16, zero-padded rows)
+int prop_upload_x(rome_ctx* ctx, int n, const float* x) {
+    ProposalState& P = ctx->prop;
+    int r = prop_acts(ctx, n);
+    if (r) return r;
+    std::vector<float> h((size_t)n * P.Fin_pad, 0.f);
+    for (int i = 0; i < n; ++i) std::memcpy(&h[(size_t)i * P.Fin_pad], x + (size_t)i * P.Fin, P.Fin * sizeof(float));
+    DevBuf<float> tmp;
+    HIPCHK(tmp.upload(h.data(), h.size(), ctx->stream));
+    HIPCHK(mlp_to_bf16(ctx->stream, tmp.p, P.x.p, h.size()));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ROME_OK;
+}
+}  // namespace
+
+int rome_proposal_forward(rome_ctx* ctx, int n, const float* x, float* z) {
+    if (!ctx || n <= 0 || !x || !z) return ROME_ERR_ARG;
+    ProposalState& P = ctx->prop;
+    if (!P.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    int r = prop_upload_x(ctx, n, x);
+    if (r || (r = prop_forward(ctx, n))) return r;
+    HIPCHK(hipMemcpy2DAsync(z, P.O * sizeof(float), P.z.p, P.O_pad * sizeof(float), P.O * sizeof(float), n,
+                            hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ROME_OK;
+}
+
+int rome_proposal_gradients(rome_ctx* ctx, int n, const float* x, const float* target, const float* shift,
+                            double* loss, rome_proposal_params* grads) {
+    if (!ctx || n <= 0 || n % 32 || !x || !target || !shift) return ROME_ERR_ARG;
+    ProposalState& P = ctx->prop;
+    if (!P.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int tc = P.cfg.target_cells;
+    std::vector<int> cell((size_t)n * tc, 0), cnt(n, 0);
+    std::vector<float> prob((size_t)n * tc, 0.f);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < P.ncell; ++c) {
+            const float p = target[(size_t)i * P.ncell + c];
+            if (p == 0.f) continue;
+            if (cnt[i] == tc) return fail(ctx, ROME_ERR_ARG, "target row %d has more than %d cells", i, tc);
+            cell[(size_t)i * tc + cnt[i]] = c;
+            prob[(size_t)i * tc + cnt[i]++] = p;
+        }
+    int r = prop_upload_x(ctx, n, x);
+    if (r) return r;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(P.bcell.p, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P.bprob.p, prob.data(), prob.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P.bcnt.p, cnt.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P.bshift.p, shift, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(P.loss.p, 0, sizeof(float), s));
+    if ((r = prop_forward(ctx, n)) || (r = prop_backward(ctx, n))) return r;
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(&l, P.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (loss) *loss = l;
+    if (grads) {
+        float* dst[8] = {grads->W1, grads->b1, grads->W2, grads->b2, grads->W3, grads->b3, grads->W4, grads->b4};
+        for (int q = 0; q < 8; ++q) {
+            if (!dst[q]) continue;
+            const int lq = q / 2;
+            if (q & 1) {
+                HIPCHK(hipMemcpy(dst[q], P.g[q].p, (size_t)P.rows[lq] * sizeof(float), hipMemcpyDeviceToHost));
+            } else {
+                const int c = lq == 0 ? P.Fin : P.cols[lq];
+                HIPCHK(hipMemcpy2D(dst[q], c * sizeof(float), P.g[q].p, P.cols[lq] * sizeof(float), c * sizeof(float),
+                                   P.rows[lq], hipMemcpyDeviceToHost));
+            }
+        }
+    }
+    return ROME_OK;
+}
+
+int rome_proposal_training_rows(rome_ctx* ctx, int row0, int n, float* x, float* target, float* shift) {
+    if (!ctx || n < 0 || row0 < 0) return ROME_ERR_ARG;
+    ProposalState& P = ctx->prop;
+    if (!P.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    if (row0 + n > P.ntrain) return fail(ctx, ROME_ERR_ARG, "rows past the training set (%d)", P.ntrain);
+    if (n == 0) return ROME_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int tc = P.cfg.target_cells;
+    std::vector<uint16_t> hx((size_t)n * P.Fin_pad);
+    std::vector<int> cell((size_t)n * tc), cnt(n);
+    std::vector<float> prob((size_t)n * tc), sh(2 * (size_t)n);
+    HIPCHK(hipMemcpy(hx.data(), P.tx.p + (size_t)row0 * P.Fin_pad, hx.size() * 2, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cell.data(), P.tcell.p + (size_t)row0 * tc, cell.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(prob.data(), P.tprob.p + (size_t)row0 * tc, prob.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), P.tcnt.p + row0, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sh.data(), P.tshift.p + 2 * (size_t)row0, sh.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        if (x)
+            for (int q = 0; q < P.Fin; ++q) {
+                const uint32_t b = (uint32_t)hx[(size_t)i * P.Fin_pad + q] << 16;
+                float f;
+                std::memcpy(&f, &b, 4);
+                x[(size_t)i * P.Fin + q] = f;
+            }
+        if (target) {
+            std::fill(target + (size_t)i * P.ncell, target + (size_t)(i + 1) * P.ncell, 0.f);
+            for (int q = 0; q < cnt[i]; ++q) target[(size_t)i * P.ncell + cell[(size_t)i * tc + q]] += prob[(size_t)i * tc + q];
+        }
+        if (shift) {
+            shift[2 * i] = sh[2 * i];
+            shift[2 * i + 1] = sh[2 * i + 1];
+        }
+    }
+    return ROME_OK;
+}
+
+// ------------------------------------------------------------------ offline reconstruction (backprojection)
+int rome_backproject_images(rome_ctx* ctx, int iclass, int n, const double* fre, const double* fim, const double* fctf,
+                            const double* euler, const double* shift) {
+    if (!ctx || n < 0 || (n > 0 && (!fre || !fim || !euler || !shift))) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
+    if (n == 0) return ROME_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int F2 = ctx->F2f, Pf = ctx->Pf;
+    const int chunk = 2048;
+    DevBuf<float2> X, S;
+    DevBuf<float> C;
+    DevBuf<double> Ai, sx, sy;
+    std::vector<float2> hx((size_t)std::min(n, chunk) * F2);
+    std::vector<float> hc((size_t)std::min(n, chunk) * F2);
+    std::vector<double> ha(6 * (size_t)std::min(n, chunk)), hsx(std::min(n, chunk)), hsy(std::min(n, chunk));
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int m = std::min(chunk, n - i0);
+        for (size_t q = 0; q < (size_t)m * F2; ++q) {
+            hx[q] = make_float2((float)fre[(size_t)i0 * F2 + q], (float)fim[(size_t)i0 * F2 + q]);
+            hc[q] = fctf && ctx->cfg.do_ctf_correction ? (float)fctf[(size_t)i0 * F2 + q] : 1.f;
+        }
+        for (int i = 0; i < m; ++i) {
+            double A[9];
+            const double* e = euler + 3 * (size_t)(i0 + i);
+            if (ctx->is3d) euler_matrix(e[0], e[1], e[2], A);
+            else euler_matrix(0, 0, e[2], A);
+            rot6(A, &ha[6 * (size_t)i]);
+            hsx[i] = shift[2 * (size_t)(i0 + i)];
+            hsy[i] = shift[2 * (size_t)(i0 + i) + 1];
+        }
+        HIPCHK(X.upload(hx.data(), (size_t)m * F2, s));
+        HIPCHK(C.upload(hc.data(), (size_t)m * F2, s));
+        HIPCHK(Ai.upload(ha.data(), 6 * (size_t)m, s));
+        HIPCHK(sx.upload(hsx.data(), m, s));
+        HIPCHK(sy.upload(hsy.data(), m, s));
+        HIPCHK(S.reserve((size_t)m * Pf));
+        HIPCHK(launch_phase_table(s, sx.p, sy.p, m, ctx->f_xy.p, Pf, ctx->cfg.ori_size, ctx->sintab.p, ctx->costab.p,
+                                  S.p));
+        BpImageArgs ba{X.p,     C.p,       S.p,     Ai.p,       F2,        Pf,      ctx->f_nfine.p,
+                       ctx->f_xy.p, ctx->is3d, ctx->pad, ctx->padF, ctx->cgrid, ctx->r_max,
+                       ctx->bp_data.p + 2 * ctx->grid_n * iclass, ctx->bp_weight.p + ctx->grid_n * iclass};
+        HIPCHK(launch_bp_images(s, m, ba));
+        HIPCHK(hipStreamSynchronize(s));  // the host staging is reused by the next chunk
     }
     return ROME_OK;
 }

@@ -434,6 +434,20 @@ struct MetaArgs {
     int* argmax;               // [n][6]
 };
 hipError_t launch_meta(hipStream_t s, const MetaArgs& a);
+// Whole images scattered into one class grid at their own poses (the offline reconstruction).
+struct BpImageArgs {
+    const float2* X;      // [n][F2] transforms (current_size half-plane)
+    const float* ctf;     // [n][F2]
+    const float2* phase;  // [n][Pf] tabulated shift phase of each image's offset
+    const double* Ai;     // [n][6] 2 A^T restricted to (x, y)
+    int F2, Pf;
+    const int* f_nfine;
+    const int* f_xy;
+    int is3d, pad, padF, c, r_max;
+    float* bp_data;       // the class's grid [pad][pad][padF] float2 (interleaved)
+    float* bp_weight;
+};
+hipError_t launch_bp_images(hipStream_t s, int n, const BpImageArgs& a);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
 hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a);
 // M-step reconstruction (reconstruct.hip)

@@ -1118,6 +1118,56 @@ __global__ void k_meta(MetaArgs a) {
     md[6] = a.offy[i] + a.sy[(size_t)t * a.NOT + ot];
 }
 
+// ------------------------------------------------------------------ backprojection of whole images
+// MyBackProjector::backproject of one image per block at its own pose (apps/rome_reconstruct.cpp:104-330):
+// the transform X (current_size half-plane) shifted by the image's tabulated phase, times the CTF, and the
+// weight CTF^2, scattered trilinearly (bilinearly in 2D) into one class grid; the pixel set is the fine
+// one (x = 0 column for y >= 0 only, as the reference's first_x rule), radius r_max.
+__global__ __launch_bounds__(256) void k_bp_images(BpImageArgs a) {
+    const int img = blockIdx.x;
+    const float2* X = a.X + (size_t)img * a.F2;
+    const float* ctf = a.ctf + (size_t)img * a.F2;
+    const float2* S = a.phase + (size_t)img * a.Pf;
+    const double* A6 = a.Ai + 6 * (size_t)img;
+    float* gd = a.bp_data;
+    float* gw = a.bp_weight;
+    for (int p = threadIdx.x; p < a.Pf; p += blockDim.x) {
+        const int xy = a.f_xy[p];
+        const int x = px_x(xy), y = px_y(xy);
+        if (x * x + y * y > a.r_max * a.r_max) continue;
+        const int nf = a.f_nfine[p];
+        const float cv = ctf[nf];
+        const float wgt = cv * cv;
+        if (!(wgt > 0.f)) continue;
+        const float2 v = cmul(X[nf], S[p]);
+        const Coords c = map_coords(A6, x, y, a.c, a.is3d != 0);
+        const float vr = v.x * cv, vi = (c.neg ? -v.y : v.y) * cv;
+        const float mfx = 1.f - c.fx, mfy = 1.f - c.fy, mfz = 1.f - c.fz;
+        if (a.is3d) {
+            const float dd[8] = {mfz * mfy * mfx, mfz * mfy * c.fx, mfz * c.fy * mfx, mfz * c.fy * c.fx,
+                                 c.fz * mfy * mfx, c.fz * mfy * c.fx, c.fz * c.fy * mfx, c.fz * c.fy * c.fx};
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) {
+                const int dz = cc >> 2, dy = (cc >> 1) & 1, dx = cc & 1;
+                const size_t idx = ((size_t)(c.z0 + dz) * a.pad + (c.y0 + dy)) * a.padF + (c.x0 + dx);
+                atomicAdd(&gd[2 * idx], dd[cc] * vr);
+                atomicAdd(&gd[2 * idx + 1], dd[cc] * vi);
+                atomicAdd(&gw[idx], dd[cc] * wgt);
+            }
+        } else {
+            const float dd[4] = {mfy * mfx, mfy * c.fx, c.fy * mfx, c.fy * c.fx};
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                const int dy = cc >> 1, dx = cc & 1;
+                const size_t idx = (size_t)(c.y0 + dy) * a.padF + (c.x0 + dx);
+                atomicAdd(&gd[2 * idx], dd[cc] * vr);
+                atomicAdd(&gd[2 * idx + 1], dd[cc] * vi);
+                atomicAdd(&gw[idx], dd[cc] * wgt);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ launch wrappers
 #define LAUNCH_CHECK()                                   \
     do {                                                 \
@@ -1321,6 +1371,13 @@ hipError_t launch_meta(hipStream_t s, const MetaArgs& a) {
 
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a) {
     k_store_sums<<<(a.n + 127) / 128, 128, 0, s>>>(a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_bp_images(hipStream_t s, int n, const BpImageArgs& a) {
+    if (n <= 0) return hipSuccess;
+    k_bp_images<<<n, 256, 0, s>>>(a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

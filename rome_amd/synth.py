@@ -239,6 +239,7 @@ def make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, psi_ste
     # particles at ori size, then windowed
     imgs = np.zeros((n, F2o), np.complex128)
     ctfs = np.zeros((n, F2o))
+    ctf_params = np.zeros((n, 7))  # defocus U, V (A), angle (deg), voltage (kV), Cs (mm), Q0, B-factor
     truth = []
     sig_power = []
     for i in range(n):
@@ -259,7 +260,9 @@ def make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, psi_ste
                 full[idx[(a, b)]] = proj[q]
             proj = full
         if do_ctf:
-            ctf = ctf_image(ori, ori, angpix, rng.uniform(1e4, 3e4), rng.uniform(1e4, 3e4), rng.uniform(0, 180))
+            dU, dV, da = rng.uniform(1e4, 3e4), rng.uniform(1e4, 3e4), rng.uniform(0, 180)
+            ctf = ctf_image(ori, ori, angpix, dU, dV, da)
+            ctf_params[i] = (dU, dV, da, 300.0, 2.7, 0.1, 0.0)
         else:
             ctf = np.ones(F2o)
         xo, yo = halfplane_xy(ori)
@@ -301,7 +304,8 @@ def make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, psi_ste
         shift_fine_x=sfx, shift_fine_y=sfy, trans_x=tx, trans_y=ty)
     batch = dict(n=n, fmask_re=fimg.real.copy(), fmask_im=fimg.imag.copy(), fnomask_re=fimg.real.copy(),
                  fnomask_im=fimg.imag.copy(), fctf=fctf, highres_Xi2=hxi2, power_img=power,
-                 old_offx=np.zeros(n), old_offy=np.zeros(n), group=groups, norm=np.ones(n))
+                 old_offx=np.zeros(n), old_offy=np.zeros(n), group=groups, norm=np.ones(n),
+                 ctf_params=ctf_params, angpix=angpix)
     return prob, batch, truth
 
 
@@ -406,3 +410,42 @@ def load_engine(engine, prob, precision="fp32"):
     engine.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
                      prob["prior_offy"])
     engine.reset_sums()
+
+
+def real_space_images(F, ori):
+    """Real-space images [n][ori][ori] whose centred, 1/ori^2-normalised transform is the half-plane F
+    [n][ori * (ori/2 + 1)] (the convention of ParticleModel::setFourierTransforms: CenterFFT, then the r2c
+    transform divided by ori^2); the x = 0 column takes its Hermitian-symmetric part."""
+    F = np.asarray(F).reshape(-1, ori, ori // 2 + 1)
+    img = np.fft.irfft2(F * float(ori * ori), s=(ori, ori))
+    s_ = -(ori // 2)
+    return np.roll(np.roll(img, s_, axis=2), s_, axis=1)
+
+
+def write_dataset(directory, prefix, batch, truth, ori, angpix=None, stack_size=10000):
+    """A synthetic pool as the reference reads it: prefix.star (image names, micrograph, group, CTF
+    parameters, the true angles, origins and class) and prefix_NNN.mrcs stacks of real-space float32
+    images.  batch / truth as make_problem returns them (current_size = ori_size)."""
+    import os
+    from . import starfile
+    os.makedirs(directory, exist_ok=True)
+    n = int(batch["n"])
+    apx = float(angpix if angpix is not None else batch.get("angpix", 1.0))
+    F = np.asarray(batch["fnomask_re"]) + 1j * np.asarray(batch["fnomask_im"])
+    names = []
+    for s0 in range(0, n, stack_size):
+        s1 = min(n, s0 + stack_size)
+        stack = f"{prefix}_{s0 // stack_size:03d}.mrcs"
+        starfile.write_mrcs(os.path.join(directory, stack), real_space_images(F[s0:s1], ori), apx)
+        names += [starfile.image_name(i - s0 + 1, stack) for i in range(s0, s1)]
+    tr = np.asarray(truth, dtype=float)
+    cp = np.asarray(batch.get("ctf_params", np.zeros((n, 7))))
+    cols = {"rlnImageName": names, "rlnMicrographName": ["mic1"] * n,
+            "rlnGroupNumber": [int(g) + 1 for g in batch["group"]],
+            "rlnDefocusU": cp[:, 0], "rlnDefocusV": cp[:, 1], "rlnDefocusAngle": cp[:, 2], "rlnVoltage": cp[:, 3],
+            "rlnSphericalAberration": cp[:, 4], "rlnAmplitudeContrast": cp[:, 5], "rlnBfactor": cp[:, 6],
+            "rlnAngleRot": tr[:, 1], "rlnAngleTilt": tr[:, 2], "rlnAnglePsi": tr[:, 3],
+            "rlnOriginX": tr[:, 4], "rlnOriginY": tr[:, 5], "rlnClassNumber": [int(k) + 1 for k in tr[:, 0]]}
+    path = os.path.join(directory, prefix + ".star")
+    starfile.write_star(path, cols)
+    return path
