@@ -1035,14 +1035,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
         rc = significance(nullptr, dense, true);
         if (rc) return rc;
-        if (!dense) {
-            const int mx = *std::max_element(scount.begin(), scount.end());
-            if (mx > ctx->sig_cap) {
-                ctx->sig_dense_left = 8;
-                rc = significance(nullptr, true, true);
-                if (rc) return rc;
-            }
-        }
+        // The significant samples are drawn from the marked ones, whose keys fit the cap when this pass
+        // is sparse; a rerun here could not read D2c anyway (the task build above replaced its block lists).
+        if (!dense && *std::max_element(scount.begin(), scount.end()) > ctx->sig_cap)
+            return fail(ctx, ROME_ERR_NUMERIC, "recomputed significant set exceeds the marked set's key cap");
     }
     for (int i = 0; i < n; ++i)
         if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
