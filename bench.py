@@ -179,6 +179,112 @@ def pose_differences(meta_a, meta_b, amax_a, amax_b):
             "pmax_mixed_median": round(float(np.median(r[:, 4])), 4)}
 
 
+def adaptive_leg(args, log):
+    """The batched adaptive fine pass (one GPU's share of it): 192^2 particles, K=8 blob volumes 192^3, coarse
+    HEALPix order 3 x 48 psi (36864 rotations per class) x 49 translations, oversampling 1; per particle a
+    lognormal number of significant coarse samples (median 300, clipped to 1-5000) around its true pose.
+    Each pool is copied into one of two staging slots (the way a caller hands over consecutive pools) and
+    its fine pass runs from that significance (rome_expectation_fine_device) as a replayed hipGraph of the
+    device-driven pool, the fine diff2 on its work queue, the persistent grids autotuned on the first pool."""
+    import torch
+    import rome_amd
+    from rome_amd import synth
+    ori, K, order, n_img, B = 192, 8, 3, args.adaptive_n, args.adaptive_batch
+    rng = np.random.default_rng(args.seed + 300)
+    t0 = time.time()
+    e = rome_amd.Engine(0)
+    pose_c = rome_amd.pose_table3d(order, 0, args.perturbation)
+    pose_f = rome_amd.pose_table3d(order, 1, args.perturbation)
+    tx, ty = rome_amd.translations(4.0, 1.0)
+    scx, scy = rome_amd.shift_table(4.0, 1.0, 0, args.perturbation)
+    sfx, sfy = rome_amd.shift_table(4.0, 1.0, 1, args.perturbation)
+    D, PSI, T = pose_c.shape[0], pose_c.shape[1], len(tx)
+    prob = dict(is_3d=1, ori_size=ori, current_size=ori, coarse_size=48, nr_classes=K, nr_groups=1,
+                adaptive_oversampling=1, nr_dir=D, nr_psi=PSI, nr_trans=T, adaptive_fraction=0.999,
+                sigma2_fudge=1.0, sigma2_offset=4.0, avg_norm_correction=1.0, do_ctf_correction=1,
+                refs_are_ctf_corrected=1, do_norm_correction=1, do_map=1,
+                pdf_class=np.full(K, 1.0 / K), pdf_direction=np.full((K, D), 1.0 / D), prior_offx=np.zeros(K),
+                prior_offy=np.zeros(K), pose_coarse=pose_c, pose_fine=pose_f,
+                shift_coarse_x=scx[:, 0], shift_coarse_y=scy[:, 0], shift_fine_x=sfx, shift_fine_y=sfy,
+                trans_x=tx, trans_y=ty)
+    e.configure(**synth.engine_config(prob))
+    e.set_sampling(pose_c, pose_f, scx[:, 0], scy[:, 0], sfx, sfy, tx, ty)
+    for k in range(K):  # references: blob maps through the engine's own Fourier map (installed on the device)
+        vol = synth.blob_volume_fast(ori, 300, (3.0, 9.0), 75.0, rng)
+        e.fourier_map(k, vol, do_gridding=True, want_grid=False)
+    F, ctf, truth, sigma2 = synth.make_particles_device(e, K, n_img, ori, ori, 1.5, 4.0, 0.05, args.seed + 301)
+    prob["sigma2_noise"] = np.full((1, ori // 2 + 1), sigma2)
+    e.set_model(prob["sigma2_noise"], prob["pdf_class"], prob["pdf_direction"], prob["prior_offx"],
+                prob["prior_offy"])
+    counts = synth.lognormal_counts(n_img, 300, 1.0, 1, 5000, rng)
+    cap = int(counts.max())
+    keys, cnt = synth.clustered_significance_device(prob, truth, counts, cap)
+    t_gen = time.time() - t0
+    dev = F.device
+    F2, oriF = F.shape[1], ori // 2 + 1
+    zeros = torch.zeros(B, dtype=torch.float64, device=dev)
+    power = torch.zeros((B, oriF), dtype=torch.float64, device=dev)
+    one = torch.ones(B, dtype=torch.float64, device=dev)
+    grp = torch.zeros(B, dtype=torch.int32, device=dev)
+    slots = [dict(F=torch.empty((B, F2), dtype=F.dtype, device=dev), ctf=torch.empty((B, F2), dtype=ctf.dtype, device=dev),
+                  keys=torch.empty((B, cap), dtype=torch.int32, device=dev),
+                  cnt=torch.empty(B, dtype=torch.int32, device=dev)) for _ in range(2)]
+    nb = n_img // B
+
+    def pool(i, results=False):
+        sl = slots[i % 2]
+        o = (i % nb) * B
+        sl["F"].copy_(F[o:o + B])
+        sl["ctf"].copy_(ctf[o:o + B])
+        sl["keys"].copy_(keys[o:o + B])
+        sl["cnt"].copy_(cnt[o:o + B])
+        return e.expectation_fine_device(B, sl["F"].data_ptr(), sl["F"].data_ptr(), sl["ctf"].data_ptr(),
+                                         zeros.data_ptr(), power.data_ptr(), zeros.data_ptr(), zeros.data_ptr(),
+                                         one.data_ptr(), grp.data_ptr(), sl["keys"].data_ptr(),
+                                         sl["cnt"].data_ptr(), cap, results=results)
+    # the first pool, untimed: persistent grids autotuned on it, then one graph per staging slot
+    torch.cuda.synchronize()
+    sl0 = slots[0]
+    pool(0)
+    grids, tune_ms = e.autotune_device(B, sl0["F"].data_ptr(), sl0["F"].data_ptr(), sl0["ctf"].data_ptr(),
+                                       zeros.data_ptr(), power.data_ptr(), zeros.data_ptr(), zeros.data_ptr(),
+                                       one.data_ptr(), grp.data_ptr(), sl0["keys"].data_ptr(), sl0["cnt"].data_ptr(),
+                                       cap)
+    for i in range(2):
+        pool(i)
+        pool(i)
+    e.reset_sums()
+    log(f"[adaptive] data {t_gen:.1f}s, grids {grids}", e.execution_stats())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    last = []
+    for i in range(nb):
+        pool(i)
+        last.append(e.execution_stats()["last"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = e.batch_stats()
+    # stage times of the same pools, device-driven with stage timers (outside the timed pass)
+    e.set_profiling(True)
+    for i in range(min(nb, 4)):
+        pool(i)
+    torch.cuda.synchronize()
+    prof = e.profile()
+    e.set_profiling(False)
+    out = {"workload": "batched adaptive fine pass: 192^2, K=8, hp3 x 48 psi (36864) x 49 shifts, oversampling 1, "
+                       "significant samples per particle lognormal (median 300, 1-5000) around the true pose",
+           "n_images": nb * B, "batch_images": B, "value": round(nb * B / dt, 2), "unit": "images/s",
+           "ms_per_pool": round(dt / nb * 1e3, 2), "significant_per_image_mean": round(float(np.mean(counts)), 1),
+           "persistent_grids": grids, "autotune_stage_ms": np.round(tune_ms, 3).tolist(),
+           "execution": {"last_per_pool": sorted(set(last)), **e.execution_stats()},
+           "stages_ms_per_pool": {k: round(v[0] / max(v[1], 1), 3) for k, v in prof.items() if v[1]},
+           "data_s": round(t_gen, 1), "last_pool": st}
+    e.close()
+    del F, ctf, keys, cnt, slots
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +308,11 @@ def main():
                     help="particles (held apart from the timed ones) whose exhaustive E-step posteriors train the "
                          "proposal network")
     ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--execution", choices=["host", "device", "graph"], default="graph",
+                    help="pool execution: host-synchronised, device-driven, or device-driven captured as hipGraphs")
+    ap.add_argument("--adaptive-n", type=int, default=0,
+                    help="particles of the batched adaptive fine-pass leg (192^2, K=8, hp3; 0: not run)")
+    ap.add_argument("--adaptive-batch", type=int, default=1000)
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -287,23 +398,36 @@ def main():
 
     def run_arm(prec, learned=False):
         synth.load_engine(eng, prob, precision=prec)
+        eng.set_execution(args.execution)
         if learned:
             eng.proposal_mode("use")
         for i in range(args.warmup):
             step(i)
-            log(f"[{prec}] warmup {i} done", eng.batch_stats(), eng.coarse_stats())
-        eng.set_profiling(True)
+            log(f"[{prec}] warmup {i} done", eng.batch_stats(), eng.coarse_stats(), eng.execution_stats())
+        if args.execution == "graph":
+            # graph preparation (untimed): every batch the timed steps use is captured once
+            for i in range(args.warmup, args.warmup + min(args.steps, nb)):
+                step(i)
+            log(f"[{prec}] graphs", eng.execution_stats())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = []
         full_batches = 0  # batches whose coarse list overflowed (the pool then ran on the full pose term)
+        execs = []
         for i in range(args.steps):
             res.append(step(args.warmup + i))
             full_batches += eng.coarse_stats()["mode"] != 0
+            execs.append(eng.execution_stats()["last"])
             if args.verbose:
-                log(f"[{prec}] step {i} done", eng.batch_stats(), eng.coarse_stats())
+                log(f"[{prec}] step {i} done", eng.batch_stats(), eng.coarse_stats(), execs[-1])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        # stage times (HIP events on the engine stream) from a second, profiled pass over the same batches:
+        # stage timers run the pools device-driven without the graph, so they stay out of the timed region
+        eng.set_profiling(True)
+        for i in range(args.steps):
+            step(args.warmup + i)
+        torch.cuda.synchronize()
         # checks of the results, outside the timed region
         agree, amax, meta = [], {}, {}
         for i, r in enumerate(res):
@@ -320,7 +444,8 @@ def main():
             eng.proposal_mode("off")
         return dict(value=args.steps * B / dt, dt=dt, prof=prof, stats={**eng.batch_stats(), **eng.coarse_stats()},
                     amax=amax, meta=meta, truth_agree=float(np.mean(agree)), full_batches=full_batches,
-                    proposal=pstats)
+                    proposal=pstats, execution={"mode": args.execution, "last_per_step": execs,
+                                                **eng.execution_stats()})
 
     arms = {} if (args.only_mixed and args.precision != "fp32") else {"fp32": run_arm("fp32")}
     if args.precision != "fp32":
@@ -409,6 +534,12 @@ def main():
             finally:
                 if k == "learned":
                     eng.proposal_mode("off")
+    adaptive = None
+    if args.adaptive_n > 0:
+        eng.close()  # the adaptive leg's context holds its own capacities
+        del F, ctf
+        torch.cuda.empty_cache()
+        adaptive = adaptive_leg(args, log)
     dtype = {"fp32": "fp32", "bf16": "bf16+fp32", "fp16": "fp16+fp32"}[head_prec]
     out = {
         "metric": "E-step images/sec (3D 128^2, K=4) at >=99.5% pose agreement; mixed vs FP32",
@@ -423,7 +554,7 @@ def main():
         "arms": {k: {"value": round(v["value"], 2), "ms_per_step": round(v["dt"] / args.steps * 1e3, 2),
                      "class_agreement_vs_truth": round(v["truth_agree"], 4),
                      "pose_agreement_vs_fp32": round(agreement[k], 5) if k in agreement else None,
-                     "proposal": v["proposal"],
+                     "proposal": v["proposal"], "execution": v["execution"],
                      "stages_ms_per_step": {s_: round(x[0] / max(x[1], 1), 3) for s_, x in v["prof"].items()}}
                  for k, v in arms.items()},
         "pose_agreement_mixed_vs_fp32": None if pose_agree is None else round(pose_agree, 5),
@@ -432,6 +563,7 @@ def main():
         "parity_vs_fp64_oracle": parity,
         "proposal_training": training,
         "batch_stats": stats,
+        "adaptive_fine_pass": adaptive,
     }
     print(json.dumps(out), flush=True)
 

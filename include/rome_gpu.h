@@ -207,6 +207,18 @@ int rome_reconstruct(rome_ctx* ctx, int iclass, const rome_recon_params* params,
 /* MLModel::resetZero of the E-step sums + zeroed backprojector grids (MAPModel). */
 int rome_reset_sums(rome_ctx* ctx);
 
+/* The fine pass of a pool whose coarse significance is given (exp_Mcoarse_significant, which
+ * findAllSignificantPoints sets at src/map3d_optimizer.cpp:1880-1915): getAllSquaredDifferencesFine
+ * (src/map3d_optimizer.cpp:1261-1372) over those samples, then the weights, significance, sums,
+ * backprojection and storeWeightedSums of the second pass of expectationSomeParticles
+ * (src/map3d_optimizer.cpp:883-953), without the coarse pass.  sig_keys [n][cap] (device int32) holds
+ * sig_cnt[i] (device int32, 1 .. cap, cap <= 8192) distinct keys per image, key = m * nr_trans + t with
+ * m = (class * nr_dir + dir) * nr_psi + psi; an image outside that is ROME_ERR_ARG.  NR_SIGN = sig_cnt;
+ * the coarse-pass results (min_diff2_coarse, sum_weight_coarse, sig_weight_coarse) are NaN.  Needs
+ * adaptive_oversampling = 1. */
+int rome_expectation_fine_device(rome_ctx* ctx, const rome_device_images* images, const int* sig_keys,
+                                 const int* sig_cnt, int cap, rome_results* results);
+
 /* Map3dOptimizer_old::expectationSomeParticles (src/map3d_optimizer.cpp:883-953) or
  * Map2dOptimizer_old::expectationSomeParticles (src/map2d_optimizer.cpp:802-854), with
  * storeWeightedSums, for one pool of images ("iteration >= 2" semantics: all classes). */
@@ -234,6 +246,37 @@ int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9
 
 /* Projector into a device buffer (float2 [nrot][size*(size/2+1)]): synthetic particles. */
 int rome_project_device(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, void* out_device);
+
+/* Execution of a pool (no counterpart in the reference, whose pools run as OpenMP loops):
+ * ROME_EXEC_HOST   the host reads the fine pass's task totals back after the coarse pass and sizes every
+ *                  later launch from them (two host synchronisations per pool);
+ * ROME_EXEC_DEVICE the fine pass runs from buffers sized by capacities (significant samples per image x
+ *                  pool size) with its counts read on the device: persistent kernels stride over them, the
+ *                  fine diff2 takes its (image, rotation) tasks from a work queue; one synchronisation per
+ *                  pool, when the results are read back;
+ * ROME_EXEC_GRAPH  (default) as DEVICE, and the pool is captured once as a hipGraph per (pool size, input
+ *                  arrays) and replayed for later pools with the same inputs.
+ * A pool whose coarse list or significant-sample keys overflow, or that needs the dense significance mask,
+ * the learned proposal or profiling stage timers, runs (or is rerun) host-synchronised; the results are the
+ * same in every mode. */
+enum { ROME_EXEC_HOST = 0, ROME_EXEC_DEVICE = 1, ROME_EXEC_GRAPH = 2 };
+int rome_set_execution(rome_ctx* ctx, int mode);
+/* out4: how the last pool ran (0 host, 1 device, 2 graph replay; +8 when a device-driven pool was rerun
+ * host-synchronised), graph launches, graph captures, device-driven reruns. */
+int rome_get_execution_stats(rome_ctx* ctx, long long* out4);
+
+/* Tile / grid autotuner of the device-driven kernels (no counterpart in the reference): runs the pool
+ * `images` (with sig_keys / sig_cnt / cap: its fine pass from that significance, as
+ * rome_expectation_fine_device; NULL: the whole E-step) device-driven with 4 candidate persistent grids for each of the fine diff2 work queue, the
+ * fine projection, the per-slice post kernel and the binned backprojection (blocks per CU: {4, 8, 12,
+ * 16}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}), twice each, times their stages with HIP events and
+ * keeps the fastest grid of each kernel for the context.  chosen [4] receives the grids (blocks),
+ * stage_ms [4][4] the measured stage times per candidate.  The runs add to the E-step sums: call
+ * rome_reset_sums afterwards.  rome_set_persistent_grids sets the four grids directly (0: the default,
+ * 8, 8, 2 and 8 blocks per CU). */
+int rome_autotune(rome_ctx* ctx, const rome_device_images* images, const int* sig_keys, const int* sig_cnt, int cap,
+                  int* chosen, double* stage_ms);
+int rome_set_persistent_grids(rome_ctx* ctx, const int* blocks);
 
 /* Measurement: stage times from HIP events recorded on the engine stream around each kernel
  * group (no counterpart in the reference, whose timers are TIMEPOINT macros, src/time.h:61-75).
@@ -266,7 +309,7 @@ int rome_get_coarse_stats(rome_ctx* ctx, long long* out3);
 
 /* Form of the coarse significance the fine pass is built from (no counterpart in the reference, whose
  * Exp_Mweight stores it per sample): by default the keys of each image's significant samples, up to
- * `cap` (<= 4096) per image; a pool where some image has more goes through a dense per-sample mask
+ * `cap` (<= 8192) per image; a pool where some image has more goes through a dense per-sample mask
  * instead (and so do the next 8 pools).  cap = 0: the mask always; < 0: the default (4096).
  * Results are the same either way. */
 int rome_set_significance_cap(rome_ctx* ctx, int cap);
@@ -331,7 +374,7 @@ typedef struct {
     int nr_coef;           /* low-resolution coefficients of the input (812) */
     int nr_psi_bins;       /* in-plane bins per cell; divides nr_psi (4) */
     int top_k;             /* cells kept per image before the neighbours (64) */
-    double min_mass;       /* predicted captured mass below which an image takes every rotation (0.99) */
+    double min_mass;       /* predicted captured mass below which an image takes every rotation (0.9) */
     double audit_fraction; /* share of each pool that also runs the exhaustive coarse pass (0.02) */
     double min_audit_mass; /* audited captured mass below which the pool reruns exhaustively (0.99) */
     int target_cells;      /* cells per training target (32) */
@@ -347,6 +390,8 @@ enum { ROME_PROPOSAL_OFF = 0, ROME_PROPOSAL_COLLECT = 1, ROME_PROPOSAL_USE = 2 }
  * posteriors over cells to the training set; USE: pools run on the proposal; OFF: exhaustive. */
 int rome_proposal_configure(rome_ctx* ctx, const rome_proposal_config* cfg);
 int rome_proposal_mode(rome_ctx* ctx, int mode);
+/* top_k, min_mass and min_audit_mass of a configured proposal, keeping its network and training set. */
+int rome_proposal_set_thresholds(rome_ctx* ctx, int top_k, double min_mass, double min_audit_mass);
 /* Adam over the training set: epochs x (training images / batch) steps of `batch` images (a multiple of
  * 32), shuffled per epoch; loss_per_epoch [epochs] (may be NULL) receives each epoch's mean loss. */
 int rome_proposal_train(rome_ctx* ctx, int epochs, int batch, double lr, double* loss_per_epoch);

@@ -194,7 +194,7 @@ class OrConfig(C.Structure):
 class OrImages(C.Structure):
     _fields_ = [("n", C.c_int)] + [(n, _dp) for n in (
         "fmask_re", "fmask_im", "fnomask_re", "fnomask_im", "fctf", "highres_Xi2", "power_img",
-        "old_offx", "old_offy")] + [("group", _ip), ("norm", _dp)]
+        "old_offx", "old_offy")] + [("group", _ip), ("norm", _dp), ("coarse_sig_in", C.POINTER(C.c_ubyte))]
 
 
 class OrOutputs(C.Structure):
@@ -257,6 +257,11 @@ def estep(prob, batch, dump_coarse=False, dump_fine=False, dump_fine_weight=Fals
                    highres_Xi2=P(batch["highres_Xi2"]), power_img=P(batch["power_img"]),
                    old_offx=P(batch["old_offx"]), old_offy=P(batch["old_offy"]),
                    group=P(batch["group"], C.c_int), norm=P(batch["norm"]))
+    if batch.get("coarse_sig_in") is not None:  # coarse significance given: the fine pass alone
+        cs = np.ascontiguousarray(batch["coarse_sig_in"], dtype=np.uint8)
+        assert cs.size == n * K * D * PSI * T
+        keep.append(cs)
+        ims.coarse_sig_in = cs.ctypes.data_as(C.POINTER(C.c_ubyte))
     r = {}
 
     def alloc(name, shape, dtype=np.float64, fill=0.0):

@@ -734,7 +734,26 @@ int or_estep(const OrConfig* cfg, const OrImages* im, OrOutputs* out, char* errb
     Pass last{};
     std::vector<double> local_ctf_fine, local_minv_fine;  // last pass (fine) per-image CTF and Minv
 
-    for (int ipass = 0; ipass < npasses; ++ipass) {
+    // coarse significance given (the fine pass of getAllSquaredDifferencesFine over exp_Mcoarse_significant,
+    // src/map3d_optimizer.cpp:1261-1372, without the coarse pass that would have set it)
+    const bool given = im->coarse_sig_in != nullptr;
+    if (given) {
+        if (npasses != 2) return fail("given coarse significance needs adaptive_oversampling = 1");
+        std::memcpy(coarse_sig.data(), im->coarse_sig_in, coarse_sig.size());
+        for (int i = 0; i < N; ++i) {
+            const unsigned char* cs = &coarse_sig[(size_t)i * K * D * PSI * T];
+            int cnt = 0;
+            for (size_t q = 0; q < (size_t)K * D * PSI * T; ++q) cnt += cs[q] != 0;
+            if (cnt == 0) return fail("my_nr_significant_coarse_samples == 0");
+            nr_sign[i] = cnt;
+            const double nan = std::numeric_limits<double>::quiet_NaN();
+            if (out->min_diff2_coarse) out->min_diff2_coarse[i] = nan;
+            if (out->sum_weight_coarse) out->sum_weight_coarse[i] = nan;
+            if (out->sig_weight_coarse) out->sig_weight_coarse[i] = nan;
+        }
+    }
+
+    for (int ipass = given ? 1 : 0; ipass < npasses; ++ipass) {
         const bool coarse = (ipass == 0);
         Pass P;
         P.size = (coarse && os > 0) ? c.coarse_size : cur;

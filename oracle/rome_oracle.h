@@ -90,6 +90,10 @@ typedef struct {
     const double* old_offx, *old_offy;       /* [n] rounded old offsets */
     const int* group;                        /* [n] 0-based */
     const double* norm;                      /* [n] metadata NORM */
+    /* optional [n][K][dir][psi][T]: the coarse significance (exp_Mcoarse_significant) given instead of
+     * computed; the coarse pass is skipped and the fine pass runs over these samples (NR_SIGN = their
+     * count; the coarse-pass outputs are NaN).  Needs adaptive_oversampling = 1. */
+    const unsigned char* coarse_sig_in;
 } OrImages;
 
 typedef struct {

@@ -30,7 +30,11 @@ EXPORTED = (
     "rome_set_significance_cap", "rome_get_significance_stats", "rome_proposal_configure", "rome_proposal_mode",
     "rome_proposal_train", "rome_proposal_stats", "rome_proposal_get_params", "rome_proposal_set_params",
     "rome_proposal_forward", "rome_proposal_gradients", "rome_proposal_training_rows", "rome_backproject_images",
+    "rome_set_execution", "rome_get_execution_stats", "rome_expectation_fine_device",
+    "rome_autotune", "rome_set_persistent_grids", "rome_proposal_set_thresholds",
 )
+
+EXEC_HOST, EXEC_DEVICE, EXEC_GRAPH = 0, 1, 2
 
 COARSE_LIST, COARSE_FULL = 0, 1
 
@@ -309,6 +313,18 @@ class Engine:
         self._check(lib().rome_expectation_device(self._h, C.byref(im), None), "rome_expectation_device")
         return None
 
+    def expectation_fine_device(self, n, fmask, fnomask, fctf, highres_Xi2, power_img, old_offx, old_offy, norm,
+                                group, sig_keys, sig_cnt, cap, results=True):
+        """The fine pass of a pool whose coarse significance is given: sig_keys [n][cap] int32 keys
+        m * nr_trans + t (m = (class * nr_dir + dir) * nr_psi + psi), sig_cnt [n] int32, device pointers."""
+        im = rome_device_images(n=n, fmask=fmask, fnomask=fnomask, fctf=fctf, highres_Xi2=highres_Xi2,
+                                power_img=power_img, old_offx=old_offx, old_offy=old_offy, norm=norm, group=group)
+        r, cr = self._results(n) if results else (None, None)
+        self._check(lib().rome_expectation_fine_device(self._h, C.byref(im), C.c_void_p(sig_keys), C.c_void_p(sig_cnt),
+                                                       int(cap), C.byref(cr) if results else None),
+                    "rome_expectation_fine_device")
+        return r
+
     def get_sums(self):
         s = {"wsum_sigma2_noise": np.zeros((self.G, self.oriF)), "wsum_sumw_group": np.zeros(self.G),
              "wsum_pdf_class": np.zeros(self.K), "wsum_pdf_direction": np.zeros((self.K, self.D)),
@@ -441,7 +457,7 @@ class Engine:
                                                   _p(e), _p(sh)), "rome_backproject_images")
 
     # ---- learned pose proposal
-    def proposal_configure(self, hidden=1024, nr_coef=812, nr_psi_bins=4, top_k=64, min_mass=0.99,
+    def proposal_configure(self, hidden=1024, nr_coef=812, nr_psi_bins=4, top_k=64, min_mass=0.9,
                            audit_fraction=0.02, min_audit_mass=0.99, target_cells=32, shift_weight=1.0,
                            max_train=20000, seed=1):
         cfg = rome_proposal_config(hidden, nr_coef, nr_psi_bins, top_k, min_mass, audit_fraction, min_audit_mass,
@@ -449,6 +465,20 @@ class Engine:
         self._check(lib().rome_proposal_configure(self._h, C.byref(cfg)), "rome_proposal_configure")
         self.prop = dict(hidden=hidden, nr_coef=nr_coef, ncell=self.K * 4 * self.D * nr_psi_bins,
                          target_cells=target_cells)
+        self.prop_cfg = dict(top_k=top_k, min_mass=min_mass, min_audit_mass=min_audit_mass)
+
+    def proposal_set_gate(self, min_mass, top_k=None, min_audit_mass=None):
+        """Per-image predicted-mass gate (and optionally top_k, the audit's captured-mass bar) of the
+        configured proposal, keeping the trained network."""
+        c = self.prop_cfg
+        c["min_mass"] = min_mass
+        if top_k is not None:
+            c["top_k"] = top_k
+        if min_audit_mass is not None:
+            c["min_audit_mass"] = min_audit_mass
+        self._check(lib().rome_proposal_set_thresholds(self._h, int(c["top_k"]), C.c_double(c["min_mass"]),
+                                                       C.c_double(c["min_audit_mass"])),
+                    "rome_proposal_set_thresholds")
 
     def proposal_mode(self, mode):
         self._check(lib().rome_proposal_mode(self._h, PROPOSAL_MODES.get(mode, mode)), "rome_proposal_mode")
@@ -521,6 +551,35 @@ class Engine:
         out = np.zeros(3, np.int64)
         self._check(lib().rome_get_coarse_stats(self._h, _p(out, C.c_longlong)), "rome_get_coarse_stats")
         return dict(zip(("mode", "list_max", "list_total"), out.tolist()))
+
+    def set_execution(self, mode):
+        """How pools run: "host" (host-synchronised launches), "device" (device-driven fine pass, one
+        synchronisation per pool) or "graph" (device-driven, captured once per input set and replayed)."""
+        m = {"host": EXEC_HOST, "device": EXEC_DEVICE, "graph": EXEC_GRAPH}.get(mode, mode)
+        self._check(lib().rome_set_execution(self._h, int(m)), "rome_set_execution")
+
+    def autotune_device(self, n, fmask, fnomask, fctf, highres_Xi2, power_img, old_offx, old_offy, norm, group,
+                        sig_keys=None, sig_cnt=None, cap=0):
+        """rome_autotune on a pool of device arrays (with sig_keys / sig_cnt / cap: its fine pass from that
+        coarse significance): the persistent grids of the device-driven kernels (fine diff2, fine
+        projection, post slices, backprojection) are set to the fastest of 4 candidates each.  Returns
+        (chosen grids, stage ms [4][4]).  The runs add to the sums: reset them after."""
+        im = rome_device_images(n=n, fmask=fmask, fnomask=fnomask, fctf=fctf, highres_Xi2=highres_Xi2,
+                                power_img=power_img, old_offx=old_offx, old_offy=old_offy, norm=norm, group=group)
+        chosen = np.zeros(4, np.int32)
+        ms = np.zeros((4, 4))
+        self._check(lib().rome_autotune(self._h, C.byref(im), C.c_void_p(sig_keys), C.c_void_p(sig_cnt), int(cap),
+                                        _p(chosen, C.c_int), _p(ms)), "rome_autotune")
+        return chosen.tolist(), ms
+
+    def set_persistent_grids(self, blocks):
+        b = np.ascontiguousarray(blocks, np.int32)
+        self._check(lib().rome_set_persistent_grids(self._h, _p(b, C.c_int)), "rome_set_persistent_grids")
+
+    def execution_stats(self):
+        out = np.zeros(4, np.int64)
+        self._check(lib().rome_get_execution_stats(self._h, _p(out, C.c_longlong)), "rome_get_execution_stats")
+        return dict(zip(("last", "graph_launches", "graph_captures", "reruns"), out.tolist()))
 
     def set_significance_cap(self, cap=-1):
         """Significant samples per image kept as keys (<= 4096); 0: the dense mask always; < 0: default."""

@@ -77,21 +77,43 @@ DEV int brick_of(const BpArgs& a, const Sample& sm) {
     return sm.k * a.nb_class + (bz * a.nby + by) * a.nbx + bx;
 }
 
-__global__ __launch_bounds__(256) void k_bp_count(BpArgs a) {
-    __shared__ unsigned keys[kHash], cnt[kHash];
-    if (!(a.slice_w[blockIdx.x] > 0.)) return;  // a slice without weight has no record (whole block)
+// Slices of the pool: the host's count, or (device-driven pools) the device task total x NOR; 0 when
+// the pool is aborted or its slices exceed the slice buffer (the direct scatter took it).
+DEV long long bp_slices(const BpArgs& a, long long S_host) {
+    if (!a.ntask_dev) return S_host;
+    if (*a.abort) return 0;
+    const long long S = *a.ntask_dev * a.NOR;
+    return S > a.slice_cap ? 0 : S;
+}
+
+DEV void bp_count_one(const BpArgs& a, long long s, int pb, unsigned* keys, unsigned* cnt) {
+    if (!(a.slice_w[s] > 0.)) return;  // a slice without weight has no record (whole block)
     for (int h = threadIdx.x; h < kHash; h += blockDim.x) {
         keys[h] = 0u;
         cnt[h] = 0u;
     }
     __syncthreads();
-    const long long s = blockIdx.x;
-    const int p = blockIdx.y * blockDim.x + threadIdx.x;
+    const int p = pb * blockDim.x + threadIdx.x;
     Sample sm;
     if (p < a.Pf && bp_sample(a, s, p, sm)) atomicAdd(&cnt[hash_slot(keys, (unsigned)brick_of(a, sm))], 1u);
     __syncthreads();
     for (int h = threadIdx.x; h < kHash; h += blockDim.x)
         if (keys[h]) atomicAdd(&a.count[keys[h] - 1], cnt[h]);
+}
+
+// blockIdx.x = slice, blockIdx.y = pixel block; device-driven: a fixed 1-D grid strides over both
+__global__ __launch_bounds__(256) void k_bp_count(BpArgs a, long long S_host) {
+    __shared__ unsigned keys[kHash], cnt[kHash];
+    if (!a.ntask_dev) {
+        bp_count_one(a, blockIdx.x, blockIdx.y, keys, cnt);
+        return;
+    }
+    const long long S = bp_slices(a, S_host);
+    const int nPB = (a.Pf + 255) / 256;
+    for (long long L = blockIdx.x; L < S * nPB; L += gridDim.x) {
+        bp_count_one(a, L / nPB, (int)(L % nPB), keys, cnt);
+        __syncthreads();  // the hash table of the next item
+    }
 }
 
 // Exclusive scans of count[0..nb) into off[0..nb] and of the chunks per brick into coff[0..nb];
@@ -156,16 +178,14 @@ __global__ __launch_bounds__(1024) void k_bp_scan(const unsigned* __restrict__ c
     }
 }
 
-__global__ __launch_bounds__(256) void k_bp_fill(BpArgs a) {
-    __shared__ unsigned keys[kHash], cnt[kHash], base[kHash];
-    if (!(a.slice_w[blockIdx.x] > 0.)) return;  // a slice without weight has no record (whole block)
+DEV void bp_fill_one(const BpArgs& a, long long s, int pb, unsigned* keys, unsigned* cnt, unsigned* base) {
+    if (!(a.slice_w[s] > 0.)) return;  // a slice without weight has no record (whole block)
     for (int h = threadIdx.x; h < kHash; h += blockDim.x) {
         keys[h] = 0u;
         cnt[h] = 0u;
     }
     __syncthreads();
-    const long long s = blockIdx.x;
-    const int p = blockIdx.y * blockDim.x + threadIdx.x;
+    const int p = pb * blockDim.x + threadIdx.x;
     Sample sm;
     int h = -1;
     unsigned rank = 0;
@@ -180,19 +200,31 @@ __global__ __launch_bounds__(256) void k_bp_fill(BpArgs a) {
     if (h >= 0) a.rec[base[h] + rank] = ((unsigned long long)s << 32) | (unsigned)p;
 }
 
+__global__ __launch_bounds__(256) void k_bp_fill(BpArgs a, long long S_host) {
+    __shared__ unsigned keys[kHash], cnt[kHash], base[kHash];
+    if (!a.ntask_dev) {
+        bp_fill_one(a, blockIdx.x, blockIdx.y, keys, cnt, base);
+        return;
+    }
+    const long long S = bp_slices(a, S_host);
+    const int nPB = (a.Pf + 255) / 256;
+    for (long long L = blockIdx.x; L < S * nPB; L += gridDim.x) {
+        bp_fill_one(a, L / nPB, (int)(L % nPB), keys, cnt, base);
+        __syncthreads();  // the hash table of the next item
+    }
+}
+
 // One chunk of a brick's records without floating-point atomics: the records are counting-sorted by
 // base voxel in LDS, each thread sums the 8 corner contributions of the records of its base voxels in
 // registers, then the corners are added to the box in 8 passes, one corner offset per pass, in which
 // distinct base voxels reach distinct box voxels.
-__global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
+DEV void bp_brick_one(const BpArgs& a, int nb, unsigned ck) {
     constexpr int kVox = kBrick * kBrick * kBrick;  // base voxels of a brick
     __shared__ float box[3][kBox3];
     __shared__ unsigned vcnt[kVox], voff[kVox + 1];
     __shared__ unsigned short order[kChunk], vox[kChunk];
     __shared__ float rec_f[6][kChunk];  // fx, fy, fz, value re, value im (mirror applied), weight
-    const unsigned ck = blockIdx.x;
     const int tid = threadIdx.x;
-    if (ck >= a.coff[nb]) return;  // the grid is sized for the most chunks there can be
     // brick b with coff[b] <= ck < coff[b + 1] (bricks without records hold no chunk)
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
@@ -307,18 +339,43 @@ __global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
     }
 }
 
+// One block per chunk (the grid sized for the most chunks there can be), or (device-driven pools) a
+// fixed grid striding over the chunk total the scan left in coff[nb].
+__global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
+    if (!a.ntask_dev) {
+        if (blockIdx.x < a.coff[nb]) bp_brick_one(a, nb, blockIdx.x);
+        return;
+    }
+    if (*a.abort) return;
+    const unsigned nck = a.coff[nb];
+    for (unsigned ck = blockIdx.x; ck < nck; ck += gridDim.x) {
+        bp_brick_one(a, nb, ck);
+        __syncthreads();  // the box of the next chunk
+    }
+}
+
 }  // namespace
 
-hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a) {
+hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks) {
+    const int nb = a.nb_class * a.K;
+    hipError_t e;
+    if (a.ntask_dev) {  // device-driven: fixed grids over the device counts (a zero count does nothing)
+        if (!a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
+        if ((unsigned long long)a.slice_cap * a.Pf >= (1ULL << 32)) return hipErrorInvalidValue;
+        if ((e = launch_fill32(s, a.count, 0u, (size_t)nb)) != hipSuccess) return e;
+        k_bp_count<<<persist_blocks, 256, 0, s>>>(a, 0);
+        k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
+        k_bp_fill<<<persist_blocks, 256, 0, s>>>(a, 0);
+        k_bp_brick<<<persist_blocks, 256, 0, s>>>(a, nb);
+        return hipGetLastError();
+    }
     if (S <= 0) return hipSuccess;
     if (S > 0x7fffffffLL || (unsigned long long)S * a.Pf >= (1ULL << 32)) return hipErrorInvalidValue;
-    const int nb = a.nb_class * a.K;
-    hipError_t e = hipMemsetAsync(a.count, 0, (size_t)nb * sizeof(unsigned), s);
-    if (e != hipSuccess) return e;
+    if ((e = launch_fill32(s, a.count, 0u, (size_t)nb)) != hipSuccess) return e;
     const dim3 grid((unsigned)S, (a.Pf + 255) / 256);
-    k_bp_count<<<grid, 256, 0, s>>>(a);
+    k_bp_count<<<grid, 256, 0, s>>>(a, S);
     k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
-    k_bp_fill<<<grid, 256, 0, s>>>(a);
+    k_bp_fill<<<grid, 256, 0, s>>>(a, S);
     // at most one chunk per brick plus one per kChunk records
     const unsigned long long chunks = (unsigned long long)nb + ((unsigned long long)S * a.Pf + kChunk - 1) / kChunk;
     k_bp_brick<<<(unsigned)chunks, 256, 0, s>>>(a, nb);

@@ -7,6 +7,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -23,6 +24,12 @@
 
 namespace rome {
 
+// Device allocations move when a buffer grows; a captured pool graph holds the addresses it was captured
+// with, so every (re)allocation advances this epoch and graphs of an older epoch are not replayed.  While a
+// pool is being captured no buffer may grow (hipMalloc is not part of a graph): reserve() then fails.
+std::atomic<unsigned long long> g_alloc_epoch{0};
+thread_local bool g_capturing = false;
+
 // Pinned host memory (hipHostMalloc), grown on demand: read-backs and uploads go through it as one
 // DMA each instead of staged copies of pageable memory.
 struct PinnedBuf {
@@ -36,6 +43,8 @@ struct PinnedBuf {
     }
     hipError_t reserve(size_t bytes) {
         if (bytes <= n && p) return hipSuccess;
+        if (g_capturing) return hipErrorStreamCaptureUnsupported;
+        g_alloc_epoch.fetch_add(1);
         if (p) (void)hipHostFree(p);
         p = nullptr;
         n = 0;
@@ -62,9 +71,22 @@ struct DevBuf {
     // poses) gets 25 % headroom: hipFree waits for the device, so every regrowth stalls the stream.
     hipError_t reserve(size_t count) {
         if (count <= n && p) return hipSuccess;
+        if (g_capturing) return hipErrorStreamCaptureUnsupported;
+        g_alloc_epoch.fetch_add(1);
         if (p) count += count / 4;
         release();
         size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = std::max<size_t>(count, 1);
+        return e;
+    }
+    // capacities (device-driven pools): exactly the size asked for, no headroom
+    hipError_t reserve_exact(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        if (g_capturing) return hipErrorStreamCaptureUnsupported;
+        g_alloc_epoch.fetch_add(1);
+        release();
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
         hipError_t e = hipMalloc(&p, bytes);
         if (e == hipSuccess) n = std::max<size_t>(count, 1);
         return e;
@@ -84,6 +106,20 @@ struct DevBuf {
 
 struct Error {
     int code;
+};
+
+// Host-side results of a device-driven pool: every array the pool reads back, in one pinned block.
+struct PoolOut {
+    std::vector<double> meta, dmin, sumw, thr, dmin_c, sumw_c, thr_c, wnorm;
+    std::vector<int> argmax, nsig, cstatus, lcount, scount;
+    long long tot[2] = {0, 0};
+    int U = 0, flags[2] = {0, 0};
+    void size(int n) {
+        for (auto* v : {&dmin, &sumw, &thr, &dmin_c, &sumw_c, &thr_c, &wnorm}) v->assign(n, 0.);
+        meta.assign((size_t)9 * n, 0.);
+        argmax.assign((size_t)6 * n, 0);
+        for (auto* v : {&nsig, &cstatus, &lcount, &scount}) v->assign(n, 0);
+    }
 };
 
 // State of the learned pose proposal (rome_proposal_*): parameters (FP32 masters, BF16 copies and their
@@ -205,7 +241,7 @@ struct rome_ctx {
     DevBuf<int> c_nsig, c_status, f_status;
     DevBuf<uint32_t> sigmask;
     DevBuf<int> sig_keys, sig_cnt;
-    int sig_cap = kSigCap;   // significant samples per image kept as keys (0: the mask always)
+    int sig_cap = kSigCapDefault;  // significant samples per image kept as keys (0: the mask always)
     int sig_dense_left = 0;  // batches that go straight to the significance mask after a key overflow
     long long last_sig_form = 0, last_sig_max = -1;
     DevBuf<int> ntask, nblk;
@@ -217,7 +253,7 @@ struct rome_ctx {
     DevBuf<double> D2;
     DevBuf<int> argmax;
     // binned backprojection
-    static constexpr size_t kBinnedBytes = (size_t)24 << 30;  // slice values + records per batch
+    static constexpr size_t kBinnedBytes = (size_t)40 << 30;  // slice values + records per batch
     DevBuf<float2> slice_val;
     DevBuf<double> slice_w;
     DevBuf<unsigned> bp_count, bp_off, bp_coff, bp_cursor;
@@ -255,9 +291,56 @@ struct rome_ctx {
     // last batch bookkeeping (parity hooks)
     int last_n = 0;
     long long last_blocks = 0, last_tasks = 0;
+    // execution of a pool (rome_set_execution): host-synchronised launches, device-driven launches (the
+    // fine pass sized by capacities, its counts read on the device, one synchronisation per pool), or the
+    // device-driven pool captured once as a hipGraph and replayed
+    int exec_mode = ROME_EXEC_GRAPH;
+    int last_exec = 0;              // how the last pool ran (0 host, 1 device, 2 graph replay, +8 rerun)
+    int n_cu = 256;
+    int persist[4] = {0, 0, 0, 0};  // fixed grids of the device-driven kernels: fine diff2, projection,
+                                    // post slices, backprojection (0: from n_cu; rome_autotune sets them)
+    DevBuf<int> pool_flags;         // [0] abort (list or key overflow, no significant sample), [1] queue
+    unsigned long long state_epoch = 1;  // advanced by every call that changes what a pool's kernels read
+    static constexpr int kGraphKeys = 12;
+    struct PoolGraph {
+        hipGraphExec_t exec = nullptr;
+        unsigned long long alloc_epoch = 0, state_epoch = 0;
+        int n = 0, prec = 0;
+        const void* in[kGraphKeys] = {};  // inputs, given keys and counts, key cap
+        size_t rb_bytes = 0;
+        long long hits = 0;
+        bool failed = false;        // the pool could not be captured: no further attempts for this key
+    };
+    PoolOut pool_out;  // host side of device-driven pools' read-backs
+    // coarse significance of the current pool given by the caller (rome_expectation_fine_device)
+    struct {
+        const int* keys = nullptr;  // device [n][cap] keys m * T + t
+        const int* cnt = nullptr;   // device [n]
+        int cap = 0;
+    } given;
+    std::vector<PoolGraph> graphs;  // most recently used last
+    long long graph_launches = 0, graph_captures = 0, dd_reruns = 0;
+    static constexpr size_t kPfineBytes = (size_t)32 << 30;  // fine projections of a device-driven pool
+    int persist_blocks(int i) const {
+        static const int per_cu[4] = {8, 8, 2, 8};
+        return persist[i] > 0 ? persist[i] : n_cu * per_cu[i];
+    }
+    // unique rotations a device-driven pool holds fine projections for (a pool that needs more aborts)
+    int dd_ucap(int n) const {
+        const long long by_mem = (long long)(kPfineBytes / ((size_t)NORf * Pf * sizeof(float2)));
+        return (int)std::max<long long>(1, std::min<long long>({(long long)Mtot, (long long)n * sig_cap, by_mem}));
+    }
+    // slices of the binned backprojection a device-driven pool can hold
+    long long dd_slice_cap(int n) const {
+        const long long by_mem = (long long)(kBinnedBytes / ((size_t)Pf * 16));
+        const long long by_index = (long long)(((1ULL << 32) - 1) / (unsigned long long)Pf);
+        return std::max<long long>(1, std::min<long long>({(long long)n * sig_cap * NORf, by_mem, by_index}));
+    }
 };
 
 namespace {
+
+constexpr int kTuneCands = 4;  // candidate grids per kernel tried by rome_autotune
 
 int fail(rome_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
@@ -415,35 +498,74 @@ struct Seg {
     size_t bytes;  // a multiple of 4
 };
 
-// Device arrays -> host arrays: one pack kernel and one copy to pinned memory per kPackSegs arrays;
-// returns once the data is on the host.  Segments with a null host pointer are skipped.
-int readback(rome_ctx* ctx, const std::vector<Seg>& segs) {
-    hipStream_t s = ctx->stream;
-    for (size_t g0 = 0; g0 < segs.size(); g0 += kPackSegs) {
-        PackArgs pa{};
-        size_t off = 0;
-        std::vector<const Seg*> used;
-        for (size_t g = g0; g < std::min(segs.size(), g0 + kPackSegs); ++g) {
-            if (!segs[g].host || !segs[g].bytes) continue;
-            pa.src[pa.nseg] = segs[g].dev;
-            pa.off[pa.nseg] = off;
-            pa.len[pa.nseg] = segs[g].bytes;
-            off += (segs[g].bytes + 15) & ~(size_t)15;
-            ++pa.nseg;
-            used.push_back(&segs[g]);
-        }
-        if (!pa.nseg) continue;
-        pa.off[pa.nseg] = off;
-        HIPCHK(ctx->rb_dev.reserve(off));
-        HIPCHK(ctx->rb_host.reserve(off));
-        pa.dst = ctx->rb_dev.p;
-        HIPCHK(launch_pack(s, pa));
-        HIPCHK(hipMemcpyAsync(ctx->rb_host.p, ctx->rb_dev.p, off, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int q = 0; q < pa.nseg; ++q)
-            std::memcpy(used[q]->host, static_cast<const char*>(ctx->rb_host.p) + pa.off[q], used[q]->bytes);
+// Device arrays -> host arrays in two halves: readback_begin packs the segments (one pack kernel per
+// kPackSegs arrays, all into one device block) and copies the block into pinned host memory, without
+// waiting; readback_end synchronises and scatters into the host arrays.  `all`: segments with a null host
+// pointer are packed too (a captured pool keeps one layout whatever the caller asks for).
+struct RbLayout {
+    std::vector<size_t> off;  // per segment, SIZE_MAX when not packed
+    size_t bytes = 0;
+};
+
+void rb_layout(const std::vector<Seg>& segs, RbLayout& L, bool all) {
+    L.off.assign(segs.size(), SIZE_MAX);
+    L.bytes = 0;
+    for (size_t g = 0; g < segs.size(); ++g) {
+        if (!segs[g].bytes || !segs[g].dev || !(all || segs[g].host)) continue;
+        L.off[g] = L.bytes;
+        L.bytes += (segs[g].bytes + 15) & ~(size_t)15;
     }
+}
+
+int readback_begin(rome_ctx* ctx, const std::vector<Seg>& segs, RbLayout& L, bool all) {
+    hipStream_t s = ctx->stream;
+    rb_layout(segs, L, all);
+    if (!L.bytes) return ROME_OK;
+    if (g_capturing && (L.bytes > ctx->rb_dev.n || L.bytes > ctx->rb_host.n))
+        return fail(ctx, ROME_ERR_HIP, "read-back buffers would grow during capture");
+    HIPCHK(ctx->rb_dev.reserve(L.bytes));
+    HIPCHK(ctx->rb_host.reserve(L.bytes));
+    PackArgs pa{};
+    size_t base = 0;
+    auto flush = [&](size_t end) -> int {
+        if (!pa.nseg) return ROME_OK;
+        pa.off[pa.nseg] = end - base;
+        pa.dst = ctx->rb_dev.p + base;
+        HIPCHK(launch_pack(s, pa));
+        pa = PackArgs{};
+        base = end;
+        return ROME_OK;
+    };
+    for (size_t g = 0; g < segs.size(); ++g) {
+        if (L.off[g] == SIZE_MAX) continue;
+        if (pa.nseg == kPackSegs) {
+            const int r = flush(L.off[g]);
+            if (r) return r;
+        }
+        pa.src[pa.nseg] = segs[g].dev;
+        pa.off[pa.nseg] = L.off[g] - base;
+        pa.len[pa.nseg] = segs[g].bytes;
+        ++pa.nseg;
+    }
+    const int r = flush(L.bytes);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(ctx->rb_host.p, ctx->rb_dev.p, L.bytes, hipMemcpyDeviceToHost, s));
     return ROME_OK;
+}
+
+int readback_end(rome_ctx* ctx, const std::vector<Seg>& segs, const RbLayout& L) {
+    if (L.bytes) HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t g = 0; g < segs.size(); ++g)
+        if (L.off[g] != SIZE_MAX && segs[g].host)
+            std::memcpy(segs[g].host, static_cast<const char*>(ctx->rb_host.p) + L.off[g], segs[g].bytes);
+    return ROME_OK;
+}
+
+// returns once the data is on the host; segments with a null host pointer are skipped
+int readback(rome_ctx* ctx, const std::vector<Seg>& segs) {
+    RbLayout L;
+    const int r = readback_begin(ctx, segs, L, false);
+    return r ? r : readback_end(ctx, segs, L);
 }
 
 WeightParams weight_params(rome_ctx* ctx) {
@@ -639,11 +761,46 @@ int prop_collect(rome_ctx* ctx, int n) {
 }
 
 // The E-step on the device inputs ctx->in points to.
-int run_estep(rome_ctx* ctx, int n, rome_results* res) {
+// Pool modes of run_pool
+enum { kPoolHost = 0, kPoolDevice = 1, kPoolCapture = 2 };
+
+// The segments a device-driven pool reads back (the same list, in the same order, for a given pool
+// size: a captured pool's read-back layout).
+std::vector<Seg> pool_segs(rome_ctx* ctx, int n, PoolOut& o) {
+    const size_t nd = n * sizeof(double), ni = n * sizeof(int);
+    const bool tp = ctx->two_pass;
+    return {{ctx->f_sumw.p, o.sumw.data(), nd},
+            {ctx->meta.p, o.meta.data(), 9 * nd},
+            {ctx->argmax.p, o.argmax.data(), 6 * ni},
+            {ctx->c_nsig.p, o.nsig.data(), ni},
+            {ctx->f_dmin.p, o.dmin.data(), nd},
+            {ctx->f_thr.p, o.thr.data(), nd},
+            {tp ? ctx->c_dmin.p : ctx->f_dmin.p, o.dmin_c.data(), nd},
+            {tp ? ctx->c_sumw.p : ctx->f_sumw.p, o.sumw_c.data(), nd},
+            {tp ? ctx->c_thr.p : ctx->f_thr.p, o.thr_c.data(), nd},
+            {ctx->wnorm.p, o.wnorm.data(), nd},
+            {ctx->c_status.p, o.cstatus.data(), ni},
+            {ctx->list_cnt.p, o.lcount.data(), ni},
+            {ctx->sig_cnt.p, o.scount.data(), ni},
+            {ctx->task_off.p + n, &o.tot[0], sizeof(long long)},
+            {ctx->blk_off.p + n, &o.tot[1], sizeof(long long)},
+            {ctx->ucount.p, &o.U, sizeof(int)},
+            {ctx->pool_flags.p, o.flags, 2 * sizeof(int)}};
+}
+
+int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* dout, RbLayout* dlayout) {
+    const bool dd = pool_mode != kPoolHost;  // device-driven: no host synchronisation until the read-back
+    // buffers whose size follows the pool: 25 % headroom on the host path, exact capacities device-driven
+    auto sized = [dd](auto& buf, size_t count) { return dd ? buf.reserve_exact(count) : buf.grow(count); };
     int rc = ensure_ready(ctx);
     if (rc) return rc;
     const rome_config& c = ctx->cfg;
     hipStream_t s = ctx->stream;
+    if (dd || ctx->given.keys) {
+        HIPCHK(ctx->pool_flags.reserve(4));
+        HIPCHK(launch_fill32(s, ctx->pool_flags.p, 0u, 4));  // abort flag, work-queue head, invalid keys
+    }
+    if (dd) dout->size(n);
     const int T = ctx->T, M = ctx->Mtot;
     const int Ncols = n * T;
     const int Npad = (Ncols + 255) / 256 * 256;
@@ -653,18 +810,18 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     // ---- image preparation
     if (prec == ROME_PREC_FP32) {
         HIPCHK(ctx->WcT.reserve((size_t)Kdim * Npad));
-        HIPCHK(hipMemsetAsync(ctx->WcT.p, 0, (size_t)Kdim * Npad * sizeof(float), s));
+        HIPCHK(launch_fill32(s, ctx->WcT.p, 0u, (size_t)Kdim * Npad));
     } else {
         // 256 rows of slack: the audit slice of a learned-proposal pool reads whole 256-row tiles from its start
         HIPCHK(ctx->W16.reserve((size_t)(Npad + 256) * K16));
         HIPCHK(ctx->W16lo.reserve((size_t)(Npad + 256) * K16));
         HIPCHK(ctx->inv_scale.reserve(n));
         // rows past the last column only feed discarded outputs; keep them finite
-        HIPCHK(hipMemsetAsync(ctx->W16.p + (size_t)Ncols * K16, 0, (size_t)(Npad + 256 - Ncols) * K16 * 2, s));
+        HIPCHK(launch_fill32(s, ctx->W16.p + (size_t)Ncols * K16, 0u, (size_t)(Npad + 256 - Ncols) * K16 / 2));
     }
     const int ldn = (n + 127) / 128 * 128;  // images of the A-term operand, padded with zero columns
     HIPCHK(ctx->MC2c.reserve((size_t)ctx->Kq * ldn));
-    HIPCHK(hipMemsetAsync(ctx->MC2c.p, 0, (size_t)ctx->Kq * ldn * sizeof(float), s));
+    HIPCHK(launch_fill32(s, ctx->MC2c.p, 0u, (size_t)ctx->Kq * ldn));
     HIPCHK(ctx->C2c.reserve(n));
     HIPCHK(ctx->C2f.reserve(n));
     const size_t nPf = (size_t)n * ctx->Pf;
@@ -733,12 +890,14 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(launch_prep_lowp(s, prec, n, pl));
     }
     stage_end(ctx, 0);
-    // ---- coarse cross term + norms
-    HIPCHK(ctx->At.reserve((size_t)n * ctx->Mpad));
+    // ---- coarse cross term + norms (a pool with its coarse significance given has no coarse pass)
+    const bool given = ctx->given.keys != nullptr;
+    if (given && !ctx->two_pass) return fail(ctx, ROME_ERR_ARG, "given coarse significance needs oversampling 1");
+    HIPCHK(ctx->At.reserve(given ? 1 : (size_t)n * ctx->Mpad));
     WeightParams wp = weight_params(ctx);
     stage_begin(ctx, 1);
-    HIPCHK(launch_coarse_aterm(s, n, ctx->Qc.p, ctx->Kq, M, ctx->Mpad, ctx->MC2c.p, ldn, ctx->At.p));
-    if (ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass) {
+    if (!given) HIPCHK(launch_coarse_aterm(s, n, ctx->Qc.p, ctx->Kq, M, ctx->Mpad, ctx->MC2c.p, ldn, ctx->At.p));
+    if (!given && ctx->coarse_mode == ROME_COARSE_LIST && ctx->two_pass) {
         HIPCHK(ctx->reach.reserve(n));
         HIPCHK(launch_list_reach(s, n, wp, ctx->reach.p));
     }
@@ -752,7 +911,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     HIPCHK(ctx->c_nsig.reserve(n));
     HIPCHK(ctx->c_status.reserve(n));
     HIPCHK(ctx->f_status.reserve(n));
-    HIPCHK(ctx->used.reserve(M));
+    HIPCHK(ctx->used.reserve(((size_t)M + 3) / 4 * 4));
     HIPCHK(ctx->ulist.reserve(M));
     HIPCHK(ctx->m2u.reserve(M));
     HIPCHK(ctx->ucount.reserve(1));
@@ -770,11 +929,11 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     // A pool on the learned proposal computes its candidates' diff2 as split three-product sums already;
     // it needs the recompute only when it falls back to the exhaustive low-precision pass.
     ProposalState& P = ctx->prop;
-    const bool learned = P.ready && P.mode == ROME_PROPOSAL_USE && prec != ROME_PREC_FP32 && ctx->two_pass &&
+    const bool learned = !given && P.ready && P.mode == ROME_PROPOSAL_USE && prec != ROME_PREC_FP32 && ctx->two_pass &&
                          ctx->is3d && ctx->coarse_mode == ROME_COARSE_LIST && P.K == ctx->K && P.D == ctx->D &&
                          P.PSI == ctx->PSI && P.cur == c.current_size;
-    bool lowp_recompute = ctx->two_pass && prec != ROME_PREC_FP32 && !learned;
-    if (lowp_recompute) wp.mark_margin = kLowpRecomputeMargin;
+    bool lowp_recompute = ctx->two_pass && prec != ROME_PREC_FP32 && !learned && !given;
+    if (lowp_recompute) wp.mark_margin = prec == ROME_PREC_BF16 ? kLowpRecomputeMarginBF16 : kLowpRecomputeMargin;
     WeightParams wp_exact = wp;
     wp_exact.mark_margin = 0.0;
     WeightOutArgs wc{ctx->c_dmin.p, ctx->c_sumw.p,  ctx->c_maxw.p,
@@ -805,7 +964,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             sg.cnt = ctx->sig_cnt.p;
             sg.cap = ctx->sig_cap;
         }
-        HIPCHK(hipMemsetAsync(ctx->used.p, 0, M, s));
+        HIPCHK(launch_fill32(s, ctx->used.p, 0u, ((size_t)M + 3) / 4));
         stage_begin(ctx, 3);
         if (from_d2c)
             HIPCHK(launch_weights_fine(s, n, wp_exact, ctx->D2c.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p,
@@ -821,6 +980,11 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         stage_end(ctx, 4);
         if (L) lcount.resize(n);
         sig_dense = dense;
+        if (dd) {  // no read-back: a list or key overflow, or an image without significant samples, aborts
+            HIPCHK(launch_pool_check(s, n, ctx->c_status.p, sg.cnt, sg.cap, ctx->ucount.p, ctx->dd_ucap(n),
+                                     ctx->pool_flags.p));
+            return ROME_OK;
+        }
         std::vector<Seg> segs{{ctx->task_off.p + n, &tot[0], sizeof(long long)},
                               {ctx->blk_off.p + n, &tot[1], sizeof(long long)},
                               {ctx->ucount.p, &U, sizeof(int)},
@@ -844,13 +1008,14 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(ctx->list_cnt.reserve(n));
         HIPCHK(ctx->list_ent.reserve((size_t)n * cap));
         HIPCHK(ctx->list_val.reserve((size_t)n * cap));
-        HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->vbound.p, 0x7fffffff, n, s));
-        HIPCHK(hipMemsetAsync(ctx->list_cnt.p, 0, (size_t)n * sizeof(int), s));
+        HIPCHK(launch_fill32(s, ctx->vbound.p, 0x7fffffffu, (size_t)n));
+        HIPCHK(launch_fill32(s, ctx->list_cnt.p, 0u, (size_t)n));
         L = ListArgs{ctx->lpd_row.p, ctx->reach.p, ctx->vbound.p, ctx->list_ent.p, ctx->list_val.p, ctx->list_cnt.p, cap};
         return ROME_OK;
     };
     // weights and significance of the coarse pass (keys, or the mask past their cap)
     auto finish = [&](const ListArgs* L) -> int {
+        if (dd) return significance(L, false);  // keys only (the pool was chosen for it)
         const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
         int r = significance(L, dense);
         ctx->last_sig_form = dense ? 1 : 0;
@@ -864,7 +1029,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         return 0;
     };
     auto coarse = [&](bool list) -> int {
-        HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->emin_key.p, 0x7fffffff, n, s));
+        HIPCHK(launch_fill32(s, ctx->emin_key.p, 0x7fffffffu, (size_t)n));
         ListArgs L{};
         if (list) {
             const int lr = list_setup(L);
@@ -924,7 +1089,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             HIPCHK(P.E_audit.reserve((size_t)na * M * T));
             HIPCHK(P.audit_emin.reserve(na));
             HIPCHK(P.audit_mass.reserve(na));
-            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)P.audit_emin.p, 0x7fffffff, na, s));
+            HIPCHK(launch_fill32(s, P.audit_emin.p, 0x7fffffffu, (size_t)na));
             const int ncol = na * T, npad = (ncol + 255) / 256 * 256;
             LowpGemmArgs la{ctx->A16.p, ctx->W16.p + (size_t)a0 * T * K16, ctx->At.p + (size_t)a0 * ctx->Mpad,
                             ctx->inv_scale.p + a0, P.E_audit.p, ctx->valid.p, P.audit_emin.p, K16, ctx->Mpad, npad,
@@ -981,11 +1146,53 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     auto exhaustive_setup = [&]() {
         if (prec != ROME_PREC_FP32 && ctx->two_pass && !lowp_recompute) {
             lowp_recompute = true;
-            wp.mark_margin = kLowpRecomputeMargin;
+            wp.mark_margin = prec == ROME_PREC_BF16 ? kLowpRecomputeMarginBF16 : kLowpRecomputeMargin;
             wc.tail = ctx->c_tail.p;
         }
     };
-    if (!learned || fell_back) {
+    // The coarse significance given as keys (rome_expectation_fine_device): the task build reads them as
+    // the weights pass would have left them; NR_SIGN = their count, the coarse outputs are NaN.
+    auto given_sig = [&]() -> int {
+        const int gc = ctx->given.cap;
+        HIPCHK(ctx->sig_keys.reserve((size_t)n * gc));
+        HIPCHK(ctx->sig_cnt.reserve(n));
+        HIPCHK(hipMemcpyAsync(ctx->sig_keys.p, ctx->given.keys, (size_t)n * gc * sizeof(int),
+                              hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->sig_cnt.p, ctx->given.cnt, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->c_nsig.p, ctx->given.cnt, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(launch_fill32(s, ctx->c_status.p, 0u, (size_t)n));
+        for (auto* b : {&ctx->c_dmin, &ctx->c_sumw, &ctx->c_thr})  // all-ones words: NaN
+            HIPCHK(launch_fill32(s, b->p, 0xffffffffu, 2 * (size_t)n));
+        HIPCHK(launch_fill32(s, ctx->used.p, 0u, ((size_t)M + 3) / 4));
+        const SigArgs sg{nullptr, ctx->TW, ctx->sig_keys.p, ctx->sig_cnt.p, gc};
+        stage_begin(ctx, 4);
+        HIPCHK(launch_check_given_keys(s, n, ctx->sig_keys.p, gc, ctx->sig_cnt.p, (long long)M * T, ctx->pool_flags.p));
+        HIPCHK(launch_task_build(s, n, sg, M, T, ctx->ntask.p, ctx->nblk.p, ctx->task_off.p, ctx->blk_off.p,
+                                 ctx->used.p, ctx->ulist.p, ctx->m2u.p, ctx->ucount.p));
+        HIPCHK(launch_check_sorted_dups(s, n, ctx->sig_keys.p, gc, ctx->sig_cnt.p, ctx->pool_flags.p));
+        stage_end(ctx, 4);
+        sig_dense = false;
+        if (dd) {
+            HIPCHK(launch_pool_check(s, n, ctx->c_status.p, ctx->sig_cnt.p, gc, ctx->ucount.p, ctx->dd_ucap(n),
+                                     ctx->pool_flags.p));
+            return ROME_OK;
+        }
+        int flags[4] = {0, 0, 0, 0};
+        rc = readback(ctx, {{ctx->task_off.p + n, &tot[0], sizeof(long long)},
+                            {ctx->blk_off.p + n, &tot[1], sizeof(long long)},
+                            {ctx->ucount.p, &U, sizeof(int)},
+                            {ctx->pool_flags.p, flags, sizeof(flags)}});
+        if (rc) return rc;
+        if (flags[2])
+            return fail(ctx, ROME_ERR_ARG,
+                        "given coarse significance: an image with no key, more than %d, a key outside [0, %lld) "
+                        "or a repeated key", gc, (long long)M * T);
+        return ROME_OK;
+    };
+    if (given) {
+        rc = given_sig();
+        if (rc) return rc;
+    } else if (!learned || fell_back) {
         if (fell_back) P.st_fallback += 1;
         exhaustive_setup();
         rc = coarse(use_list);
@@ -999,7 +1206,10 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         ctx->last_list_total += v;
     }
     ctx->last_coarse_full = use_list ? 0 : 1;
-    if (use_list && std::any_of(cstatus.begin(), cstatus.end(), [](int v) { return (v & 8) != 0; })) {
+    // Device-driven pools size the fine pass by capacities: at most sig_cap significant (or marked) samples,
+    // hence tasks and blocks, per image; the counts stay on the device.
+    const long long cap_tb = (long long)n * ctx->sig_cap;
+    if (!dd && use_list && std::any_of(cstatus.begin(), cstatus.end(), [](int v) { return (v & 8) != 0; })) {
         ctx->last_coarse_full = 2;  // a list overflowed: the whole batch again with the full pose term
         exhaustive_setup();
         rc = coarse(false);
@@ -1008,17 +1218,18 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     ctx->last_marked = 0;
     if (lowp_recompute) {
         // tasks of the marked samples -> split three-product diff2 -> exact weights and significance
-        for (int i = 0; i < n; ++i)
-            if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: no coarse sample marked", i);
-        const long long mt = tot[0], mb = tot[1];
-        ctx->last_marked = mb;
-        HIPCHK(ctx->task_img.grow(mt));
-        HIPCHK(ctx->task_m.grow(mt));
-        HIPCHK(ctx->task_nt.grow(mt));
-        HIPCHK(ctx->task_blk0.grow(mt));
-        HIPCHK(ctx->blk_task.grow(mb));
-        HIPCHK(ctx->blk_t.grow(mb));
-        HIPCHK(ctx->D2c.grow(mb));
+        if (!dd)
+            for (int i = 0; i < n; ++i)
+                if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: no coarse sample marked", i);
+        const long long mt = dd ? cap_tb : tot[0], mb = dd ? cap_tb : tot[1];
+        ctx->last_marked = dd ? -1 : mb;
+        HIPCHK(sized(ctx->task_img, mt));
+        HIPCHK(sized(ctx->task_m, mt));
+        HIPCHK(sized(ctx->task_nt, mt));
+        HIPCHK(sized(ctx->task_blk0, mt));
+        HIPCHK(sized(ctx->blk_task, mb));
+        HIPCHK(sized(ctx->blk_t, mb));
+        HIPCHK(sized(ctx->D2c, mb));
         HIPCHK(ctx->c_dmin1.reserve(n));
         const SigArgs sg1 = sig_dense ? SigArgs{ctx->sigmask.p, ctx->TW, nullptr, nullptr, 0}
                                       : SigArgs{nullptr, ctx->TW, ctx->sig_keys.p, ctx->sig_cnt.p, ctx->sig_cap};
@@ -1032,36 +1243,42 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         HIPCHK(launch_recompute_split(s, prec, n, ra));
         stage_end(ctx, 11);
         HIPCHK(hipMemcpyAsync(ctx->c_dmin1.p, ctx->c_dmin.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        const bool dense = ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL;
+        const bool dense =
+            !dd && (ctx->sig_cap == 0 || ctx->sig_dense_left > 0 || (long long)M * T > 0x7fffffffLL);
         rc = significance(nullptr, dense, true);
         if (rc) return rc;
         // The significant samples are drawn from the marked ones, whose keys fit the cap when this pass
         // is sparse; a rerun here could not read D2c anyway (the task build above replaced its block lists).
-        if (!dense && *std::max_element(scount.begin(), scount.end()) > ctx->sig_cap)
+        if (!dd && !dense && *std::max_element(scount.begin(), scount.end()) > ctx->sig_cap)
             return fail(ctx, ROME_ERR_NUMERIC, "recomputed significant set exceeds the marked set's key cap");
     }
-    for (int i = 0; i < n; ++i)
-        if (cstatus[i] & 1) return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
-    const long long ntasks = tot[0], nblocks = tot[1];
+    if (!dd)
+        for (int i = 0; i < n; ++i)
+            if (cstatus[i] & 1)
+                return fail(ctx, ROME_ERR_NUMERIC, "image %d: my_nr_significant_coarse_samples == 0", i);
+    const long long ntasks = dd ? cap_tb : tot[0], nblocks = dd ? cap_tb : tot[1];
+    if (dd) U = ctx->dd_ucap(n);  // capacity of the unique rotations (more aborts the pool)
     ctx->last_n = n;
     ctx->last_tasks = ntasks;
     ctx->last_blocks = nblocks;
     const int NOR = ctx->NORf, NOT = ctx->NOTf, NE = NOR * NOT;
-    HIPCHK(ctx->task_img.grow(ntasks));
-    HIPCHK(ctx->task_m.grow(ntasks));
-    HIPCHK(ctx->task_nt.grow(ntasks));
-    HIPCHK(ctx->task_blk0.grow(ntasks));
-    HIPCHK(ctx->blk_task.grow(nblocks));
-    HIPCHK(ctx->blk_t.grow(nblocks));
+    HIPCHK(sized(ctx->task_img, ntasks));
+    HIPCHK(sized(ctx->task_m, ntasks));
+    HIPCHK(sized(ctx->task_nt, ntasks));
+    HIPCHK(sized(ctx->task_blk0, ntasks));
+    HIPCHK(sized(ctx->blk_task, nblocks));
+    HIPCHK(sized(ctx->blk_t, nblocks));
     const SigArgs sg = sig_dense ? SigArgs{ctx->sigmask.p, ctx->TW, nullptr, nullptr, 0}
                                  : SigArgs{nullptr, ctx->TW, ctx->sig_keys.p, ctx->sig_cnt.p, ctx->sig_cap};
     HIPCHK(launch_task_emit(s, n, sg, M, T, ctx->task_off.p, ctx->blk_off.p, ctx->task_img.p, ctx->task_m.p,
                             ctx->task_blk0.p, ctx->task_nt.p, ctx->blk_task.p, ctx->blk_t.p));
-    HIPCHK(ctx->D2.grow((size_t)nblocks * NE));
-    HIPCHK(ctx->Pfine.grow((size_t)U * NOR * ctx->Pf));
+    HIPCHK(sized(ctx->D2, (size_t)nblocks * NE));
+    HIPCHK(sized(ctx->Pfine, (size_t)U * NOR * ctx->Pf));
     stage_begin(ctx, 5);
-    HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->is3d ? ctx->cube.p : nullptr, ctx->grid_n, ctx->pad, ctx->padF, ctx->cgrid, ctx->is3d, ctx->Ai_f.p,
-                               ctx->R, NOR, ctx->ulist.p, U, ctx->f_xy.p, ctx->Pf, ctx->r_max, ctx->Pfine.p));
+    HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->is3d ? ctx->cube.p : nullptr, ctx->grid_n, ctx->pad, ctx->padF,
+                               ctx->cgrid, ctx->is3d, ctx->Ai_f.p, ctx->R, NOR, ctx->ulist.p, U,
+                               dd ? ctx->ucount.p : nullptr, ctx->persist_blocks(1), ctx->f_xy.p, ctx->Pf, ctx->r_max,
+                               ctx->Pfine.p));
     stage_end(ctx, 5);
     FineArgs fa;
     fa.task_img = ctx->task_img.p;
@@ -1078,6 +1295,12 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     fa.C2f = ctx->C2f.p;
     fa.D2 = ctx->D2.p;
     fa.Pf = ctx->Pf;
+    if (dd) {
+        fa.ntask_dev = ctx->task_off.p + n;
+        fa.queue = ctx->pool_flags.p + 1;
+        fa.abort = ctx->pool_flags.p;
+        HIPCHK(launch_fill32(s, ctx->pool_flags.p + 1, 0u, 1));  // work-queue head
+    }
     // Two passes: the fine pass over the coarse-significant samples (src/map3d_optimizer.cpp:1261-1372).
     // Single pass (oversampling 0): the same kernels recompute, with NOR = NOT = 1, the diff2 of the
     // samples the FP32 cross term marked significant (with kRecomputeMargin nats of slack); their weights,
@@ -1088,13 +1311,13 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
                      ctx->two_pass ? nullptr : ctx->c_nsig.p, ctx->f_status.p, nullptr};
     const TailArgs tail = ctx->two_pass ? TailArgs{nullptr, nullptr} : TailArgs{ctx->c_tail.p, ctx->c_dmin.p};
     stage_begin(ctx, 6);
-    HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa));
+    HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa, ctx->persist_blocks(0)));
     stage_end(ctx, 6);
     stage_begin(ctx, 7);
     HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE, wf,
                                cand, tail));
     stage_end(ctx, 7);
-    if (P.ready && P.mode == ROME_PROPOSAL_COLLECT && ctx->two_pass && ctx->is3d && P.K == ctx->K && P.D == ctx->D &&
+    if (!given && P.ready && P.mode == ROME_PROPOSAL_COLLECT && ctx->two_pass && ctx->is3d && P.K == ctx->K && P.D == ctx->D &&
         P.PSI == ctx->PSI && P.cur == c.current_size) {
         rc = prop_collect(ctx, n);  // the pool's posteriors become training targets
         if (rc) return rc;
@@ -1158,25 +1381,43 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         pp.wsum_prior_y = ctx->wsum_prior_y.p;
         pp.wsum_scalars = ctx->wsum_scalars.p;
         pp.wnorm = ctx->wnorm.p;
-        // 3D: slices to HBM, then the binned scatter (bounded by a slice buffer of kBinnedBytes)
-        const long long S = ntasks * NOR;
+        // 3D: slices to HBM, then the binned scatter (bounded by a slice buffer of kBinnedBytes).
+        // Device-driven pools: the slice buffer has a fixed capacity; the per-slice kernel and the binned
+        // scatter take a pool whose slices fit it, the direct scatter (k_post) one whose slices do not.
+        long long S = ntasks * NOR;
+        const long long slice_cap = ctx->dd_slice_cap(n);
+        if (dd) S = post_slices_supported(NOR, NOT, ctx->Pf, ctx->is3d) ? slice_cap : 0;
         const bool binned = ctx->is3d && S > 0 && (unsigned long long)S * ctx->Pf < (1ULL << 32) &&
                             (size_t)S * ctx->Pf * 16 <= rome_ctx::kBinnedBytes;
+        if (dd) {
+            pp.ntask_dev = ctx->task_off.p + n;
+            pp.abort = ctx->pool_flags.p;
+            pp.slice_cap = binned ? slice_cap : -1;  // -1: the direct scatter takes every pool
+        }
         pp.slice_val = nullptr;
         pp.slice_w = nullptr;
         if (binned) {
-            HIPCHK(ctx->slice_val.grow((size_t)S * ctx->Pf));
-            HIPCHK(ctx->slice_w.grow((size_t)S));
+            HIPCHK(sized(ctx->slice_val, (size_t)S * ctx->Pf));
+            HIPCHK(sized(ctx->slice_w, (size_t)S));
             pp.slice_val = ctx->slice_val.p;
             pp.slice_w = ctx->slice_w.p;
         }
         stage_begin(ctx, 8);
         hipError_t pe = hipErrorNotSupported;
-        if (binned) {
-            HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
-            pe = launch_post_slices(s, ntasks, NOR, NOT, pp);
+        if (dd) {
+            HIPCHK(launch_fill32(s, ctx->wnorm.p, 0u, 2 * (size_t)n));
+            if (binned) HIPCHK(launch_post_slices(s, ntasks, NOR, NOT, pp, ctx->persist_blocks(2)));
+            PostArgs pd = pp;  // the direct scatter of pools past the slice buffer
+            pd.slice_val = nullptr;
+            pd.slice_w = nullptr;
+            pe = launch_post(s, n, NOR, NOT, pd);
+        } else {
+            if (binned) {
+                HIPCHK(launch_fill32(s, ctx->wnorm.p, 0u, 2 * (size_t)n));
+                pe = launch_post_slices(s, ntasks, NOR, NOT, pp);
+            }
+            if (pe == hipErrorNotSupported) pe = launch_post(s, n, NOR, NOT, pp);
         }
-        if (pe == hipErrorNotSupported) pe = launch_post(s, n, NOR, NOT, pp);
         HIPCHK(pe);
         stage_end(ctx, 8);
         if (binned) {
@@ -1199,7 +1440,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             HIPCHK(ctx->bp_off.reserve(nb + 1));
             HIPCHK(ctx->bp_coff.reserve(nb + 1));
             HIPCHK(ctx->bp_cursor.reserve(nb));
-            HIPCHK(ctx->bp_rec.grow((size_t)S * ctx->Pf));
+            HIPCHK(sized(ctx->bp_rec, (size_t)S * ctx->Pf));
             ba.task_img = ctx->task_img.p;
             ba.task_m = ctx->task_m.p;
             ba.Ai_fine = ctx->Ai_f.p;
@@ -1214,12 +1455,17 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
             ba.rec = ctx->bp_rec.p;
             ba.bp_data = ctx->bp_data.p;
             ba.bp_weight = ctx->bp_weight.p;
+            if (dd) {
+                ba.ntask_dev = ctx->task_off.p + n;
+                ba.slice_cap = slice_cap;
+                ba.abort = ctx->pool_flags.p;
+            }
             stage_begin(ctx, 10);
-            HIPCHK(launch_backproject_binned(s, S, ba));
+            HIPCHK(launch_backproject_binned(s, S, ba, ctx->persist_blocks(3)));
             stage_end(ctx, 10);
         }
     } else {
-        HIPCHK(hipMemsetAsync(ctx->wnorm.p, 0, n * sizeof(double), s));
+        HIPCHK(launch_fill32(s, ctx->wnorm.p, 0u, 2 * (size_t)n));
     }
     // ---- metadata of the last pass (on the device; read back with the results)
     MetaArgs ma;
@@ -1264,9 +1510,14 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     sa.meta = ctx->meta.p;
     sa.wsum_sigma2 = ctx->wsum_sigma2.p;
     sa.wsum_scalars = ctx->wsum_scalars.p;
+    if (dd) sa.abort = ctx->pool_flags.p;
     stage_begin(ctx, 9);
     HIPCHK(launch_store_sums(s, sa));
     stage_end(ctx, 9);
+    if (dd) {  // everything the host needs in one read-back; the caller synchronises (or replays)
+        ctx->last_U = U;
+        return readback_begin(ctx, pool_segs(ctx, n, *dout), *dlayout, true);
+    }
     std::vector<double> sumw(n);
     {
         const size_t nd = n * sizeof(double);
@@ -1292,9 +1543,187 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     return ROME_OK;
 }
 
+void graph_release(rome_ctx::PoolGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+}
+
+void graphs_clear(rome_ctx* ctx) {
+    for (auto& g : ctx->graphs) graph_release(g);
+    ctx->graphs.clear();
+}
+
+// The cached graph of a pool with these inputs (nullptr: none).  Graphs of older allocation or state
+// epochs are released.
+rome_ctx::PoolGraph* graph_find(rome_ctx* ctx, int n, const void* const in[rome_ctx::kGraphKeys], bool create) {
+    const unsigned long long ae = g_alloc_epoch.load();
+    for (size_t i = 0; i < ctx->graphs.size(); ++i) {
+        auto& g = ctx->graphs[i];
+        if (g.alloc_epoch != ae || g.state_epoch != ctx->state_epoch) {
+            graph_release(g);
+            ctx->graphs.erase(ctx->graphs.begin() + (long)i);
+            --i;
+            continue;
+        }
+        if (g.n == n && g.prec == ctx->cfg.precision && std::equal(in, in + rome_ctx::kGraphKeys, g.in)) return &g;
+    }
+    if (!create) return nullptr;
+    if (ctx->graphs.size() >= 16) {  // least recently used first
+        graph_release(ctx->graphs.front());
+        ctx->graphs.erase(ctx->graphs.begin());
+    }
+    ctx->graphs.emplace_back();
+    auto& g = ctx->graphs.back();
+    g.n = n;
+    g.prec = ctx->cfg.precision;
+    std::copy(in, in + rome_ctx::kGraphKeys, g.in);
+    g.alloc_epoch = ae;
+    g.state_epoch = ctx->state_epoch;
+    return &g;
+}
+
+// Records the device-driven pool (same buffers, same inputs as the pool that just ran) into a graph.  A
+// failure leaves no graph and marks the key so that it is not tried again; it is not an error of the pool.
+void graph_capture(rome_ctx* ctx, int n, rome_ctx::PoolGraph& g) {
+    hipStream_t s = ctx->stream;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        g.failed = true;
+        return;
+    }
+    g_capturing = true;
+    PoolOut tmp;
+    RbLayout L;
+    const std::string err = ctx->err;
+    const int rc = run_pool(ctx, n, nullptr, kPoolCapture, &tmp, &L);
+    g_capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    ctx->err = err;
+    if (rc || e != hipSuccess || !graph ||
+        hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        g.exec = nullptr;
+        g.failed = true;
+    } else {
+        g.rb_bytes = L.bytes;
+        ++ctx->graph_captures;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    // the epochs the graph was captured in (capture itself allocates nothing)
+    g.alloc_epoch = g_alloc_epoch.load();
+    g.state_epoch = ctx->state_epoch;
+}
+
+// One pool: host-synchronised, device-driven, or a replay of its captured graph (rome_set_execution).
+int run_estep(rome_ctx* ctx, int n, rome_results* res) {
+    const ProposalState& P = ctx->prop;
+    const bool prop_shape = P.ready && ctx->two_pass && ctx->is3d && P.K == ctx->K && P.D == ctx->D &&
+                            P.PSI == ctx->PSI && P.cur == ctx->cfg.current_size;
+    const bool prop_on = prop_shape && (P.mode == ROME_PROPOSAL_COLLECT ||
+                                        (P.mode == ROME_PROPOSAL_USE && ctx->cfg.precision != ROME_PREC_FP32));
+    const bool dd_ok = ctx->exec_mode != ROME_EXEC_HOST && ctx->configured && ctx->two_pass &&
+                       ctx->coarse_mode == ROME_COARSE_LIST && ctx->sig_cap > 0 && ctx->sig_dense_left == 0 &&
+                       (long long)ctx->Mtot * ctx->T <= 0x7fffffffLL && !prop_on;
+    if (!dd_ok) {
+        ctx->last_exec = 0;
+        return run_pool(ctx, n, res, kPoolHost, nullptr, nullptr);
+    }
+    PoolOut& o = ctx->pool_out;
+    hipStream_t s = ctx->stream;
+    const bool use_graph = ctx->exec_mode == ROME_EXEC_GRAPH && !ctx->profiling;
+    const void* in[rome_ctx::kGraphKeys] = {ctx->in.fmask, ctx->in.fnomask, ctx->in.fctf, ctx->in.hxi2, ctx->in.power,
+                                  ctx->in.offx, ctx->in.offy, ctx->in.norm, ctx->in.group, ctx->given.keys,
+                                  ctx->given.cnt, reinterpret_cast<const void*>((uintptr_t)ctx->sig_cap)};
+    rome_ctx::PoolGraph* g = use_graph ? graph_find(ctx, n, in, false) : nullptr;
+    RbLayout L;
+    int rc;
+    bool capture_after = false;
+    if (g && g->exec) {
+        o.size(n);
+        const std::vector<Seg> segs = pool_segs(ctx, n, o);
+        rb_layout(segs, L, true);
+        if (L.bytes != g->rb_bytes) return fail(ctx, ROME_ERR_HIP, "captured pool read-back layout changed");
+        HIPCHK(hipGraphLaunch(g->exec, s));
+        rc = readback_end(ctx, segs, L);
+        if (rc) return rc;
+        ++g->hits;
+        ++ctx->graph_launches;
+        ctx->last_exec = 2;
+    } else {
+        rc = run_pool(ctx, n, res, kPoolDevice, &o, &L);
+        if (rc) {
+            (void)hipStreamSynchronize(s);  // launches before the failure may still run
+            return rc;
+        }
+        rc = readback_end(ctx, pool_segs(ctx, n, o), L);
+        if (rc) return rc;
+        stage_collect(ctx);
+        ctx->last_exec = 1;
+        capture_after = use_graph && !(g && g->failed);
+    }
+    if (o.flags[0]) {  // a list or key overflow, no significant sample, or too many rotations: host path
+        ++ctx->dd_reruns;
+        const int le = ctx->last_exec;
+        rc = run_pool(ctx, n, res, kPoolHost, nullptr, nullptr);
+        ctx->last_exec = le | 8;
+        return rc;
+    }
+    for (int i = 0; i < n; ++i)
+        if (!(o.sumw[i] > 0)) return fail(ctx, ROME_ERR_NUMERIC, "image %d: exp_sum_weight == 0", i);
+    if (capture_after) {  // before the bookkeeping below: a capture pass sets it from capacities
+        g = graph_find(ctx, n, in, true);
+        if (!g->exec && !g->failed) graph_capture(ctx, n, *g);
+    }
+    if (res) {
+        auto put = [n](void* dst, const void* src, size_t per) {
+            if (dst) std::memcpy(dst, src, per * n);
+        };
+        put(res->meta, o.meta.data(), 9 * sizeof(double));
+        put(res->argmax, o.argmax.data(), 6 * sizeof(int));
+        put(res->nr_sign, o.nsig.data(), sizeof(int));
+        put(res->min_diff2, o.dmin.data(), sizeof(double));
+        put(res->sum_weight, o.sumw.data(), sizeof(double));
+        put(res->sig_weight, o.thr.data(), sizeof(double));
+        put(res->min_diff2_coarse, o.dmin_c.data(), sizeof(double));
+        put(res->sum_weight_coarse, o.sumw_c.data(), sizeof(double));
+        put(res->sig_weight_coarse, o.thr_c.data(), sizeof(double));
+        put(res->wsum_norm_correction, o.wnorm.data(), sizeof(double));
+    }
+    ctx->last_n = n;
+    ctx->last_tasks = o.tot[0];
+    ctx->last_blocks = o.tot[1];
+    ctx->last_U = o.U;
+    ctx->last_list_max = 0;
+    ctx->last_list_total = 0;
+    for (int v : o.lcount) {
+        ctx->last_list_max = std::max<long long>(ctx->last_list_max, v);
+        ctx->last_list_total += v;
+    }
+    ctx->last_coarse_full = 0;
+    ctx->last_sig_form = 0;
+    ctx->last_sig_max = *std::max_element(o.scount.begin(), o.scount.end());
+    return ROME_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int rome_set_execution(rome_ctx* ctx, int mode) {
+    if (!ctx || mode < ROME_EXEC_HOST || mode > ROME_EXEC_GRAPH) return ROME_ERR_ARG;
+    ctx->exec_mode = mode;
+    return ROME_OK;
+}
+
+int rome_get_execution_stats(rome_ctx* ctx, long long* out) {
+    if (!ctx || !out) return ROME_ERR_ARG;
+    out[0] = ctx->last_exec;
+    out[1] = ctx->graph_launches;
+    out[2] = ctx->graph_captures;
+    out[3] = ctx->dd_reruns;
+    return ROME_OK;
+}
 
 int rome_create(int device, rome_ctx** out) {
     if (!out) return ROME_ERR_ARG;
@@ -1308,6 +1737,9 @@ int rome_create(int device, rome_ctx** out) {
         delete ctx;
         return ROME_ERR_HIP;
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        ctx->n_cu = ncu;
     // 5000-entry sin/cos table of shiftImageInFourierTransform (src/image.h:1486-1510)
     std::vector<double> st(5000), ct(5000);
     const double step = 2 * 3.14159265358979323846 / 5000.0;
@@ -1334,6 +1766,7 @@ void rome_destroy(rome_ctx* ctx) {
             for (hipEvent_t e : pr)
                 if (e) (void)hipEventDestroy(e);
     for (auto& p : ctx->fft_plans) (void)hipfftDestroy(p.h);
+    graphs_clear(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1341,6 +1774,7 @@ void rome_destroy(rome_ctx* ctx) {
 const char* rome_last_error(const rome_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx || !cfg) return ROME_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     const rome_config& c = *cfg;
@@ -1447,6 +1881,7 @@ int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
 int rome_set_sampling(rome_ctx* ctx, const double* pose_coarse, const double* pose_fine, const double* shift_coarse_x,
                       const double* shift_coarse_y, const double* shift_fine_x, const double* shift_fine_y,
                       const double* trans_x, const double* trans_y) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx) return ROME_ERR_ARG;
     if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
     if (!pose_coarse || !shift_coarse_x || !shift_coarse_y || !trans_x || !trans_y)
@@ -1489,6 +1924,7 @@ int rome_set_sampling(rome_ctx* ctx, const double* pose_coarse, const double* po
 
 int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_class, const double* pdf_direction,
                    const double* prior_offx, const double* prior_offy) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx) return ROME_ERR_ARG;
     if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
     if (!sigma2_noise || !pdf_class || !pdf_direction || !prior_offx || !prior_offy)
@@ -1561,6 +1997,7 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
 }
 
 int rome_set_reference(rome_ctx* ctx, int iclass, const double* grid) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx || !grid) return ROME_ERR_ARG;
     if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
     if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
@@ -1645,6 +2082,32 @@ int rome_expectation_device(rome_ctx* ctx, const rome_device_images* im, rome_re
     ctx->in = {reinterpret_cast<const float2*>(im->fmask), reinterpret_cast<const float2*>(im->fnomask), im->fctf,
                im->highres_Xi2, im->power_img, im->old_offx, im->old_offy, im->norm, im->group};
     return run_estep(ctx, n, res);
+}
+
+int rome_expectation_fine_device(rome_ctx* ctx, const rome_device_images* im, const int* sig_keys,
+                                 const int* sig_cnt, int cap, rome_results* res) {
+    if (!ctx || !im || !sig_keys || !sig_cnt) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (!ctx->two_pass) return fail(ctx, ROME_ERR_ARG, "given coarse significance needs adaptive_oversampling 1");
+    if (cap < 1 || cap > kSigCap) return fail(ctx, ROME_ERR_ARG, "key cap %d outside 1 .. %d", cap, kSigCap);
+    if ((long long)ctx->Mtot * ctx->T > 0x7fffffffLL)
+        return fail(ctx, ROME_ERR_UNSUPPORTED, "coarse samples per image exceed the 31-bit keys");
+    if ((reinterpret_cast<uintptr_t>(sig_keys) | reinterpret_cast<uintptr_t>(sig_cnt)) % 4)
+        return fail(ctx, ROME_ERR_ARG, "key arrays misaligned");
+    // the pool runs with the caller's cap as its key stride (capacities of the device-driven path too)
+    const int saved_cap = ctx->sig_cap, saved_left = ctx->sig_dense_left;
+    ctx->sig_cap = cap;
+    ctx->sig_dense_left = 0;
+    ctx->given.keys = sig_keys;
+    ctx->given.cnt = sig_cnt;
+    ctx->given.cap = cap;
+    const int rc = rome_expectation_device(ctx, im, res);
+    ctx->given.keys = nullptr;
+    ctx->given.cnt = nullptr;
+    ctx->given.cap = 0;
+    ctx->sig_cap = saved_cap;
+    ctx->sig_dense_left = saved_left;
+    return rc;
 }
 
 int rome_get_sums(rome_ctx* ctx, rome_sums* o) {
@@ -1781,6 +2244,60 @@ int rome_set_profiling(rome_ctx* ctx, int on) {
     return ROME_OK;
 }
 
+int rome_autotune(rome_ctx* ctx, const rome_device_images* im, const int* sig_keys, const int* sig_cnt, int cap,
+                  int* chosen, double* stage_ms) {
+    if (!ctx || !im) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    // per tuned kernel: its stage and the candidate grids in blocks per CU
+    static const int stage_of[4] = {6, 5, 8, 10};
+    static const int per_cu[4][kTuneCands] = {{4, 8, 12, 16}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}};
+    const bool prof = ctx->profiling;
+    const int mode = ctx->exec_mode;
+    int saved[4];
+    std::copy(ctx->persist, ctx->persist + 4, saved);
+    int rc = rome_set_profiling(ctx, 1);
+    if (rc) return rc;
+    ctx->exec_mode = ROME_EXEC_DEVICE;
+    double t[4][kTuneCands];
+    for (int c = 0; c < kTuneCands && !rc; ++c) {
+        for (int i = 0; i < 4; ++i) ctx->persist[i] = ctx->n_cu * per_cu[i][c];
+        for (int rep = 0; rep < 2 && !rc; ++rep) {  // the first run of a candidate is not timed
+            for (int i = 0; i < rome_ctx::kStages; ++i) ctx->stage_ms[i] = 0.;
+            rc = sig_keys ? rome_expectation_fine_device(ctx, im, sig_keys, sig_cnt, cap, nullptr)
+                          : rome_expectation_device(ctx, im, nullptr);
+        }
+        if (!rc && ctx->last_exec != 1) rc = fail(ctx, ROME_ERR_UNSUPPORTED, "the pool does not run device-driven");
+        for (int i = 0; i < 4; ++i) t[i][c] = ctx->stage_ms[stage_of[i]];
+    }
+    ctx->exec_mode = mode;
+    (void)rome_set_profiling(ctx, prof ? 1 : 0);
+    ++ctx->state_epoch;  // kernels of captured pools read what this call changes
+    if (rc) {
+        std::copy(saved, saved + 4, ctx->persist);
+        return rc;
+    }
+    for (int i = 0; i < 4; ++i) {
+        int best = 0;
+        for (int c = 1; c < kTuneCands; ++c)
+            if (t[i][c] < t[i][best]) best = c;
+        ctx->persist[i] = ctx->n_cu * per_cu[i][best];
+        if (chosen) chosen[i] = ctx->persist[i];
+        if (stage_ms)
+            for (int c = 0; c < kTuneCands; ++c) stage_ms[i * kTuneCands + c] = t[i][c];
+    }
+    return ROME_OK;
+}
+
+int rome_set_persistent_grids(rome_ctx* ctx, const int* blocks) {
+    if (!ctx || !blocks) return ROME_ERR_ARG;
+    for (int i = 0; i < 4; ++i)
+        if (blocks[i] < 0) return fail(ctx, ROME_ERR_ARG, "negative grid size");
+    std::copy(blocks, blocks + 4, ctx->persist);
+    ++ctx->state_epoch;  // kernels of captured pools read what this call changes
+    return ROME_OK;
+}
+
 int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, int nstages) {
     if (!ctx || nstages < 0) return ROME_ERR_ARG;
     for (int i = 0; i < nstages && i < rome_ctx::kStages; ++i) {
@@ -1791,6 +2308,7 @@ int rome_get_profile(rome_ctx* ctx, double* stage_ms, long long* stage_calls, in
 }
 
 int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx) return ROME_ERR_ARG;
     if (mode != ROME_COARSE_LIST && mode != ROME_COARSE_FULL) return fail(ctx, ROME_ERR_ARG, "unknown coarse mode %d", mode);
     ctx->coarse_mode = mode;
@@ -1799,9 +2317,10 @@ int rome_set_coarse_mode(rome_ctx* ctx, int mode, int list_cap) {
 }
 
 int rome_set_significance_cap(rome_ctx* ctx, int cap) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx) return ROME_ERR_ARG;
     if (cap > kSigCap) return fail(ctx, ROME_ERR_ARG, "significance cap %d above %d", cap, kSigCap);
-    ctx->sig_cap = cap < 0 ? kSigCap : cap;
+    ctx->sig_cap = cap < 0 ? kSigCapDefault : cap;
     ctx->sig_dense_left = 0;
     return ROME_OK;
 }
@@ -1872,6 +2391,7 @@ int rome_healpix_pix2ang(int order, int pix, double* z, double* phi) {
 }  // extern "C"
 
 int rome_update_model(rome_ctx* ctx, int install, rome_model* out) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx) return ROME_ERR_ARG;
     if (!ctx->configured || !ctx->have_model) return fail(ctx, ROME_ERR_ARG, "no configured model to update");
     const rome_config& c = ctx->cfg;
@@ -1931,6 +2451,7 @@ static int fft_plan(rome_ctx* ctx, hipfftType type, int rank, int n, hipfftHandl
 
 int rome_fourier_map(rome_ctx* ctx, int iclass, const double* vol, int do_gridding, double* power_spectrum,
                      double* corrected_map, double* grid_out) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx || !vol) return ROME_ERR_ARG;
     if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
     if (iclass < 0 || iclass >= ctx->K) return fail(ctx, ROME_ERR_ARG, "class index out of range");
@@ -2272,6 +2793,7 @@ int rome_prepare_images(rome_ctx* ctx, const rome_raw_images* in, const rome_pre
 
 // ------------------------------------------------------------------ learned pose proposal (C-ABI)
 int rome_proposal_configure(rome_ctx* ctx, const rome_proposal_config* cfg) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx || !cfg) return ROME_ERR_ARG;
     if (!ctx->configured || !ctx->have_sampling)
         return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure needs rome_configure and rome_set_sampling");
@@ -2396,12 +2918,23 @@ int rome_proposal_configure(rome_ctx* ctx, const rome_proposal_config* cfg) {
 }
 
 int rome_proposal_mode(rome_ctx* ctx, int mode) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     if (!ctx || mode < ROME_PROPOSAL_OFF || mode > ROME_PROPOSAL_USE) return ROME_ERR_ARG;
     if (mode != ROME_PROPOSAL_OFF && !ctx->prop.ready)
         return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
     if (mode == ROME_PROPOSAL_USE && ctx->cfg.precision == ROME_PREC_FP32)
         return fail(ctx, ROME_ERR_UNSUPPORTED, "the learned proposal needs precision BF16 or FP16");
     ctx->prop.mode = mode;
+    return ROME_OK;
+}
+
+int rome_proposal_set_thresholds(rome_ctx* ctx, int top_k, double min_mass, double min_audit_mass) {
+    if (!ctx || top_k < 1 || !(min_mass >= 0. && min_mass <= 1.) || !(min_audit_mass >= 0. && min_audit_mass <= 1.))
+        return ROME_ERR_ARG;
+    if (!ctx->prop.ready) return fail(ctx, ROME_ERR_ARG, "rome_proposal_configure has not been called");
+    ctx->prop.cfg.top_k = top_k;
+    ctx->prop.cfg.min_mass = min_mass;
+    ctx->prop.cfg.min_audit_mass = min_audit_mass;
     return ROME_OK;
 }
 

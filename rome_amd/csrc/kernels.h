@@ -58,10 +58,13 @@ constexpr double kRecomputeMargin = 2.0;
 
 // Mixed arms: the BF16 / FP16 cross term marks the samples within this many nats of its significance
 // threshold for the split three-product recompute, which then decides significance.  The rounding of the
-// low-precision operands moves a diff2 by about 0.04 nat (BF16, 2^-8 per operand over a few hundred
-// products of a 128^2 image; FP16 an eighth of that), so 2 nats keep the exact significant set inside
-// the marked one by a wide margin.
-constexpr double kLowpRecomputeMargin = 2.0;
+// low-precision operands moves a diff2 by an amount that grows with the image's signal: about 0.04 nat
+// for BF16 (2^-8 per operand over a few hundred products) on a 128^2 image at SNR 0.05, 4.2 nats on a
+// 64^2 image at SNR 1; FP16 an eighth of that.  The margins, 12 nats (BF16) and 2 nats (FP16), exceed
+// twice the largest error measured (the threshold's and the sample's); the mixed-arm tests check them
+// against the measured coarse error of every case.
+constexpr double kLowpRecomputeMargin = 2.0;       // FP16
+constexpr double kLowpRecomputeMarginBF16 = 12.0;  // BF16
 
 // Coarse list output of the cross-term GEMMs (instead of the full pose term E): per image the
 // entries e = t*Mtot + m that can lie within kWeightWindow of the image's minimum u, with their E
@@ -209,6 +212,11 @@ struct FineArgs {
     const double* C2f;
     double* D2;           // [blocks][NOR*NOT]
     int Pf;
+    // device-driven pools (all null otherwise): task total on the device, work-queue head (zeroed
+    // before the launch), pool abort flag (non-zero: the pool is rerun, nothing is computed)
+    const long long* ntask_dev = nullptr;
+    int* queue = nullptr;
+    const int* abort = nullptr;
 };
 
 struct PostArgs {
@@ -259,6 +267,11 @@ struct PostArgs {
     // binned 3D backprojection: slices go to HBM instead of the grids (null: direct scatter)
     float2* slice_val;          // [ntask * NOR][Pf]
     double* slice_w;            // [ntask * NOR]
+    // device-driven pools (null otherwise): the task total on the device, the slice buffer's capacity
+    // (pools with more slices take the direct scatter) and the pool abort flag
+    const long long* ntask_dev = nullptr;
+    long long slice_cap = 0;
+    const int* abort = nullptr;
 };
 
 // Binned 3D backprojection (backproject.hip)
@@ -280,6 +293,11 @@ struct BpArgs {
     unsigned long long* rec;      // [records] slice << 32 | pixel
     float* bp_data;
     float* bp_weight;
+    // device-driven pools (null otherwise): slices = task total on the device x NOR; more than
+    // slice_cap (the direct scatter took the pool) or a set abort flag: nothing to do
+    const long long* ntask_dev = nullptr;
+    long long slice_cap = 0;
+    const int* abort = nullptr;
 };
 
 // Image preparation of a pool (setFourierTransforms), FP64; any output pointer may be null.
@@ -329,6 +347,7 @@ struct StoreArgs {
     double* meta;             // [n][9]
     double* wsum_sigma2;
     double* wsum_scalars;
+    const int* abort = nullptr;  // device-driven pools: non-zero, the pool adds nothing
 };
 
 hipError_t launch_phase_table(hipStream_t s, const double* sx, const double* sy, int nshift, const int* xy, int P, int ori,
@@ -338,9 +357,11 @@ hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid
                                  int Mpad, float* FcT);
 // cube[K][grid_n] records of 4 float4: the 2x2x2 trilinear neighbourhood of every voxel of a 3D grid
 hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, int K, int pad, int padF, float4* cube);
+// U_dev non-null: the unique-rotation count is read on the device and persist_blocks blocks stride over it
 hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4* cube, size_t grid_stride, int pad,
                                int padF, int c, int is3d, const double* Ai_fine, int R, int NOR, const int* ulist, int U,
-                               const int* xy, int Pf, int r_max, float2* Pfine);
+                               const int* U_dev, int persist_blocks, const int* xy, int Pf, int r_max,
+                               float2* Pfine);
 hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
                                 int nrot, int size, int my_r_max, float2* out);
 hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a);
@@ -362,7 +383,8 @@ struct CandArgs {  // per-image candidate list of the weight pass: [n][cap]
 //                    keys[img][0 .. cap); cnt[img] = how many there are (may exceed cap: the caller
 //                    then redoes the pass with the mask);
 //   otherwise      : mask[img][m][TW] bit t of rotation m.
-constexpr int kSigCap = 4096;  // most keys the sparse task build sorts per image (in LDS)
+constexpr int kSigCap = 8192;  // most keys the sparse task build sorts per image (in LDS)
+constexpr int kSigCapDefault = 4096;  // keys per image the significance pass keeps by default
 struct SigArgs {
     uint32_t* mask;
     int TW;
@@ -398,11 +420,27 @@ hipError_t launch_task_build(hipStream_t s, int n, const SigArgs& sig, int M, in
 hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int T, const long long* task_off,
                             const long long* blk_off, int* task_img, int* task_m, long long* task_blk0, int* task_nt,
                             int* blk_task, int* blk_t);
-hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a);
+hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a, int persist_blocks = 0);
 hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a);
 // The binned 3D form with one block per slice (hipErrorNotSupported outside NOR = 8, NOT = 4,
 // Pf <= 8192): wnorm must be zero before the launch.
-hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a);
+// The per-slice post kernel handles NOR 8 x NOT 4, 3D, Pf <= 8192 (else hipErrorNotSupported).  With
+// a.ntask_dev (device-driven pools) persist_blocks blocks stride over the device task total.
+bool post_slices_supported(int NOR, int NOT, int Pf, int is3d);
+hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a,
+                              int persist_blocks = 0);
+// Coarse significance given by a caller (keys m * T + t): an image with a count outside 1 .. cap or a key
+// outside [0, MT) keeps none (cnt[img] = 0) and sets flags[2] and flags[0]; after the task build has
+// sorted the keys, a repeated key sets the same flags.
+hipError_t launch_check_given_keys(hipStream_t s, int n, const int* keys, int cap, int* cnt, long long MT,
+                                   int* flags);
+hipError_t launch_check_sorted_dups(hipStream_t s, int n, const int* keys, int cap, const int* cnt, int* flags);
+// Device-driven pools: flags[0] |= 1 (abort, rerun host-synchronised) on a list overflow (status 8), an
+// image without significant samples (status 1), more keys than cap, or more unique rotations than ucap.
+hipError_t launch_pool_check(hipStream_t s, int n, const int* status, const int* cnt, int cap, const int* ucount,
+                             int ucap, int* flags);
+// p[0 .. words) = value (a kernel: safe to record into a graph, unlike hipMemsetAsync nodes)
+hipError_t launch_fill32(hipStream_t s, void* p, uint32_t value, size_t words);
 // Gathers up to kPackSegs device arrays into one contiguous staging buffer (one launch), so that a
 // group of small read-backs is one copy to pinned host memory.  Sizes and offsets are multiples of 4.
 constexpr int kPackSegs = 16;
@@ -449,7 +487,9 @@ struct BpImageArgs {
 };
 hipError_t launch_bp_images(hipStream_t s, int n, const BpImageArgs& a);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
-hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a);
+// With a.ntask_dev (device-driven pools) S is read on the device and persist_blocks blocks stride over
+// the slices, pixel blocks and brick chunks.
+hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks = 0);
 // M-step reconstruction (reconstruct.hip)
 hipError_t rc_fold(hipStream_t s, double2* data, double* weight, int Dz, int P, int PF);
 hipError_t rc_decenter_real(hipStream_t s, const double* cen, double* out, int Dz, int P, int PF, int max_r2);

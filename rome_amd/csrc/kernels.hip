@@ -90,33 +90,40 @@ __global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad
 
 // Fine projections for the unique rotations the batch needs: Pf[u][o][p].  3D classes read the
 // neighbourhood grid (cube, same values and arithmetic as the flat gather); 2D the flat grid.
+// U_dev (device-driven pools): the count of unique rotations is read from the device (at most U_host, the
+// capacity of Pfine; a pool with more is aborted) and the grid is a fixed set of blocks that strides over
+// the work items; otherwise one work item per block.
 __global__ void k_project_fine(const float2* __restrict__ grids, const float4* __restrict__ cube, size_t grid_stride,
                                int pad, int padF, int c, int is3d, const double* __restrict__ Ai_fine, int R, int NOR,
-                               const int* __restrict__ ulist, int U, const int* __restrict__ xy, int Pf, int r_max,
-                               float2* __restrict__ Pfine) {
+                               const int* __restrict__ ulist, int U_host, const int* __restrict__ U_dev,
+                               const int* __restrict__ xy, int Pf, int r_max, float2* __restrict__ Pfine) {
     // Workgroups go to the 8 XCDs round-robin by launch index; each XCD takes one contiguous range of
     // work items w = (u * nPB + pixel block) * NOR + o, so the NOR oversampled orientations of a
-    // rotation, which read neighbouring records, share one L2.
+    // rotation, which read neighbouring records, share one L2.  The grid is a multiple of 8 blocks, so a
+    // block striding by the grid size stays on its XCD's range.
+    const int U = U_dev ? min(*U_dev, U_host) : U_host;  // device-driven: U_host is the buffer's capacity
     const int nPB = (Pf + 255) / 256;
     const long long G = (long long)U * NOR * nPB, per = (G + 7) / 8;
-    const long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x;  // gridDim.x is a multiple of 8
-    if ((L >> 3) >= per) return;  // the last grid row runs past 8 * per
-    const long long w = (L & 7) * per + (L >> 3);
-    if (w >= G) return;
-    const int o = (int)(w % NOR);
-    const long long upb = w / NOR;
-    const int u = (int)(upb / nPB);
-    const int p = (int)(upb % nPB) * 256 + threadIdx.x;
-    if (p >= Pf) return;
-    const int m = ulist[u];
-    const int k = m / R, r = m % R;
-    const int x = px_x(xy[p]), y = px_y(xy[p]);
-    float2 v = make_float2(0.f, 0.f);
-    if (x * x + y * y <= r_max * r_max) {
-        Coords q = map_coords(Ai_fine + 6 * ((size_t)r * NOR + o), x, y, c, is3d != 0);
-        v = cube ? gather_cube(cube + 4 * grid_stride * k, pad, padF, q) : gather(grids + grid_stride * k, pad, padF, q, false);
+    const long long nL = (long long)gridDim.y * gridDim.x;  // gridDim.x is a multiple of 8
+    for (long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x; (L >> 3) < per; L += nL) {
+        const long long w = (L & 7) * per + (L >> 3);
+        if (w >= G) continue;
+        const int o = (int)(w % NOR);
+        const long long upb = w / NOR;
+        const int u = (int)(upb / nPB);
+        const int p = (int)(upb % nPB) * 256 + threadIdx.x;
+        if (p >= Pf) continue;
+        const int m = ulist[u];
+        const int k = m / R, r = m % R;
+        const int x = px_x(xy[p]), y = px_y(xy[p]);
+        float2 v = make_float2(0.f, 0.f);
+        if (x * x + y * y <= r_max * r_max) {
+            Coords q = map_coords(Ai_fine + 6 * ((size_t)r * NOR + o), x, y, c, is3d != 0);
+            v = cube ? gather_cube(cube + 4 * grid_stride * k, pad, padF, q)
+                     : gather(grids + grid_stride * k, pad, padF, q, false);
+        }
+        Pfine[((size_t)u * NOR + o) * Pf + p] = v;
     }
-    Pfine[((size_t)u * NOR + o) * Pf + p] = v;
 }
 
 // Standalone projection of arbitrary rotations into full half-planes (parity tests and
@@ -646,13 +653,13 @@ __global__ __launch_bounds__(1024) void k_unique_rot(const unsigned char* __rest
 // with FP32 products, per-thread FP32 partials and an FP64 block reduction.
 // src/map3d_optimizer.cpp:1261-1372 (fine), 2D src/map2d_optimizer.cpp:964-1051.
 template <int NOR, int NOT>
-__global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
+DEV void fine_task(const FineArgs& a, int q) {
     constexpr int NT = 256;
     constexpr int TG = (NOR * NOT >= 32) ? 2 : ((NOR * NOT >= 8) ? 8 : 16);  // translations per group
     constexpr int NJ = TG * NOT;
     __shared__ double red[NT / 64][NOR * NJ + NOR];
     __shared__ int ts[kMaxT];
-    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int img = a.task_img[q], m = a.task_m[q], nt = a.task_nt[q];
     const long long b0 = a.task_blk0[q];
     const int u = a.m2u[m];
@@ -731,10 +738,29 @@ __global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
             }
         }
         __syncthreads();
-        // keep A partial sums for later groups
-        if (g0 == 0) {
-            // nothing: Aacc is reused (it was fully accumulated in the first group)
-        }
+    }
+}
+
+// One task (image, rotation) per block, or (device-driven pools) a persistent grid whose blocks take
+// tasks from a work queue: the head counter a.queue (zero at launch) against the task total the task
+// build left on the device.  Tasks hold 1 .. T significant translations, so their costs differ; the
+// queue hands the next task to whichever block finishes first.
+template <int NOR, int NOT>
+__global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
+    if (!a.queue) {
+        fine_task<NOR, NOT>(a, blockIdx.x);
+        return;
+    }
+    if (*a.abort) return;  // the pool is rerun: nothing here is used
+    const long long ntask = *a.ntask_dev;
+    __shared__ int s_q;
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int q = s_q;
+        if (q >= ntask) break;  // the same value in every thread
+        fine_task<NOR, NOT>(a, q);
+        __syncthreads();  // s_q is rewritten next
     }
 }
 
@@ -753,6 +779,7 @@ __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
     __shared__ double red[NT / 64];
     __shared__ double shell[kMaxShell];
     __shared__ double s_pdfk[kMaxK], s_px[kMaxK], s_py[kMaxK];
+    if (a.ntask_dev && (*a.abort || *a.ntask_dev * NOR <= a.slice_cap)) return;  // aborted, or binned
     const int img = blockIdx.x, tid = threadIdx.x;
     const int g = a.group[img];
     const double sumw = a.sumw[img], thr = a.thr[img], dmin = a.dmin[img];
@@ -937,16 +964,12 @@ __global__ __launch_bounds__(NT) void k_post(PostArgs a) {
 // pixels added to the group's spectrum, and its share of the per-image / per-class sums (atomics; the
 // caller zeroes wnorm first).
 template <int NOR, int NOT, int PPT, int NT>
-__global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) {
+DEV void post_slice_one(const PostArgs& a, long long w, int h) {
     __shared__ double wts[kMaxT * NOT];
     __shared__ double red[NT / 64];
     __shared__ double shell[kMaxShell];
     const int tid = threadIdx.x;
-    const long long G = ntask * NOR, per = (G + 7) / 8;
-    const long long L = blockIdx.x;
-    if ((L >> 3) >= per) return;
-    const long long w = (L & 7) * per + (L >> 3);
-    if (w >= G) return;
+    const int p0 = h * PPT * NT;  // this block's pixel range of the slice: [p0, p0 + PPT * NT)
     const long long q = w / NOR;
     const int o = (int)(w - q * NOR);
     const int img = a.task_img[q];
@@ -972,7 +995,7 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
     }
     // a slice without a significant sample adds nothing: leave before reading its pixels
     if (!__syncthreads_or(sig)) {
-        if (tid == 0) a.slice_w[(size_t)q * NOR + o] = 0.;
+        if (tid == 0 && h == 0) a.slice_w[(size_t)q * NOR + o] = 0.;
         return;
     }
     const float2* X = a.Xf + (size_t)img * a.Pf;
@@ -983,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
     double wsig[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const int p = tid + j * NT;
+        const int p = p0 + tid + j * NT;
         const bool in = p < a.Pf;
         Xr[j] = in ? X[p] : make_float2(0.f, 0.f);
         NMr[j] = in ? NM[p] : make_float2(0.f, 0.f);
@@ -1010,7 +1033,7 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
             const float2* S = a.Sf + ((size_t)t * NOT + ot) * a.Pf;
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
-                const int p = tid + j * NT;
+                const int p = p0 + tid + j * NT;
                 if (p < a.Pf) {
                     const float2 s = S[p];
                     const float2 xs = cmul(Xr[j], s);
@@ -1024,13 +1047,13 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
         }
     }
     const size_t sl = (size_t)q * NOR + o;
-    if (tid == 0) a.slice_w[sl] = wslice;
+    if (tid == 0 && h == 0) a.slice_w[sl] = wslice;
     if (!(wslice > 0.)) return;  // the same value in every thread: the whole block leaves
     float2* V = a.slice_val + sl * a.Pf;
     double tot = 0.;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const int p = tid + j * NT;
+        const int p = p0 + tid + j * NT;
         if (p < a.Pf) {
             V[p] = acc[j];
             atomicAdd(&shell[a.f_ires[p]], wsig[j]);
@@ -1043,17 +1066,39 @@ __global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) 
         if (shell[s] != 0.) atomicAdd(&a.wsum_sigma2[(size_t)g * a.oriF + s], shell[s]);
     if (tid == 0) {
         atomicAdd(&a.wnorm[img], tot);
-        atomicAdd(&a.wsum_sumw_group[g], wslice);
-        atomicAdd(&a.wsum_pdf_class[k], wslice);
-        atomicAdd(&a.wsum_pdf_dir[(size_t)k * a.D + dir], wslice);
-        atomicAdd(&a.wsum_scalars[0], t_off);
+        if (h == 0) {  // the slice's own sums, once per slice
+            atomicAdd(&a.wsum_sumw_group[g], wslice);
+            atomicAdd(&a.wsum_pdf_class[k], wslice);
+            atomicAdd(&a.wsum_pdf_dir[(size_t)k * a.D + dir], wslice);
+            atomicAdd(&a.wsum_scalars[0], t_off);
+        }
+    }
+}
+
+// Work items w = task * NOR + o over the blocks: each XCD takes one contiguous range (the grid is a
+// multiple of 8 blocks, so a block striding by the grid size stays on its XCD's range).  Device-driven
+// pools read the task total on the device; a pool whose slices exceed the slice buffer (slice_cap) goes
+// to the direct scatter instead, and an aborted pool adds nothing.
+template <int NOR, int NOT, int PPT, int NT>
+__global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) {
+    if (a.ntask_dev) {
+        if (*a.abort) return;
+        ntask = *a.ntask_dev;
+        if (ntask * NOR > a.slice_cap) return;
+    }
+    const int NH = (a.Pf + PPT * NT - 1) / (PPT * NT);  // blocks per slice (pixel ranges)
+    const long long G = ntask * NOR * NH, per = (G + 7) / 8;
+    for (long long L = blockIdx.x; (L >> 3) < per; L += gridDim.x) {
+        const long long w = (L & 7) * per + (L >> 3);
+        if (w < G) post_slice_one<NOR, NOT, PPT, NT>(a, w / NH, (int)(w % NH));
+        __syncthreads();  // LDS of the next work item
     }
 }
 
 // ------------------------------------------------------------------ storeWeightedSums (src/map3d_optimizer.cpp:1933-2057)
 __global__ void k_store_sums(StoreArgs a) {
     const int img = blockIdx.x * blockDim.x + threadIdx.x;
-    if (img >= a.n) return;
+    if (img >= a.n || (a.abort && *a.abort)) return;
     const int g = a.group[img];
     const double* pw = a.power + (size_t)img * a.oriF;
     double wn = a.wnorm[img];
@@ -1204,15 +1249,23 @@ hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, 
 
 hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4* cube, size_t grid_stride, int pad,
                                int padF, int c, int is3d, const double* Ai_fine, int R, int NOR, const int* ulist, int U,
-                               const int* xy, int Pf, int r_max, float2* Pfine) {
-    if (U == 0) return hipSuccess;
+                               const int* U_dev, int persist_blocks, const int* xy, int Pf, int r_max,
+                               float2* Pfine) {
     if (is3d && !cube) return hipErrorInvalidValue;  // 3D projections read the neighbourhood grid
-    const long long G = (long long)U * NOR * ((Pf + 255) / 256);
-    const long long blocks = 8 * ((G + 7) / 8);
-    const unsigned gx = (unsigned)std::min<long long>(blocks, 32768), gy = (unsigned)((blocks + gx - 1) / gx);
-    if (gy > 65535) return hipErrorInvalidValue;
-    k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, is3d ? cube : nullptr, grid_stride, pad, padF, c, is3d, Ai_fine, R,
-                                          NOR, ulist, U, xy, Pf, r_max, Pfine);
+    unsigned gx, gy;
+    if (U_dev) {  // device-driven: a fixed grid (a multiple of 8 blocks) strides over the device count
+        gx = (unsigned)(8 * std::max(1, persist_blocks / 8));
+        gy = 1;
+    } else {
+        if (U == 0) return hipSuccess;
+        const long long G = (long long)U * NOR * ((Pf + 255) / 256);
+        const long long blocks = 8 * ((G + 7) / 8);
+        gx = (unsigned)std::min<long long>(blocks, 32768);
+        gy = (unsigned)((blocks + gx - 1) / gx);
+        if (gy > 65535) return hipErrorInvalidValue;
+    }
+    k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, is3d ? cube : nullptr, grid_stride, pad, padF, c, is3d, Ai_fine,
+                                               R, NOR, ulist, U, U_dev, xy, Pf, r_max, Pfine);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1281,6 +1334,61 @@ hipError_t launch_coarse_gemm(hipStream_t s, const GemmArgs& a) {
     return hipSuccess;
 }
 
+// Coarse significance given by a caller: per image 1 .. cap keys in [0, MT).  An image that breaks this
+// keeps no key (its count becomes 0, so the task build reads nothing out of range) and flags[2], flags[0]
+// are set.
+__global__ __launch_bounds__(256) void k_check_given_keys(int n, const int* __restrict__ keys, int cap,
+                                                          int* __restrict__ cnt, long long MT, int* __restrict__ flags) {
+    __shared__ int bad;
+    const int img = blockIdx.x;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    const int c = cnt[img];
+    if (c < 1 || c > cap) {
+        if (threadIdx.x == 0) bad = 1;
+    } else {
+        for (int q = threadIdx.x; q < c; q += blockDim.x) {
+            const int k = keys[(size_t)img * cap + q];
+            if (k < 0 || (long long)k >= MT) bad = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) {
+        cnt[img] = 0;
+        atomicOr(&flags[2], 1);
+        atomicOr(&flags[0], 1);
+    }
+}
+
+// After the task build sorted the keys in place: two equal neighbours are a repeated sample (flags[2], [0]).
+__global__ __launch_bounds__(256) void k_check_sorted_dups(int n, const int* __restrict__ keys, int cap,
+                                                           const int* __restrict__ cnt, int* __restrict__ flags) {
+    const int img = blockIdx.x;
+    const int c = cnt[img];
+    const int* k = keys + (size_t)img * cap;
+    bool dup = false;
+    for (int q = threadIdx.x + 1; q < c; q += blockDim.x) dup |= k[q] == k[q - 1];
+    if (dup) {
+        atomicOr(&flags[2], 1);
+        atomicOr(&flags[0], 1);
+    }
+}
+
+hipError_t launch_check_given_keys(hipStream_t s, int n, const int* keys, int cap, int* cnt, long long MT,
+                                   int* flags) {
+    if (n <= 0) return hipSuccess;
+    k_check_given_keys<<<n, 256, 0, s>>>(n, keys, cap, cnt, MT, flags);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_check_sorted_dups(hipStream_t s, int n, const int* keys, int cap, const int* cnt, int* flags) {
+    if (n <= 0) return hipSuccess;
+    k_check_sorted_dups<<<n, 256, 0, s>>>(n, keys, cap, cnt, flags);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
 hipError_t launch_task_build(hipStream_t s, int n, const SigArgs& sig, int M, int T, int* ntask, int* nblk,
                              long long* task_off, long long* blk_off, unsigned char* used, int* ulist, int* m2u,
                              int* ucount) {
@@ -1313,11 +1421,18 @@ hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int
     return hipSuccess;
 }
 
-hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a) {
-    if (ntask == 0) return hipSuccess;
-    if (NOR == 8 && NOT == 4) k_fine_diff2<8, 4><<<ntask, 256, 0, s>>>(a);
-    else if (NOR == 2 && NOT == 4) k_fine_diff2<2, 4><<<ntask, 256, 0, s>>>(a);
-    else if (NOR == 1 && NOT == 1) k_fine_diff2<1, 1><<<ntask, 256, 0, s>>>(a);
+hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a, int persist_blocks) {
+    unsigned grid;
+    if (a.queue) {  // device-driven: a persistent grid on the work queue (head zeroed by the caller)
+        if (!a.ntask_dev || !a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
+        grid = (unsigned)persist_blocks;
+    } else {
+        if (ntask == 0) return hipSuccess;
+        grid = (unsigned)ntask;
+    }
+    if (NOR == 8 && NOT == 4) k_fine_diff2<8, 4><<<grid, 256, 0, s>>>(a);
+    else if (NOR == 2 && NOT == 4) k_fine_diff2<2, 4><<<grid, 256, 0, s>>>(a);
+    else if (NOR == 1 && NOT == 1) k_fine_diff2<1, 1><<<grid, 256, 0, s>>>(a);
     else return hipErrorInvalidValue;
     LAUNCH_CHECK();
     return hipSuccess;
@@ -1352,12 +1467,54 @@ hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a
     return hipSuccess;
 }
 
-hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a) {
-    if (ntask <= 0) return hipSuccess;
-    if (NOR != 8 || NOT != 4 || !a.slice_val || a.Pf > 8 * 1024 || !a.is3d) return hipErrorNotSupported;
-    const long long blocks = 8 * ((ntask * NOR + 7) / 8);
+bool post_slices_supported(int NOR, int NOT, int Pf, int is3d) {
+    return NOR == 8 && NOT == 4 && Pf <= 4 * 8 * 1024 && is3d;  // up to 4 pixel ranges of 8192 per slice
+}
+
+hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a, int persist_blocks) {
+    if (!post_slices_supported(NOR, NOT, a.Pf, a.is3d) || !a.slice_val) return hipErrorNotSupported;
+    long long blocks;
+    if (a.ntask_dev) {  // device-driven: a fixed grid (a multiple of 8 blocks)
+        if (!a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
+        blocks = 8 * std::max(1, persist_blocks / 8);
+    } else {
+        if (ntask <= 0) return hipSuccess;
+        const long long nh = (a.Pf + 8 * 1024 - 1) / (8 * 1024);
+        blocks = 8 * ((ntask * NOR * nh + 7) / 8);
+    }
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     k_post_slice<8, 4, 8, 1024><<<(unsigned)blocks, 1024, 0, s>>>(a, ntask);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+__global__ void k_fill32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill32(hipStream_t s, void* p, uint32_t value, size_t words) {
+    if (words == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 4096);
+    k_fill32<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), value, words);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+// Device-driven pools: flags[0] = 1 when an image's coarse list overflowed (status 8), it has no
+// significant sample (status 1), its significant samples exceed the key buffer, or the pool needs more
+// fine projections than the buffer holds; the pool is then rerun host-synchronised.
+__global__ void k_pool_check(int n, const int* __restrict__ status, const int* __restrict__ cnt, int cap,
+                             const int* __restrict__ ucount, int ucap, int* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < n) bad = (status[i] & 9) != 0 || (cnt && cnt[i] > cap);
+    if (i == 0) bad = bad || *ucount > ucap;
+    if (bad) atomicOr(&flags[0], 1);
+}
+
+hipError_t launch_pool_check(hipStream_t s, int n, const int* status, const int* cnt, int cap, const int* ucount,
+                             int ucap, int* flags) {
+    k_pool_check<<<(std::max(n, 1) + 255) / 256, 256, 0, s>>>(n, status, cnt, cap, ucount, ucap, flags);
     LAUNCH_CHECK();
     return hipSuccess;
 }

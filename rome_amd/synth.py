@@ -384,6 +384,102 @@ def make_particles_device(engine, nr_classes, n, size, ori, angpix, offset_range
     return F, ctf, truth, sigma2
 
 
+def lognormal_counts(n, median, sigma, lo, hi, rng):
+    """Significant coarse samples per image: lognormal with the given median (log-space sigma), rounded and
+    clipped to [lo, hi]."""
+    return np.clip(np.rint(median * np.exp(sigma * rng.standard_normal(n))), lo, hi).astype(np.int32)
+
+
+def clustered_significance(prob, truth, counts, cap=None, chunk=256):
+    """Coarse significance around each image's true pose, as an adaptive coarse pass leaves it: the
+    counts[i] (class, direction, psi, translation) samples of the true class nearest the true pose, by
+    (angle / psi step)^2 + (offset distance / offset step)^2.  Returns int32 keys [n][cap] (key = m * T + t,
+    m = (class * D + dir) * PSI + psi; unused slots -1) and counts [n]."""
+    K, D, PSI, T = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"], prob["nr_trans"]
+    n = len(counts)
+    cap = int(cap or max(1, int(np.max(counts))))
+    counts = np.minimum(np.asarray(counts, np.int64), min(cap, D * PSI * T))
+    pc = np.asarray(prob["pose_coarse"], float)
+    if prob["is_3d"]:
+        Rc = np.stack([euler_matrix(*a) for a in pc.reshape(-1, 3)])  # [D*PSI][3][3]
+    else:
+        Rc = np.stack([euler_matrix(0.0, 0.0, a) for a in pc.reshape(-1)])
+    tx, ty = np.asarray(prob["trans_x"], float), np.asarray(prob["trans_y"], float)
+    ang_step = 360.0 / PSI
+    off_step = max(float(np.min(np.abs(tx[tx != 0]))) if np.any(tx != 0) else 1.0, 1e-6)
+    keys = np.full((n, cap), -1, np.int32)
+    R = D * PSI
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        Rt = np.stack([euler_matrix(*truth[i, 1:4]) if prob["is_3d"] else euler_matrix(0.0, 0.0, truth[i, 3])
+                       for i in range(c0, c1)])
+        cosang = np.clip((np.einsum("nij,rij->nr", Rt, Rc) - 1.0) / 2.0, -1.0, 1.0)
+        ang = np.degrees(np.arccos(cosang)) / ang_step  # [chunk][R]
+        for j, i in enumerate(range(c0, c1)):
+            c = int(counts[i])
+            d_off = np.hypot(tx - truth[i, 4], ty - truth[i, 5]) / off_step
+            nr = min(R, max(16, 4 * c))
+            rsel = np.argpartition(ang[j], nr - 1)[:nr]
+            score = ang[j][rsel][:, None] ** 2 + d_off[None, :] ** 2  # [nr][T]
+            flat = np.argpartition(score.ravel(), c - 1)[:c]
+            r_idx, t_idx = rsel[flat // T], flat % T
+            k = int(truth[i, 0])
+            keys[i, :c] = np.sort(((k * R + r_idx) * T + t_idx).astype(np.int64)).astype(np.int32)
+    return keys, counts.astype(np.int32)
+
+
+def clustered_significance_device(prob, truth, counts, cap, chunk=64):
+    """Device (torch) form of clustered_significance for large pools: int32 keys [n][cap] (unused -1) and
+    counts [n] as cuda tensors.  Ties in the score may pick other samples than the host form."""
+    import torch
+    dev = torch.device("cuda:0")
+    K, D, PSI, T = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"], prob["nr_trans"]
+    R = D * PSI
+    pc = np.asarray(prob["pose_coarse"], float).reshape(-1, 3)
+    Rc = torch.tensor(np.stack([euler_matrix(*a) for a in pc]).reshape(R, 9), dtype=torch.float32, device=dev)
+    tx = torch.tensor(np.asarray(prob["trans_x"], float), dtype=torch.float32, device=dev)
+    ty = torch.tensor(np.asarray(prob["trans_y"], float), dtype=torch.float32, device=dev)
+    txn = np.asarray(prob["trans_x"], float)
+    off_step = float(np.min(np.abs(txn[txn != 0]))) if np.any(txn != 0) else 1.0
+    ang_step = 360.0 / PSI
+    n = len(counts)
+    counts = np.minimum(np.asarray(counts, np.int64), min(cap, R * T))
+    keys = torch.full((n, cap), -1, dtype=torch.int32, device=dev)
+    tr = np.asarray(truth, float)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        Rt = torch.tensor(np.stack([euler_matrix(*tr[i, 1:4]) for i in range(c0, c1)]).reshape(-1, 9),
+                          dtype=torch.float32, device=dev)
+        cosang = ((Rt @ Rc.T - 1.0) / 2.0).clamp(-1.0, 1.0)
+        ang = torch.rad2deg(torch.arccos(cosang)) / ang_step  # [c][R]
+        cmax = int(counts[c0:c1].max())
+        nr = min(R, max(16, 4 * cmax))
+        rang, rsel = torch.topk(ang, nr, dim=1, largest=False)
+        sx = torch.tensor(tr[c0:c1, 4], dtype=torch.float32, device=dev)[:, None]
+        sy = torch.tensor(tr[c0:c1, 5], dtype=torch.float32, device=dev)[:, None]
+        doff = torch.hypot(tx[None, :] - sx, ty[None, :] - sy) / off_step  # [c][T]
+        score = (rang[:, :, None] ** 2 + doff[:, None, :] ** 2).reshape(c1 - c0, -1)
+        _, idx = torch.topk(score, cmax, dim=1, largest=False, sorted=True)
+        r = torch.gather(rsel, 1, idx // T)
+        t = idx % T
+        k = torch.tensor(tr[c0:c1, 0].astype(np.int64), device=dev)[:, None]
+        kk = ((k * R + r) * T + t).to(torch.int32)
+        keep = torch.arange(cmax, device=dev)[None, :] < torch.tensor(counts[c0:c1], device=dev)[:, None]
+        kk = torch.where(keep, kk, torch.full_like(kk, -1))
+        keys[c0:c1, :cmax] = kk
+    return keys, torch.tensor(counts.astype(np.int32), device=dev)
+
+
+def significance_mask(prob, keys, counts):
+    """The dense coarse significance [n][K][D][PSI][T] (uint8) of per-image keys."""
+    K, D, PSI, T = prob["nr_classes"], prob["nr_dir"], prob["nr_psi"], prob["nr_trans"]
+    n = len(counts)
+    mask = np.zeros((n, K * D * PSI * T), np.uint8)
+    for i in range(n):
+        mask[i, keys[i, :counts[i]]] = 1
+    return mask.reshape(n, K, D, PSI, T)
+
+
 def engine_config(prob, precision="fp32"):
     keys = ("ori_size", "current_size", "coarse_size", "nr_classes", "nr_groups", "adaptive_oversampling",
             "nr_dir", "nr_psi", "nr_trans", "adaptive_fraction", "sigma2_fudge", "sigma2_offset",

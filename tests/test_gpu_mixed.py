@@ -25,6 +25,8 @@ from test_gpu_parity import _case, _run_both, check_argmax_near_ties, zero_prior
 pytestmark = pytest.mark.gpu
 
 U = {"bf16": 2.0 ** -8, "fp16": 2.0 ** -11}
+# nats below the low-precision significance threshold within which coarse samples are recomputed exactly
+MARGIN = {"bf16": 12.0, "fp16": 2.0}
 CASES = ["3d", "3d_window_groups_offsets", "2d_two_pass", "3d_full_map3d", "3d_paired_k1", "3d_paired_k1_zero_prior",
          "3d_zero_prior"]
 FULL_SIZE = ("3d_full_map3d", "3d_paired_k1", "3d_paired_k1_zero_prior")  # no dense fine dumps
@@ -67,6 +69,12 @@ def test_mixed_precision_estep(engine, case, prec):
         assert err <= 4 * u * scale, f"{name} image {i}: coarse error {err:.3e} vs bound {4 * u * scale:.3e}"
         worst = max(worst, err / scale)
     assert worst > u / 256, f"coarse error {worst:.3e} too small for {prec}: FP32 path ran?"
+    # The recompute margin has to cover the error of the samples near the weights: the threshold's and a
+    # sample's own error, each at most the largest error among the samples within reach of the minimum.
+    near = [np.max(np.abs(d[i] - o[i])[~np.isnan(o[i]) & (o[i] < np.nanmin(o[i]) + MARGIN[prec] + 30)])
+            for i in range(n)]
+    print(f"[mixed case {name} {prec}] coarse error near the weights: max {max(near):.3f} nat", flush=True)
+    assert 2 * max(near) < MARGIN[prec], (name, prec, near)
     # poses
     if ref.get("fine_weight") is not None:
         fw = ref["fine_weight"]
@@ -77,7 +85,15 @@ def test_mixed_precision_estep(engine, case, prec):
             assert fw[i][k, dd, ps, t, o_, ot] >= np.nanmax(fw[i]) * (1 - 1e-3), f"{name} image {i}"
     else:
         check_argmax_near_ties(engine, prob, batch, r, ref, name)
-    np.testing.assert_allclose(r["meta"][:, 0], ref["meta"][:, 0], rtol=1e-3)
+    # PMAX = max / sum of the fine weights.  With the oracle's significant set (same NR_SIGN) the fine sums
+    # agree to FP32 rounding: 1e-3.  Where the 0.999-mass cut fell one side or the other of a coarse sample
+    # whose recomputed weight sits at the threshold, the fine sum gains or loses that sample's fine
+    # weights, a share of order 1 - adaptive_fraction of the sum: 5e-3, on at most a tenth of the images
+    # (at least one).
+    same_set = r["nr_sign"] == ref["nr_sign"]
+    assert np.sum(~same_set) <= max(1, n // 10), (r["nr_sign"], ref["nr_sign"])
+    np.testing.assert_allclose(r["meta"][same_set, 0], ref["meta"][same_set, 0], rtol=1e-3)
+    np.testing.assert_allclose(r["meta"][:, 0], ref["meta"][:, 0], rtol=5e-3)
     same = np.all(r["argmax"] == ref["argmax"], axis=1)
     np.testing.assert_allclose(r["meta"][same, 1:7], ref["meta"][same, 1:7], rtol=1e-9, atol=1e-9)
     # sums and maps
