@@ -247,6 +247,13 @@ int rome_project(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9
 /* Projector into a device buffer (float2 [nrot][size*(size/2+1)]): synthetic particles. */
 int rome_project_device(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, void* out_device);
 
+/* Allocation-free form of rome_project_device (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The rotations are staged in a fixed device array of the library (16384 matrices at most; more is
+ * ROME_ERR_ARG); no hipMalloc or hipFree is called.  Every call on one device shares that array, so
+ * calls that overlap on one device (two host threads, two contexts) race. */
+int rome_project_device_noalloc(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, void* out_device);
+
 /* Execution of a pool (no counterpart in the reference, whose pools run as OpenMP loops):
  * ROME_EXEC_HOST   the host reads the fine pass's task totals back after the coarse pass and sizes every
  *                  later launch from them (two host synchronisations per pool);

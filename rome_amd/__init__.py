@@ -30,7 +30,7 @@ EXPORTED = (
     "rome_set_significance_cap", "rome_get_significance_stats", "rome_proposal_configure", "rome_proposal_mode",
     "rome_proposal_train", "rome_proposal_stats", "rome_proposal_get_params", "rome_proposal_set_params",
     "rome_proposal_forward", "rome_proposal_gradients", "rome_proposal_training_rows", "rome_backproject_images",
-    "rome_set_execution", "rome_get_execution_stats", "rome_expectation_fine_device",
+    "rome_project_device_noalloc", "rome_set_execution", "rome_get_execution_stats", "rome_expectation_fine_device",
     "rome_autotune", "rome_set_persistent_grids", "rome_proposal_set_thresholds",
 )
 
@@ -400,6 +400,12 @@ class Engine:
         A = _f64(A).reshape(-1, 9)
         self._check(lib().rome_project_device(self._h, iclass, size, A.shape[0], _p(A), C.c_void_p(out_ptr)),
                     "rome_project_device")
+
+    def project_device_noalloc(self, iclass, size, A, out_ptr):
+        """rome_project_device without device allocation (rotations staged in the library's fixed array)."""
+        A = _f64(A).reshape(-1, 9)
+        self._check(lib().rome_project_device_noalloc(self._h, iclass, size, A.shape[0], _p(A), C.c_void_p(out_ptr)),
+                    "rome_project_device_noalloc")
 
     def set_profiling(self, on=True):
         self._check(lib().rome_set_profiling(self._h, int(on)), "rome_set_profiling")

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["kernels.hip", "gemm_lowp.hip", "weights.hip", "backproject.hip", "fourier_map.hip", "reconstruct.hip", "imageprep.hip",
-           "proposal.hip",
+           "proposal.hip", "noalloc.hip",
            "engine.cpp", "sampling.cpp", "mstep.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",

@@ -1740,6 +1740,11 @@ int rome_create(int device, rome_ctx** out) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         ctx->n_cu = ncu;
+    // the device address of the allocation-free entry point's staging array, once per device
+    if (noalloc_init(device) != hipSuccess) {
+        rome_destroy(ctx);
+        return ROME_ERR_HIP;
+    }
     // 5000-entry sin/cos table of shiftImageInFourierTransform (src/image.h:1486-1510)
     std::vector<double> st(5000), ct(5000);
     const double step = 2 * 3.14159265358979323846 / 5000.0;
@@ -2366,6 +2371,22 @@ int rome_project_device(rome_ctx* ctx, int iclass, int size, int nrot, const dou
     HIPCHK(dai.upload(ai.data(), ai.size(), ctx->stream));
     HIPCHK(launch_project_plane(ctx->stream, ctx->grids.p + ctx->grid_n * iclass, ctx->pad, ctx->padF, ctx->cgrid,
                                 ctx->is3d, dai.p, nrot, size, std::min(ctx->r_max, size / 2), (float2*)out));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ROME_OK;
+}
+
+int rome_project_device_noalloc(rome_ctx* ctx, int iclass, int size, int nrot, const double* A9, void* out) {
+    if (!ctx || !A9 || !out || nrot < 0) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass < 0 || iclass >= ctx->K || !ctx->have_ref[iclass]) return fail(ctx, ROME_ERR_ARG, "class not set");
+    if (size <= 0 || (size & 1) || size / 2 > ctx->r_max) return fail(ctx, ROME_ERR_ARG, "bad projection size");
+    if (nrot > kNoallocMaxRot)
+        return fail(ctx, ROME_ERR_ARG, "%d rotations exceed the %d the allocation-free projection stages", nrot,
+                    kNoallocMaxRot);
+    if (nrot == 0) return ROME_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(project_plane_noalloc(ctx->stream, ctx->device, A9, nrot, ctx->grids.p + ctx->grid_n * iclass, ctx->pad,
+                                 ctx->padF, ctx->cgrid, ctx->is3d, size, std::min(ctx->r_max, size / 2), (float2*)out));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return ROME_OK;
 }

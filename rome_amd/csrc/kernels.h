@@ -364,6 +364,12 @@ hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4*
                                float2* Pfine);
 hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
                                 int nrot, int size, int my_r_max, float2* out);
+// Allocation-free projection (noalloc.hip): rotations staged in a file-scope device array of
+// kNoallocMaxRot matrices; noalloc_init(device) takes its per-device address once (at context creation).
+constexpr int kNoallocMaxRot = 16384;
+hipError_t noalloc_init(int device);
+hipError_t project_plane_noalloc(hipStream_t s, int device, const double* A9_host, int nrot, const float2* grid,
+                                 int pad, int padF, int c, int is3d, int size, int my_r_max, float2* out);
 hipError_t launch_prep_images(hipStream_t s, int n, const PrepArgs& a);
 // Qc[p][m] = |FcT[p][m]|^2 (Re and Im rows), zero for p in [Pc, Kq); Kq = Pc rounded up to 16
 hipError_t launch_fct_norm(hipStream_t s, const float* FcT, int Pc, int Pc_pad, int Mpad, int Kq, float* Qc);
