@@ -18,6 +18,7 @@ when its max-posterior poses agree with the FP32 arm's on >= 99.5 % of the parti
     python bench.py [--steps K] [--warmup W] [--batch B] [--n-images N] [--precision bf16|fp16|fp32]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -179,13 +180,22 @@ def pose_differences(meta_a, meta_b, amax_a, amax_b):
             "pmax_mixed_median": round(float(np.median(r[:, 4])), 4)}
 
 
+def quiet_gc():
+    """Python's cyclic collector off for a timed loop (re-enabled after it): with torch imported a full
+    collection scans millions of objects and stalls one step by tens of milliseconds, a pause of this
+    driver script that a C or C++ caller of the library does not have."""
+    gc.collect()
+    gc.freeze()
+    gc.disable()
+
+
 def adaptive_leg(args, log):
     """The batched adaptive fine pass (one GPU's share of it): 192^2 particles, K=8 blob volumes 192^3, coarse
     HEALPix order 3 x 48 psi (36864 rotations per class) x 49 translations, oversampling 1; per particle a
     lognormal number of significant coarse samples (median 300, clipped to 1-5000) around its true pose.
     Each pool is copied into one of two staging slots (the way a caller hands over consecutive pools) and
     its fine pass runs from that significance (rome_expectation_fine_device) as a replayed hipGraph of the
-    device-driven pool, the fine diff2 on its work queue, the persistent grids autotuned on the first pool."""
+    device-driven pool (the fine diff2 a persistent work-queue kernel), its grids autotuned on the first pool."""
     import torch
     import rome_amd
     from rome_amd import synth
@@ -256,6 +266,7 @@ def adaptive_leg(args, log):
     e.reset_sums()
     log(f"[adaptive] data {t_gen:.1f}s, grids {grids}", e.execution_stats())
     torch.cuda.synchronize()
+    quiet_gc()
     t0 = time.perf_counter()
     last = []
     for i in range(nb):
@@ -263,6 +274,7 @@ def adaptive_leg(args, log):
         last.append(e.execution_stats()["last"])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    gc.enable()
     st = e.batch_stats()
     # stage times of the same pools, device-driven with stage timers (outside the timed pass)
     e.set_profiling(True)
@@ -312,7 +324,9 @@ def main():
                     help="pool execution: host-synchronised, device-driven, or device-driven captured as hipGraphs")
     ap.add_argument("--adaptive-n", type=int, default=0,
                     help="particles of the batched adaptive fine-pass leg (192^2, K=8, hp3; 0: not run)")
-    ap.add_argument("--adaptive-batch", type=int, default=1000)
+    ap.add_argument("--adaptive-batch", type=int, default=250,
+                    help="particles per pool of that leg (a pool's fine projections and backprojection slices "
+                         "fit the device-driven capacities up to about 400)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -410,18 +424,23 @@ def main():
                 step(i)
             log(f"[{prec}] graphs", eng.execution_stats())
         torch.cuda.synchronize()
+        quiet_gc()
         t0 = time.perf_counter()
         res = []
         full_batches = 0  # batches whose coarse list overflowed (the pool then ran on the full pose term)
         execs = []
+        step_ms = []
         for i in range(args.steps):
+            ts = time.perf_counter()
             res.append(step(args.warmup + i))
+            step_ms.append((time.perf_counter() - ts) * 1e3)
             full_batches += eng.coarse_stats()["mode"] != 0
             execs.append(eng.execution_stats()["last"])
             if args.verbose:
                 log(f"[{prec}] step {i} done", eng.batch_stats(), eng.coarse_stats(), execs[-1])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        gc.enable()
         # stage times (HIP events on the engine stream) from a second, profiled pass over the same batches:
         # stage timers run the pools device-driven without the graph, so they stay out of the timed region
         eng.set_profiling(True)
@@ -445,6 +464,7 @@ def main():
         return dict(value=args.steps * B / dt, dt=dt, prof=prof, stats={**eng.batch_stats(), **eng.coarse_stats()},
                     amax=amax, meta=meta, truth_agree=float(np.mean(agree)), full_batches=full_batches,
                     proposal=pstats, execution={"mode": args.execution, "last_per_step": execs,
+                                                "step_ms": [round(x, 2) for x in step_ms],
                                                 **eng.execution_stats()})
 
     arms = {} if (args.only_mixed and args.precision != "fp32") else {"fp32": run_arm("fp32")}

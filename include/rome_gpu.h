@@ -168,6 +168,12 @@ int rome_set_sampling(rome_ctx* ctx, const double* pose_coarse, const double* po
 int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_class,
                    const double* pdf_direction, const double* prior_offx, const double* prior_offy);
 
+/* Classes the following pools run: exp_iclass_min .. exp_iclass_max of expectationSomeParticles, which the
+ * reference narrows to one class per pool in the seeding iteration (src/map3d_optimizer.cpp:823-840;
+ * src/map2d_optimizer.cpp:731-758); other classes get no diff2, weight or sums, as classes without prior.
+ * (-1, -1): every class (the default; rome_configure resets it). */
+int rome_set_class_range(rome_ctx* ctx, int iclass_min, int iclass_max);
+
 /* MyProjector::data of one class after computeFourierTransformMap
  * (src/reconstruct.cpp:106-223): complex interleaved FP64 [pad][pad][pad/2+1] (3D) or
  * [pad][pad/2+1] (2D), pad = 2*(2*min(cur/2,ori/2)+1)+1. */
@@ -258,9 +264,10 @@ int rome_project_device_noalloc(rome_ctx* ctx, int iclass, int size, int nrot, c
  * ROME_EXEC_HOST   the host reads the fine pass's task totals back after the coarse pass and sizes every
  *                  later launch from them (two host synchronisations per pool);
  * ROME_EXEC_DEVICE the fine pass runs from buffers sized by capacities (significant samples per image x
- *                  pool size) with its counts read on the device: persistent kernels stride over them, the
- *                  fine diff2 takes its (image, rotation) tasks from a work queue; one synchronisation per
- *                  pool, when the results are read back;
+ *                  pool size) with its counts read on the device: fixed grids (sized from the counts of
+ *                  earlier pools or by rome_autotune) stride over them, the fine diff2's tasks past its
+ *                  first launch go to a persistent work-queue kernel; one synchronisation per pool, when
+ *                  the results are read back;
  * ROME_EXEC_GRAPH  (default) as DEVICE, and the pool is captured once as a hipGraph per (pool size, input
  *                  arrays) and replayed for later pools with the same inputs.
  * A pool whose coarse list or significant-sample keys overflow, or that needs the dense significance mask,
@@ -268,19 +275,22 @@ int rome_project_device_noalloc(rome_ctx* ctx, int iclass, int size, int nrot, c
  * same in every mode. */
 enum { ROME_EXEC_HOST = 0, ROME_EXEC_DEVICE = 1, ROME_EXEC_GRAPH = 2 };
 int rome_set_execution(rome_ctx* ctx, int mode);
-/* out4: how the last pool ran (0 host, 1 device, 2 graph replay; +8 when a device-driven pool was rerun
- * host-synchronised), graph launches, graph captures, device-driven reruns. */
-int rome_get_execution_stats(rome_ctx* ctx, long long* out4);
+/* out6: how the last pool ran (0 host, 1 device, 2 graph replay; +8 when a device-driven pool was rerun
+ * host-synchronised), graph launches, graph captures, device-driven reruns, and for the last device-driven
+ * pool the host time (microseconds) spent issuing it and waiting for its results. */
+int rome_get_execution_stats(rome_ctx* ctx, long long* out6);
 
 /* Tile / grid autotuner of the device-driven kernels (no counterpart in the reference): runs the pool
  * `images` (with sig_keys / sig_cnt / cap: its fine pass from that significance, as
- * rome_expectation_fine_device; NULL: the whole E-step) device-driven with 4 candidate persistent grids for each of the fine diff2 work queue, the
- * fine projection, the per-slice post kernel and the binned backprojection (blocks per CU: {4, 8, 12,
- * 16}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}), twice each, times their stages with HIP events and
+ * rome_expectation_fine_device; NULL: the whole E-step) device-driven with 4 candidate grids for each of the fine diff2's work-queue kernel, the
+ * fine projection, the per-slice post kernel and the binned backprojection (blocks per CU: {1, 2, 3,
+ * 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}), twice each, times their stages with HIP events and
  * keeps the fastest grid of each kernel for the context.  chosen [4] receives the grids (blocks),
  * stage_ms [4][4] the measured stage times per candidate.  The runs add to the E-step sums: call
  * rome_reset_sums afterwards.  rome_set_persistent_grids sets the four grids directly (0: the default,
- * 8, 8, 2 and 8 blocks per CU). */
+ * 2, 8, 2 and 8 blocks per CU) and forgets the task and rotation totals of earlier pools, from which
+ * the first launches of the fine diff2, the projection and the post slices are sized (the next pool's
+ * are then sized from 4 per image, and what exceeds them goes through the work queue / the tails). */
 int rome_autotune(rome_ctx* ctx, const rome_device_images* images, const int* sig_keys, const int* sig_cnt, int cap,
                   int* chosen, double* stage_ms);
 int rome_set_persistent_grids(rome_ctx* ctx, const int* blocks);
@@ -403,7 +413,8 @@ int rome_proposal_set_thresholds(rome_ctx* ctx, int top_k, double min_mass, doub
  * 32), shuffled per epoch; loss_per_epoch [epochs] (may be NULL) receives each epoch's mean loss. */
 int rome_proposal_train(rome_ctx* ctx, int epochs, int batch, double lr, double* loss_per_epoch);
 /* out8: pools, images, mean candidate rotations per image, images that took every rotation, audited
- * images, lowest audited captured mass, pools rerun exhaustively, training images. */
+ * images, lowest audited captured mass, pools run exhaustively (an audited captured mass below
+ * min_audit_mass, or more than half of the pool's images below min_mass), training images. */
 int rome_proposal_stats(rome_ctx* ctx, double* out8);
 
 /* Parameters in host FP32, row-major: W1 [hidden][2 nr_coef], W2, W3 [hidden][hidden],

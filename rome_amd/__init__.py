@@ -31,7 +31,7 @@ EXPORTED = (
     "rome_proposal_train", "rome_proposal_stats", "rome_proposal_get_params", "rome_proposal_set_params",
     "rome_proposal_forward", "rome_proposal_gradients", "rome_proposal_training_rows", "rome_backproject_images",
     "rome_project_device_noalloc", "rome_set_execution", "rome_get_execution_stats", "rome_expectation_fine_device",
-    "rome_autotune", "rome_set_persistent_grids", "rome_proposal_set_thresholds",
+    "rome_autotune", "rome_set_persistent_grids", "rome_proposal_set_thresholds", "rome_set_class_range",
 )
 
 EXEC_HOST, EXEC_DEVICE, EXEC_GRAPH = 0, 1, 2
@@ -558,6 +558,10 @@ class Engine:
         self._check(lib().rome_get_coarse_stats(self._h, _p(out, C.c_longlong)), "rome_get_coarse_stats")
         return dict(zip(("mode", "list_max", "list_total"), out.tolist()))
 
+    def set_class_range(self, iclass_min=-1, iclass_max=-1):
+        """Classes the next pools run (the seeding iteration's single class per pool); (-1, -1): all."""
+        self._check(lib().rome_set_class_range(self._h, int(iclass_min), int(iclass_max)), "rome_set_class_range")
+
     def set_execution(self, mode):
         """How pools run: "host" (host-synchronised launches), "device" (device-driven fine pass, one
         synchronisation per pool) or "graph" (device-driven, captured once per input set and replayed)."""
@@ -583,9 +587,9 @@ class Engine:
         self._check(lib().rome_set_persistent_grids(self._h, _p(b, C.c_int)), "rome_set_persistent_grids")
 
     def execution_stats(self):
-        out = np.zeros(4, np.int64)
+        out = np.zeros(6, np.int64)
         self._check(lib().rome_get_execution_stats(self._h, _p(out, C.c_longlong)), "rome_get_execution_stats")
-        return dict(zip(("last", "graph_launches", "graph_captures", "reruns"), out.tolist()))
+        return dict(zip(("last", "graph_launches", "graph_captures", "reruns", "launch_us", "wait_us"), out.tolist()))
 
     def set_significance_cap(self, cap=-1):
         """Significant samples per image kept as keys (<= 4096); 0: the dense mask always; < 0: default."""

@@ -101,18 +101,20 @@ DEV void bp_count_one(const BpArgs& a, long long s, int pb, unsigned* keys, unsi
         if (keys[h]) atomicAdd(&a.count[keys[h] - 1], cnt[h]);
 }
 
-// blockIdx.x = slice, blockIdx.y = pixel block; device-driven: a fixed 1-D grid strides over both
+// blockIdx.x = slice, blockIdx.y = pixel block; device-driven (Q): a fixed grid of the same shape whose
+// blocks stride over the slices of the slice total on the device.  No work counter: a device-scope
+// atomic on one address serialises about a million items a pool.
+template <bool Q>
 __global__ __launch_bounds__(256) void k_bp_count(BpArgs a, long long S_host) {
     __shared__ unsigned keys[kHash], cnt[kHash];
-    if (!a.ntask_dev) {
+    if constexpr (!Q) {
         bp_count_one(a, blockIdx.x, blockIdx.y, keys, cnt);
         return;
     }
     const long long S = bp_slices(a, S_host);
-    const int nPB = (a.Pf + 255) / 256;
-    for (long long L = blockIdx.x; L < S * nPB; L += gridDim.x) {
-        bp_count_one(a, L / nPB, (int)(L % nPB), keys, cnt);
-        __syncthreads();  // the hash table of the next item
+    for (long long sl = blockIdx.x; sl < S; sl += gridDim.x) {  // slices strided over the grid
+        bp_count_one(a, sl, (int)blockIdx.y, keys, cnt);
+        __syncthreads();  // the hash table of the next slice
     }
 }
 
@@ -200,17 +202,17 @@ DEV void bp_fill_one(const BpArgs& a, long long s, int pb, unsigned* keys, unsig
     if (h >= 0) a.rec[base[h] + rank] = ((unsigned long long)s << 32) | (unsigned)p;
 }
 
+template <bool Q>
 __global__ __launch_bounds__(256) void k_bp_fill(BpArgs a, long long S_host) {
     __shared__ unsigned keys[kHash], cnt[kHash], base[kHash];
-    if (!a.ntask_dev) {
+    if constexpr (!Q) {
         bp_fill_one(a, blockIdx.x, blockIdx.y, keys, cnt, base);
         return;
     }
     const long long S = bp_slices(a, S_host);
-    const int nPB = (a.Pf + 255) / 256;
-    for (long long L = blockIdx.x; L < S * nPB; L += gridDim.x) {
-        bp_fill_one(a, L / nPB, (int)(L % nPB), keys, cnt, base);
-        __syncthreads();  // the hash table of the next item
+    for (long long sl = blockIdx.x; sl < S; sl += gridDim.x) {  // slices strided over the grid
+        bp_fill_one(a, sl, (int)blockIdx.y, keys, cnt, base);
+        __syncthreads();  // the hash table of the next slice
     }
 }
 
@@ -340,15 +342,16 @@ DEV void bp_brick_one(const BpArgs& a, int nb, unsigned ck) {
 }
 
 // One block per chunk (the grid sized for the most chunks there can be), or (device-driven pools) a
-// fixed grid striding over the chunk total the scan left in coff[nb].
+// fixed grid striding over the chunks, up to the total the scan left in coff[nb].
+template <bool Q>
 __global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
-    if (!a.ntask_dev) {
+    if constexpr (!Q) {
         if (blockIdx.x < a.coff[nb]) bp_brick_one(a, nb, blockIdx.x);
         return;
     }
     if (*a.abort) return;
     const unsigned nck = a.coff[nb];
-    for (unsigned ck = blockIdx.x; ck < nck; ck += gridDim.x) {
+    for (unsigned ck = blockIdx.x; ck < nck; ck += gridDim.x) {  // chunks strided over the grid
         bp_brick_one(a, nb, ck);
         __syncthreads();  // the box of the next chunk
     }
@@ -359,26 +362,28 @@ __global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
 hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks) {
     const int nb = a.nb_class * a.K;
     hipError_t e;
-    if (a.ntask_dev) {  // device-driven: fixed grids over the device counts (a zero count does nothing)
+    if (a.ntask_dev) {  // device-driven: fixed grids striding over the device counts
         if (!a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
         if ((unsigned long long)a.slice_cap * a.Pf >= (1ULL << 32)) return hipErrorInvalidValue;
         if ((e = launch_fill32(s, a.count, 0u, (size_t)nb)) != hipSuccess) return e;
-        k_bp_count<<<persist_blocks, 256, 0, s>>>(a, 0);
+        const int nPB = (a.Pf + 255) / 256;
+        const dim3 g2((unsigned)std::max(1, persist_blocks / nPB), (unsigned)nPB);
+        k_bp_count<true><<<g2, 256, 0, s>>>(a, 0);
         k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
-        k_bp_fill<<<persist_blocks, 256, 0, s>>>(a, 0);
-        k_bp_brick<<<persist_blocks, 256, 0, s>>>(a, nb);
+        k_bp_fill<true><<<g2, 256, 0, s>>>(a, 0);
+        k_bp_brick<true><<<persist_blocks, 256, 0, s>>>(a, nb);
         return hipGetLastError();
     }
     if (S <= 0) return hipSuccess;
     if (S > 0x7fffffffLL || (unsigned long long)S * a.Pf >= (1ULL << 32)) return hipErrorInvalidValue;
     if ((e = launch_fill32(s, a.count, 0u, (size_t)nb)) != hipSuccess) return e;
     const dim3 grid((unsigned)S, (a.Pf + 255) / 256);
-    k_bp_count<<<grid, 256, 0, s>>>(a, S);
+    k_bp_count<false><<<grid, 256, 0, s>>>(a, S);
     k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
-    k_bp_fill<<<grid, 256, 0, s>>>(a, S);
+    k_bp_fill<false><<<grid, 256, 0, s>>>(a, S);
     // at most one chunk per brick plus one per kChunk records
     const unsigned long long chunks = (unsigned long long)nb + ((unsigned long long)S * a.Pf + kChunk - 1) / kChunk;
-    k_bp_brick<<<(unsigned)chunks, 256, 0, s>>>(a, nb);
+    k_bp_brick<false><<<(unsigned)chunks, 256, 0, s>>>(a, nb);
     return hipGetLastError();
 }
 

@@ -8,12 +8,14 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -151,7 +153,7 @@ struct ProposalState {
     DevBuf<int> audit_emin;
     DevBuf<double> audit_mass;
     long long audit_pos = 0;
-    long long st_pools = 0, st_images = 0, st_all = 0, st_audited = 0, st_fallback = 0;
+    long long st_pools = 0, st_images = 0, st_all = 0, st_audited = 0, st_fallback = 0, st_uncertain = 0;
     double st_cand = 0., st_min_mass = 1.;
     unsigned long long rng = 0;
 };
@@ -318,11 +320,35 @@ struct rome_ctx {
         const int* cnt = nullptr;   // device [n]
         int cap = 0;
     } given;
+    int class_lo = 0, class_hi = 1 << 30;  // classes a pool runs (rome_set_class_range)
     std::vector<PoolGraph> graphs;  // most recently used last
     long long graph_launches = 0, graph_captures = 0, dd_reruns = 0;
+    double last_launch_ms = 0., last_wait_ms = 0.;  // host time issuing the last device-driven pool / waiting
     static constexpr size_t kPfineBytes = (size_t)32 << 30;  // fine projections of a device-driven pool
+    // pool_flags: [0] abort, [1] fine diff2 queue head, [2] invalid given keys, [3] unused, then 64-bit
+    // work counters from word 4: 8 per-XCD of the fine projection's tail, 8 of the post slices' tail
+    static constexpr int kPoolFlagWords = 4 + 2 * 16;
+    unsigned long long* work_counters(int first) const {
+        return reinterpret_cast<unsigned long long*>(pool_flags.p + 4) + first;
+    }
+    // Device-driven launches: the first launch of the projection and the post slices covers the counts of
+    // the pools before (25 % above the largest of them, decaying), a tail kernel the rest.  Without
+    // history: 4 tasks / rotations per image.  Bounded by the capacities.
+    double hist_tasks = 0., hist_U = 0.;
+    void note_counts(long long tasks, long long u) {
+        hist_tasks = std::max((double)tasks, 0.9 * hist_tasks);
+        hist_U = std::max((double)u, 0.9 * hist_U);
+    }
+    long long guess_tasks(int n, long long cap) const {
+        const double g = hist_tasks > 0. ? 1.25 * hist_tasks + 64 : 4.0 * n;
+        return std::max<long long>(1, std::min<long long>(cap, (long long)g));
+    }
+    int guess_U(int n, long long cap) const {
+        const double g = hist_U > 0. ? 1.25 * hist_U + 64 : 4.0 * n;
+        return (int)std::max<long long>(1, std::min<long long>(cap, (long long)g));
+    }
     int persist_blocks(int i) const {
-        static const int per_cu[4] = {8, 8, 2, 8};
+        static const int per_cu[4] = {2, 8, 2, 8};
         return persist[i] > 0 ? persist[i] : n_cu * per_cu[i];
     }
     // unique rotations a device-driven pool holds fine projections for (a pool that needs more aborts)
@@ -797,8 +823,9 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
     const rome_config& c = ctx->cfg;
     hipStream_t s = ctx->stream;
     if (dd || ctx->given.keys) {
-        HIPCHK(ctx->pool_flags.reserve(4));
-        HIPCHK(launch_fill32(s, ctx->pool_flags.p, 0u, 4));  // abort flag, work-queue head, invalid keys
+        HIPCHK(ctx->pool_flags.reserve(rome_ctx::kPoolFlagWords));
+        // abort flag, fine diff2 queue head, invalid keys; the work counters of the device-driven kernels' tails
+        HIPCHK(launch_fill32(s, ctx->pool_flags.p, 0u, rome_ctx::kPoolFlagWords));
     }
     if (dd) dout->size(n);
     const int T = ctx->T, M = ctx->Mtot;
@@ -1065,6 +1092,7 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
     int na = 0;
     std::vector<double> audit_h;
     std::vector<int> cand_h, all_h;
+    bool uncertain = false;  // the learned pool's images mostly take every rotation: exhaustive instead
     auto coarse_learned = [&]() -> int {
         HIPCHK(P.gmask.reserve((size_t)n * P.gwords));
         HIPCHK(P.cand_m.reserve((size_t)n * M));
@@ -1082,6 +1110,19 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
                          ctx->PSI,  P.cfg.top_k, (float)P.cfg.min_mass, P.nbr.p, P.gmask.p, P.gwords,
                          P.cand_m.p, M,         P.cand_cnt.p, P.pmass.p,   P.cand_all.p};
         HIPCHK(select_candidates(s, n, ca));
+        {
+            // a pool where most images take every rotation gains nothing from the per-image candidate pass:
+            // it runs the exhaustive coarse pass instead
+            std::vector<int> all(n);
+            pr = readback(ctx, {{P.cand_all.p, all.data(), n * sizeof(int)}});
+            if (pr) return pr;
+            const long long nall = std::accumulate(all.begin(), all.end(), 0LL);
+            if (nall * 2 > n) {
+                uncertain = true;
+                stage_end(ctx, 12);
+                return ROME_OK;
+            }
+        }
         na = P.cfg.audit_fraction > 0. ? std::min(n, (int)std::ceil(P.cfg.audit_fraction * n)) : 0;
         const int a0 = na ? (int)(P.audit_pos % (long long)(n - na + 1)) : 0;
         P.audit_pos += na;
@@ -1130,17 +1171,24 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
     if (learned) {
         rc = coarse_learned();
         if (rc) return rc;
-        double mn = 1.0;
-        for (double v : audit_h) mn = std::min(mn, v);
         P.st_pools += 1;
         P.st_images += n;
-        for (int i = 0; i < n; ++i) {
-            P.st_cand += cand_h[i];
-            P.st_all += all_h[i];
+        if (uncertain) {  // every image of the pool is counted as taking every rotation
+            P.st_cand += (double)n * M;
+            P.st_all += n;
+            P.st_uncertain += 1;
+            fell_back = true;
+        } else {
+            double mn = 1.0;
+            for (double v : audit_h) mn = std::min(mn, v);
+            for (int i = 0; i < n; ++i) {
+                P.st_cand += cand_h[i];
+                P.st_all += all_h[i];
+            }
+            P.st_audited += na;
+            P.st_min_mass = std::min(P.st_min_mass, mn);
+            if (mn < P.cfg.min_audit_mass) fell_back = true;
         }
-        P.st_audited += na;
-        P.st_min_mass = std::min(P.st_min_mass, mn);
-        if (mn < P.cfg.min_audit_mass) fell_back = true;
     }
     // a pool off the proposal (or one the audit rejected): the exhaustive low-precision pass + recompute
     auto exhaustive_setup = [&]() {
@@ -1277,8 +1325,8 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
     stage_begin(ctx, 5);
     HIPCHK(launch_project_fine(s, ctx->grids.p, ctx->is3d ? ctx->cube.p : nullptr, ctx->grid_n, ctx->pad, ctx->padF,
                                ctx->cgrid, ctx->is3d, ctx->Ai_f.p, ctx->R, NOR, ctx->ulist.p, U,
-                               dd ? ctx->ucount.p : nullptr, ctx->persist_blocks(1), ctx->f_xy.p, ctx->Pf, ctx->r_max,
-                               ctx->Pfine.p));
+                               dd ? ctx->ucount.p : nullptr, ctx->work_counters(0), ctx->persist_blocks(1),
+                               ctx->guess_U(n, U), ctx->f_xy.p, ctx->Pf, ctx->r_max, ctx->Pfine.p));
     stage_end(ctx, 5);
     FineArgs fa;
     fa.task_img = ctx->task_img.p;
@@ -1297,9 +1345,8 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
     fa.Pf = ctx->Pf;
     if (dd) {
         fa.ntask_dev = ctx->task_off.p + n;
-        fa.queue = ctx->pool_flags.p + 1;
+        fa.queue = ctx->pool_flags.p + 1;  // zeroed with the other flags at the start of the pool
         fa.abort = ctx->pool_flags.p;
-        HIPCHK(launch_fill32(s, ctx->pool_flags.p + 1, 0u, 1));  // work-queue head
     }
     // Two passes: the fine pass over the coarse-significant samples (src/map3d_optimizer.cpp:1261-1372).
     // Single pass (oversampling 0): the same kernels recompute, with NOR = NOT = 1, the diff2 of the
@@ -1311,7 +1358,9 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
                      ctx->two_pass ? nullptr : ctx->c_nsig.p, ctx->f_status.p, nullptr};
     const TailArgs tail = ctx->two_pass ? TailArgs{nullptr, nullptr} : TailArgs{ctx->c_tail.p, ctx->c_dmin.p};
     stage_begin(ctx, 6);
-    HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa, ctx->persist_blocks(0)));
+    // device-driven: one block per task up to the expected task total, the rest through the work queue
+    const long long first = dd ? std::min<long long>(0x7fffffffLL, ctx->guess_tasks(n, ntasks)) : 0;
+    HIPCHK(launch_fine_diff2(s, (int)ntasks, NOR, NOT, fa, ctx->persist_blocks(0), (int)first));
     stage_end(ctx, 6);
     stage_begin(ctx, 7);
     HIPCHK(launch_weights_fine(s, n, wp, ctx->D2.p, ctx->blk_task.p, ctx->blk_t.p, ctx->task_m.p, ctx->blk_off.p, NE, wf,
@@ -1393,6 +1442,7 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
             pp.ntask_dev = ctx->task_off.p + n;
             pp.abort = ctx->pool_flags.p;
             pp.slice_cap = binned ? slice_cap : -1;  // -1: the direct scatter takes every pool
+            pp.xq = ctx->work_counters(8);
         }
         pp.slice_val = nullptr;
         pp.slice_w = nullptr;
@@ -1406,7 +1456,8 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
         hipError_t pe = hipErrorNotSupported;
         if (dd) {
             HIPCHK(launch_fill32(s, ctx->wnorm.p, 0u, 2 * (size_t)n));
-            if (binned) HIPCHK(launch_post_slices(s, ntasks, NOR, NOT, pp, ctx->persist_blocks(2)));
+            if (binned)
+                HIPCHK(launch_post_slices(s, ntasks, NOR, NOT, pp, ctx->persist_blocks(2), ctx->guess_tasks(n, ntasks)));
             PostArgs pd = pp;  // the direct scatter of pools past the slice buffer
             pd.slice_val = nullptr;
             pd.slice_w = nullptr;
@@ -1540,6 +1591,7 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
     HIPCHK(hipStreamSynchronize(s));
     stage_collect(ctx);
     ctx->last_U = U;
+    ctx->note_counts(ntasks, U);
     return ROME_OK;
 }
 
@@ -1644,19 +1696,27 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
         const std::vector<Seg> segs = pool_segs(ctx, n, o);
         rb_layout(segs, L, true);
         if (L.bytes != g->rb_bytes) return fail(ctx, ROME_ERR_HIP, "captured pool read-back layout changed");
+        const auto t0 = std::chrono::steady_clock::now();
         HIPCHK(hipGraphLaunch(g->exec, s));
+        const auto t1 = std::chrono::steady_clock::now();
         rc = readback_end(ctx, segs, L);
+        ctx->last_launch_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        ctx->last_wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
         if (rc) return rc;
         ++g->hits;
         ++ctx->graph_launches;
         ctx->last_exec = 2;
     } else {
+        const auto t0 = std::chrono::steady_clock::now();
         rc = run_pool(ctx, n, res, kPoolDevice, &o, &L);
         if (rc) {
             (void)hipStreamSynchronize(s);  // launches before the failure may still run
             return rc;
         }
+        const auto t1 = std::chrono::steady_clock::now();
         rc = readback_end(ctx, pool_segs(ctx, n, o), L);
+        ctx->last_launch_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        ctx->last_wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
         if (rc) return rc;
         stage_collect(ctx);
         ctx->last_exec = 1;
@@ -1671,6 +1731,7 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
     }
     for (int i = 0; i < n; ++i)
         if (!(o.sumw[i] > 0)) return fail(ctx, ROME_ERR_NUMERIC, "image %d: exp_sum_weight == 0", i);
+    ctx->note_counts(o.tot[0], o.U);  // the launch sizes of the next pools (and of a capture) follow them
     if (capture_after) {  // before the bookkeeping below: a capture pass sets it from capacities
         g = graph_find(ctx, n, in, true);
         if (!g->exec && !g->failed) graph_capture(ctx, n, *g);
@@ -1708,6 +1769,8 @@ int run_estep(rome_ctx* ctx, int n, rome_results* res) {
 
 }  // namespace
 
+static int install_model(rome_ctx* ctx);
+
 extern "C" {
 
 int rome_set_execution(rome_ctx* ctx, int mode) {
@@ -1722,6 +1785,8 @@ int rome_get_execution_stats(rome_ctx* ctx, long long* out) {
     out[1] = ctx->graph_launches;
     out[2] = ctx->graph_captures;
     out[3] = ctx->dd_reruns;
+    out[4] = (long long)(ctx->last_launch_ms * 1000.);
+    out[5] = (long long)(ctx->last_wait_ms * 1000.);
     return ROME_OK;
 }
 
@@ -1783,6 +1848,8 @@ int rome_configure(rome_ctx* ctx, const rome_config* cfg) {
     if (!ctx || !cfg) return ROME_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     const rome_config& c = *cfg;
+    ctx->class_lo = 0;
+    ctx->class_hi = 1 << 30;
     if (c.ref_dim != 2 && c.ref_dim != 3) return fail(ctx, ROME_ERR_ARG, "ref_dim must be 2 or 3");
     if (c.adaptive_oversampling < 0 || c.adaptive_oversampling > 1)
         return fail(ctx, ROME_ERR_UNSUPPORTED, "adaptive_oversampling must be 0 or 1");
@@ -1943,6 +2010,15 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
     ctx->pdf_dir_h.assign(pdf_direction, pdf_direction + (size_t)K * D);
     ctx->prior_x_h.assign(prior_offx, prior_offx + K);
     ctx->prior_y_h.assign(prior_offy, prior_offy + K);
+    return install_model(ctx);
+}
+
+// The device tables of the model the host copies hold, restricted to the classes of rome_set_class_range.
+static int install_model(rome_ctx* ctx) {
+    HIPCHK(hipSetDevice(ctx->device));
+    const int G = ctx->cfg.nr_groups, K = ctx->K, D = ctx->D;
+    const double* pdf_class = ctx->pdf_class_h.data();
+    const double* pdf_direction = ctx->pdf_dir_h.data();
     std::vector<double> lp((size_t)K * D);
     std::vector<unsigned char> valid((size_t)ctx->Mpad, 0);
     std::vector<double> pm((size_t)ctx->Mtot), lpm((size_t)ctx->Mtot);
@@ -1953,8 +2029,9 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
             const double p = pdf_direction[(size_t)k * D + d];
             lp[(size_t)k * D + d] = p > 0 ? log(p) : -INFINITY;
             // the reference skips (class, direction) pairs with a non-positive prior
-            // (src/map3d_optimizer.cpp:1204-1206): no diff2, no weight, no significance entry
-            const bool ok = pdf_class[k] > 0. && p > 0.;
+            // (src/map3d_optimizer.cpp:1204-1206): no diff2, no weight, no significance entry; and the pool
+            // runs only classes exp_iclass_min .. exp_iclass_max (src/map3d_optimizer.cpp:823-840)
+            const bool ok = pdf_class[k] > 0. && p > 0. && k >= ctx->class_lo && k <= ctx->class_hi;
             for (int s = 0; s < ctx->PSI; ++s) {
                 const size_t m = (size_t)k * ctx->R + d * ctx->PSI + s;
                 valid[m] = ok;
@@ -1999,6 +2076,21 @@ int rome_set_model(rome_ctx* ctx, const double* sigma2_noise, const double* pdf_
     HIPCHK(hipStreamSynchronize(s));
     ctx->have_model = true;
     return ROME_OK;
+}
+
+int rome_set_class_range(rome_ctx* ctx, int iclass_min, int iclass_max) {
+    if (ctx) ++ctx->state_epoch;  // kernels of captured pools read what this call changes
+    if (!ctx) return ROME_ERR_ARG;
+    if (!ctx->configured) return fail(ctx, ROME_ERR_ARG, "rome_configure has not been called");
+    if (iclass_min < 0 && iclass_max < 0) {
+        iclass_min = 0;
+        iclass_max = ctx->K - 1;
+    }
+    if (iclass_min < 0 || iclass_max >= ctx->K || iclass_min > iclass_max)
+        return fail(ctx, ROME_ERR_ARG, "class range %d .. %d outside 0 .. %d", iclass_min, iclass_max, ctx->K - 1);
+    ctx->class_lo = iclass_min;
+    ctx->class_hi = iclass_max;
+    return ctx->have_model ? install_model(ctx) : ROME_OK;
 }
 
 int rome_set_reference(rome_ctx* ctx, int iclass, const double* grid) {
@@ -2256,7 +2348,7 @@ int rome_autotune(rome_ctx* ctx, const rome_device_images* im, const int* sig_ke
     HIPCHK(hipSetDevice(ctx->device));
     // per tuned kernel: its stage and the candidate grids in blocks per CU
     static const int stage_of[4] = {6, 5, 8, 10};
-    static const int per_cu[4][kTuneCands] = {{4, 8, 12, 16}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}};
+    static const int per_cu[4][kTuneCands] = {{1, 2, 3, 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}};
     const bool prof = ctx->profiling;
     const int mode = ctx->exec_mode;
     int saved[4];
@@ -2299,6 +2391,7 @@ int rome_set_persistent_grids(rome_ctx* ctx, const int* blocks) {
     for (int i = 0; i < 4; ++i)
         if (blocks[i] < 0) return fail(ctx, ROME_ERR_ARG, "negative grid size");
     std::copy(blocks, blocks + 4, ctx->persist);
+    ctx->hist_tasks = ctx->hist_U = 0.;  // the next pool's first launches are sized from its pool size
     ++ctx->state_epoch;  // kernels of captured pools read what this call changes
     return ROME_OK;
 }
@@ -2930,7 +3023,7 @@ int rome_proposal_configure(rome_ctx* ctx, const rome_proposal_config* cfg) {
     int r = prop_refresh(ctx);
     if (r) return r;
     P.audit_pos = 0;
-    P.st_pools = P.st_images = P.st_all = P.st_audited = P.st_fallback = 0;
+    P.st_pools = P.st_images = P.st_all = P.st_audited = P.st_fallback = P.st_uncertain = 0;
     P.st_cand = 0.;
     P.st_min_mass = 1.;
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -212,8 +212,8 @@ struct FineArgs {
     const double* C2f;
     double* D2;           // [blocks][NOR*NOT]
     int Pf;
-    // device-driven pools (all null otherwise): task total on the device, work-queue head (zeroed
-    // before the launch), pool abort flag (non-zero: the pool is rerun, nothing is computed)
+    // device-driven pools (all null otherwise): task total on the device, work-queue head of the tasks
+    // past the grid (zero at launch), pool abort flag (non-zero: the pool is rerun, nothing is computed)
     const long long* ntask_dev = nullptr;
     int* queue = nullptr;
     const int* abort = nullptr;
@@ -272,6 +272,7 @@ struct PostArgs {
     const long long* ntask_dev = nullptr;
     long long slice_cap = 0;
     const int* abort = nullptr;
+    unsigned long long* xq = nullptr;  // 8 per-XCD work counters of the per-slice kernel (zero at launch)
 };
 
 // Binned 3D backprojection (backproject.hip)
@@ -357,11 +358,13 @@ hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid
                                  int Mpad, float* FcT);
 // cube[K][grid_n] records of 4 float4: the 2x2x2 trilinear neighbourhood of every voxel of a 3D grid
 hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, int K, int pad, int padF, float4* cube);
-// U_dev non-null: the unique-rotation count is read on the device and persist_blocks blocks stride over it
+// U_dev non-null (device-driven): the unique-rotation count is read on the device (at most U, the buffer's
+// capacity); one block per item covers U_guess rotations and persist_blocks blocks take the rest from 8
+// per-XCD counters xq (zero at launch)
 hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4* cube, size_t grid_stride, int pad,
                                int padF, int c, int is3d, const double* Ai_fine, int R, int NOR, const int* ulist, int U,
-                               const int* U_dev, int persist_blocks, const int* xy, int Pf, int r_max,
-                               float2* Pfine);
+                               const int* U_dev, unsigned long long* xq, int persist_blocks, int U_guess,
+                               const int* xy, int Pf, int r_max, float2* Pfine);
 hipError_t launch_project_plane(hipStream_t s, const float2* grid, int pad, int padF, int c, int is3d, const double* Ai,
                                 int nrot, int size, int my_r_max, float2* out);
 // Allocation-free projection (noalloc.hip): rotations staged in a file-scope device array of
@@ -426,15 +429,19 @@ hipError_t launch_task_build(hipStream_t s, int n, const SigArgs& sig, int M, in
 hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int T, const long long* task_off,
                             const long long* blk_off, int* task_img, int* task_m, long long* task_blk0, int* task_nt,
                             int* blk_task, int* blk_t);
-hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a, int persist_blocks = 0);
+// Device-driven (a.ntask_dev set): k_fine_diff2 over `first` blocks, then the persistent work-queue kernel
+// k_fine_diff2_queue (persist_blocks blocks) over the tasks past them.
+hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a, int persist_blocks = 0,
+                             int first = 0);
 hipError_t launch_post(hipStream_t s, int n, int NOR, int NOT, const PostArgs& a);
 // The binned 3D form with one block per slice (hipErrorNotSupported outside NOR = 8, NOT = 4,
 // Pf <= 8192): wnorm must be zero before the launch.
-// The per-slice post kernel handles NOR 8 x NOT 4, 3D, Pf <= 8192 (else hipErrorNotSupported).  With
-// a.ntask_dev (device-driven pools) persist_blocks blocks stride over the device task total.
+// The per-slice post kernel handles NOR 8 x NOT 4, 3D, Pf <= 32768 (else hipErrorNotSupported).  With
+// a.ntask_dev (device-driven pools) a first launch of one block per item covers `guess` tasks and
+// persist_blocks blocks take the rest from per-XCD counters.
 bool post_slices_supported(int NOR, int NOT, int Pf, int is3d);
 hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a,
-                              int persist_blocks = 0);
+                              int persist_blocks = 0, long long guess = 0);
 // Coarse significance given by a caller (keys m * T + t): an image with a count outside 1 .. cap or a key
 // outside [0, MT) keeps none (cnt[img] = 0) and sets flags[2] and flags[0]; after the task build has
 // sorted the keys, a repeated key sets the same flags.

@@ -96,23 +96,21 @@ __global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad
 __global__ void k_project_fine(const float2* __restrict__ grids, const float4* __restrict__ cube, size_t grid_stride,
                                int pad, int padF, int c, int is3d, const double* __restrict__ Ai_fine, int R, int NOR,
                                const int* __restrict__ ulist, int U_host, const int* __restrict__ U_dev,
-                               const int* __restrict__ xy, int Pf, int r_max, float2* __restrict__ Pfine) {
+                               unsigned long long* __restrict__ xq, long long jcap, const int* __restrict__ xy, int Pf,
+                               int r_max, float2* __restrict__ Pfine) {
     // Workgroups go to the 8 XCDs round-robin by launch index; each XCD takes one contiguous range of
     // work items w = (u * nPB + pixel block) * NOR + o, so the NOR oversampled orientations of a
-    // rotation, which read neighbouring records, share one L2.  The grid is a multiple of 8 blocks, so a
-    // block striding by the grid size stays on its XCD's range.
+    // rotation, which read neighbouring records, share one L2.
     const int U = U_dev ? min(*U_dev, U_host) : U_host;  // device-driven: U_host is the buffer's capacity
     const int nPB = (Pf + 255) / 256;
     const long long G = (long long)U * NOR * nPB, per = (G + 7) / 8;
-    const long long nL = (long long)gridDim.y * gridDim.x;  // gridDim.x is a multiple of 8
-    for (long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x; (L >> 3) < per; L += nL) {
-        const long long w = (L & 7) * per + (L >> 3);
-        if (w >= G) continue;
+    auto item = [&](long long w) {
+        if (w >= G) return;
         const int o = (int)(w % NOR);
         const long long upb = w / NOR;
         const int u = (int)(upb / nPB);
         const int p = (int)(upb % nPB) * 256 + threadIdx.x;
-        if (p >= Pf) continue;
+        if (p >= Pf) return;
         const int m = ulist[u];
         const int k = m / R, r = m % R;
         const int x = px_x(xy[p]), y = px_y(xy[p]);
@@ -123,6 +121,23 @@ __global__ void k_project_fine(const float2* __restrict__ grids, const float4* _
                      : gather(grids + grid_stride * k, pad, padF, q, false);
         }
         Pfine[((size_t)u * NOR + o) * Pf + p] = v;
+    };
+    if (!xq) {  // one work item per block: item j = L >> 3 of range L & 7 (a device count: j < jcap)
+        const long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x;  // gridDim.x is a multiple of 8
+        if ((L >> 3) < per && (jcap == 0 || (L >> 3) < jcap)) item((L & 7) * per + (L >> 3));
+        return;
+    }
+    // device-driven tail: a fixed grid (a multiple of 8 blocks); the blocks of XCD x take the items j >= jcap
+    // of range x in order from that range's counter xq[x]
+    __shared__ long long s_i;
+    const int xcd = blockIdx.x & 7;
+    for (;;) {
+        if (threadIdx.x == 0) s_i = jcap + (long long)atomicAdd(xq + xcd, 1ull);
+        __syncthreads();
+        const long long i = s_i;
+        __syncthreads();
+        if (i >= per) break;
+        item(xcd * per + i);
     }
 }
 
@@ -741,26 +756,35 @@ DEV void fine_task(const FineArgs& a, int q) {
     }
 }
 
-// One task (image, rotation) per block, or (device-driven pools) a persistent grid whose blocks take
-// tasks from a work queue: the head counter a.queue (zero at launch) against the task total the task
-// build left on the device.  Tasks hold 1 .. T significant translations, so their costs differ; the
-// queue hands the next task to whichever block finishes first.
+// One task (image, rotation) per block.  Device-driven pools: the grid is sized from the task totals of
+// earlier pools (the hardware dispatcher balances tasks of 1 .. T translations as blocks retire), task
+// totals on the device, blocks past the total return at once; the tasks past the grid are left to
+// k_fine_diff2_queue.  The body is straight-line: a loop around it keeps more registers live, and at 3
+// waves per SIMD the task body then spills.
 template <int NOR, int NOT>
-__global__ __launch_bounds__(256) void k_fine_diff2(FineArgs a) {
-    if (!a.queue) {
-        fine_task<NOR, NOT>(a, blockIdx.x);
-        return;
-    }
+__global__ __launch_bounds__(256, 3) void k_fine_diff2(FineArgs a) {
+    if (a.ntask_dev && (*a.abort || (long long)blockIdx.x >= *a.ntask_dev)) return;
+    fine_task<NOR, NOT>(a, blockIdx.x);
+}
+
+// Device-driven pools, after k_fine_diff2 over the first `first` tasks: a persistent work-queue kernel
+// whose blocks take the remaining tasks first, first + 1, ... one at a time from the queue head a.queue
+// (zero at launch) until the device task total is reached, so a pool with more tasks than earlier ones
+// still runs in one captured graph.  Every block returns at once when there is no remainder.  (2 waves
+// per SIMD: the loop's live registers do not fit 3 without spills.)
+template <int NOR, int NOT>
+__global__ __launch_bounds__(256) void k_fine_diff2_queue(FineArgs a, long long first) {
     if (*a.abort) return;  // the pool is rerun: nothing here is used
     const long long ntask = *a.ntask_dev;
-    __shared__ int s_q;
+    if (ntask <= first) return;  // the same in every block
+    __shared__ long long s_q;
     for (;;) {
-        if (threadIdx.x == 0) s_q = atomicAdd(a.queue, 1);
+        if (threadIdx.x == 0) s_q = first + atomicAdd(a.queue, 1);
         __syncthreads();
-        const int q = s_q;
+        const long long q = s_q;
         if (q >= ntask) break;  // the same value in every thread
-        fine_task<NOR, NOT>(a, q);
-        __syncthreads();  // s_q is rewritten next
+        fine_task<NOR, NOT>(a, (int)q);
+        __syncthreads();  // s_q and the translation list are rewritten next
     }
 }
 
@@ -998,16 +1022,18 @@ DEV void post_slice_one(const PostArgs& a, long long w, int h) {
         if (tid == 0 && h == 0) a.slice_w[(size_t)q * NOR + o] = 0.;
         return;
     }
-    const float2* X = a.Xf + (size_t)img * a.Pf;
-    const float* Ct = a.Ctff + (size_t)img * a.Pf;
-    const float2* NM = a.NMf + (size_t)img * a.Pf;
-    const float2* Pr = a.Pfine + ((size_t)u * NOR + o) * a.Pf;
+    // every per-pixel array is offset to the block's range [p0, p0 + np)
+    const int np = a.Pf - p0;
+    const float2* X = a.Xf + (size_t)img * a.Pf + p0;
+    const float* Ct = a.Ctff + (size_t)img * a.Pf + p0;
+    const float2* NM = a.NMf + (size_t)img * a.Pf + p0;
+    const float2* Pr = a.Pfine + ((size_t)u * NOR + o) * a.Pf + p0;
     float2 Xr[PPT], NMr[PPT], CF[PPT], acc[PPT];
     double wsig[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const int p = p0 + tid + j * NT;
-        const bool in = p < a.Pf;
+        const int p = tid + j * NT;
+        const bool in = p < np;
         Xr[j] = in ? X[p] : make_float2(0.f, 0.f);
         NMr[j] = in ? NM[p] : make_float2(0.f, 0.f);
         const float2 f = in ? Pr[p] : make_float2(0.f, 0.f);
@@ -1030,11 +1056,11 @@ DEV void post_slice_one(const PostArgs& a, long long w, int h) {
                 t_off += wn * (offx * offx + offy * offy);
             }
             const float wf = (float)wn;
-            const float2* S = a.Sf + ((size_t)t * NOT + ot) * a.Pf;
+            const float2* S = a.Sf + ((size_t)t * NOT + ot) * a.Pf + p0;
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
-                const int p = p0 + tid + j * NT;
-                if (p < a.Pf) {
+                const int p = tid + j * NT;
+                if (p < np) {
                     const float2 s = S[p];
                     const float2 xs = cmul(Xr[j], s);
                     const float dr = CF[j].x - xs.x, di = CF[j].y - xs.y;
@@ -1049,14 +1075,15 @@ DEV void post_slice_one(const PostArgs& a, long long w, int h) {
     const size_t sl = (size_t)q * NOR + o;
     if (tid == 0 && h == 0) a.slice_w[sl] = wslice;
     if (!(wslice > 0.)) return;  // the same value in every thread: the whole block leaves
-    float2* V = a.slice_val + sl * a.Pf;
+    float2* V = a.slice_val + sl * a.Pf + p0;
+    const int* ires = a.f_ires + p0;
     double tot = 0.;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const int p = p0 + tid + j * NT;
-        if (p < a.Pf) {
+        const int p = tid + j * NT;
+        if (p < np) {
             V[p] = acc[j];
-            atomicAdd(&shell[a.f_ires[p]], wsig[j]);
+            atomicAdd(&shell[ires[p]], wsig[j]);
             tot += wsig[j];
         }
     }
@@ -1075,23 +1102,40 @@ DEV void post_slice_one(const PostArgs& a, long long w, int h) {
     }
 }
 
-// Work items w = task * NOR + o over the blocks: each XCD takes one contiguous range (the grid is a
-// multiple of 8 blocks, so a block striding by the grid size stays on its XCD's range).  Device-driven
-// pools read the task total on the device; a pool whose slices exceed the slice buffer (slice_cap) goes
-// to the direct scatter instead, and an aborted pool adds nothing.
-template <int NOR, int NOT, int PPT, int NT>
-__global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask) {
+// Work items w = (task * NOR + o) * NH + pixel range, NH = the slice's pixel ranges, over the blocks:
+// each XCD takes one contiguous range of items (per items).  TAIL = false: block L takes item j = L >> 3
+// of range L & 7 (j < per); device-driven pools read the task total on the device, and the grid covers the
+// first jcap = gridDim.x / 8 items of each range.  TAIL = true (device-driven only): a small fixed grid
+// takes the items j >= jcap of each range from per-XCD counters xq (usually none: the first launch is
+// sized from the pools before).  Two kernels rather than one with both paths: the loop raises the slice
+// body's register count past the 128 of 1024-thread blocks.  A pool whose slices exceed the slice buffer
+// (slice_cap) goes to the direct scatter instead, and an aborted pool adds nothing.
+template <int NOR, int NOT, int PPT, int NT, bool TAIL>
+__global__ __launch_bounds__(NT) void k_post_slice(PostArgs a, long long ntask, long long jcap) {
     if (a.ntask_dev) {
         if (*a.abort) return;
         ntask = *a.ntask_dev;
         if (ntask * NOR > a.slice_cap) return;
     }
-    const int NH = (a.Pf + PPT * NT - 1) / (PPT * NT);  // blocks per slice (pixel ranges)
+    const int NH = (a.Pf + PPT * NT - 1) / (PPT * NT);
     const long long G = ntask * NOR * NH, per = (G + 7) / 8;
-    for (long long L = blockIdx.x; (L >> 3) < per; L += gridDim.x) {
-        const long long w = (L & 7) * per + (L >> 3);
-        if (w < G) post_slice_one<NOR, NOT, PPT, NT>(a, w / NH, (int)(w % NH));
-        __syncthreads();  // LDS of the next work item
+    const int xcd = blockIdx.x & 7;
+    if constexpr (!TAIL) {
+        const long long j = blockIdx.x >> 3;
+        const long long w = xcd * per + j;
+        if (j < per && (jcap == 0 || j < jcap) && w < G) post_slice_one<NOR, NOT, PPT, NT>(a, w / NH, (int)(w % NH));
+    } else {
+        __shared__ long long s_i;
+        for (;;) {
+            if (threadIdx.x == 0) s_i = jcap + (long long)atomicAdd(a.xq + xcd, 1ull);
+            __syncthreads();
+            const long long j = s_i;
+            __syncthreads();
+            if (j >= per) break;
+            const long long w = xcd * per + j;
+            if (w < G) post_slice_one<NOR, NOT, PPT, NT>(a, w / NH, (int)(w % NH));
+            __syncthreads();  // LDS of the next work item
+        }
     }
 }
 
@@ -1249,23 +1293,35 @@ hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, 
 
 hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4* cube, size_t grid_stride, int pad,
                                int padF, int c, int is3d, const double* Ai_fine, int R, int NOR, const int* ulist, int U,
-                               const int* U_dev, int persist_blocks, const int* xy, int Pf, int r_max,
-                               float2* Pfine) {
+                               const int* U_dev, unsigned long long* xq, int persist_blocks, int U_guess,
+                               const int* xy, int Pf, int r_max, float2* Pfine) {
     if (is3d && !cube) return hipErrorInvalidValue;  // 3D projections read the neighbourhood grid
-    unsigned gx, gy;
-    if (U_dev) {  // device-driven: a fixed grid (a multiple of 8 blocks) strides over the device count
-        gx = (unsigned)(8 * std::max(1, persist_blocks / 8));
-        gy = 1;
-    } else {
-        if (U == 0) return hipSuccess;
-        const long long G = (long long)U * NOR * ((Pf + 255) / 256);
-        const long long blocks = 8 * ((G + 7) / 8);
+    const float4* cb = is3d ? cube : nullptr;
+    const long long nPB = (Pf + 255) / 256;
+    auto grid_of = [](long long items, unsigned& gx, unsigned& gy) -> bool {  // a multiple of 8 blocks
+        const long long blocks = 8 * ((items + 7) / 8);
         gx = (unsigned)std::min<long long>(blocks, 32768);
         gy = (unsigned)((blocks + gx - 1) / gx);
-        if (gy > 65535) return hipErrorInvalidValue;
+        return gy <= 65535;
+    };
+    unsigned gx, gy;
+    if (U_dev) {
+        // device-driven: one block per item for the first U_guess rotations' items of each XCD range, then
+        // a tail grid on per-XCD counters for the rest
+        if (!xq || persist_blocks <= 0 || U_guess <= 0) return hipErrorInvalidValue;
+        const long long jcap = ((long long)U_guess * NOR * nPB + 7) / 8;
+        if (!grid_of(8 * jcap, gx, gy)) return hipErrorInvalidValue;
+        k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist,
+                                                   U, U_dev, nullptr, jcap, xy, Pf, r_max, Pfine);
+        k_project_fine<<<(unsigned)(8 * std::max(1, persist_blocks / 8)), 256, 0, s>>>(
+            grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist, U, U_dev, xq, jcap, xy, Pf, r_max, Pfine);
+        LAUNCH_CHECK();
+        return hipSuccess;
     }
-    k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, is3d ? cube : nullptr, grid_stride, pad, padF, c, is3d, Ai_fine,
-                                               R, NOR, ulist, U, U_dev, xy, Pf, r_max, Pfine);
+    if (U == 0) return hipSuccess;
+    if (!grid_of((long long)U * NOR * nPB, gx, gy)) return hipErrorInvalidValue;
+    k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist, U,
+                                               nullptr, nullptr, 0, xy, Pf, r_max, Pfine);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1421,19 +1477,29 @@ hipError_t launch_task_emit(hipStream_t s, int n, const SigArgs& sig, int M, int
     return hipSuccess;
 }
 
-hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a, int persist_blocks) {
+hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const FineArgs& a, int persist_blocks,
+                             int first) {
     unsigned grid;
-    if (a.queue) {  // device-driven: a persistent grid on the work queue (head zeroed by the caller)
-        if (!a.ntask_dev || !a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
-        grid = (unsigned)persist_blocks;
+    if (a.ntask_dev) {  // device-driven: `first` blocks, then the queue kernel (head zeroed by the caller)
+        if (!a.abort || !a.queue || persist_blocks <= 0 || first <= 0) return hipErrorInvalidValue;
+        grid = (unsigned)first;
     } else {
         if (ntask == 0) return hipSuccess;
         grid = (unsigned)ntask;
     }
-    if (NOR == 8 && NOT == 4) k_fine_diff2<8, 4><<<grid, 256, 0, s>>>(a);
-    else if (NOR == 2 && NOT == 4) k_fine_diff2<2, 4><<<grid, 256, 0, s>>>(a);
-    else if (NOR == 1 && NOT == 1) k_fine_diff2<1, 1><<<grid, 256, 0, s>>>(a);
-    else return hipErrorInvalidValue;
+    const unsigned pg = (unsigned)std::max(persist_blocks, 1);
+    if (NOR == 8 && NOT == 4) {
+        k_fine_diff2<8, 4><<<grid, 256, 0, s>>>(a);
+        if (a.ntask_dev) k_fine_diff2_queue<8, 4><<<pg, 256, 0, s>>>(a, first);
+    } else if (NOR == 2 && NOT == 4) {
+        k_fine_diff2<2, 4><<<grid, 256, 0, s>>>(a);
+        if (a.ntask_dev) k_fine_diff2_queue<2, 4><<<pg, 256, 0, s>>>(a, first);
+    } else if (NOR == 1 && NOT == 1) {
+        k_fine_diff2<1, 1><<<grid, 256, 0, s>>>(a);
+        if (a.ntask_dev) k_fine_diff2_queue<1, 1><<<pg, 256, 0, s>>>(a, first);
+    } else {
+        return hipErrorInvalidValue;
+    }
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1471,19 +1537,25 @@ bool post_slices_supported(int NOR, int NOT, int Pf, int is3d) {
     return NOR == 8 && NOT == 4 && Pf <= 4 * 8 * 1024 && is3d;  // up to 4 pixel ranges of 8192 per slice
 }
 
-hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a, int persist_blocks) {
+hipError_t launch_post_slices(hipStream_t s, long long ntask, int NOR, int NOT, const PostArgs& a, int persist_blocks,
+                              long long guess) {
     if (!post_slices_supported(NOR, NOT, a.Pf, a.is3d) || !a.slice_val) return hipErrorNotSupported;
-    long long blocks;
-    if (a.ntask_dev) {  // device-driven: a fixed grid (a multiple of 8 blocks)
-        if (!a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
-        blocks = 8 * std::max(1, persist_blocks / 8);
-    } else {
-        if (ntask <= 0) return hipSuccess;
-        const long long nh = (a.Pf + 8 * 1024 - 1) / (8 * 1024);
-        blocks = 8 * ((ntask * NOR * nh + 7) / 8);
+    const long long nh = (a.Pf + 8 * 1024 - 1) / (8 * 1024);
+    if (a.ntask_dev) {
+        // device-driven: the first launch covers guess tasks (one block per item), the tail the rest
+        if (!a.abort || !a.xq || persist_blocks <= 0 || guess <= 0) return hipErrorInvalidValue;
+        const long long jcap = (guess * NOR * nh + 7) / 8;
+        if (8 * jcap > 0x7fffffffLL) return hipErrorInvalidValue;
+        k_post_slice<8, 4, 8, 1024, false><<<(unsigned)(8 * jcap), 1024, 0, s>>>(a, ntask, jcap);
+        k_post_slice<8, 4, 8, 1024, true><<<(unsigned)(8 * std::max(1, persist_blocks / 8)), 1024, 0, s>>>(a, ntask,
+                                                                                                          jcap);
+        LAUNCH_CHECK();
+        return hipSuccess;
     }
+    if (ntask <= 0) return hipSuccess;
+    const long long blocks = 8 * ((ntask * NOR * nh + 7) / 8);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    k_post_slice<8, 4, 8, 1024><<<(unsigned)blocks, 1024, 0, s>>>(a, ntask);
+    k_post_slice<8, 4, 8, 1024, false><<<(unsigned)blocks, 1024, 0, s>>>(a, ntask, 0);
     LAUNCH_CHECK();
     return hipSuccess;
 }

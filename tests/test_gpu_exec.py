@@ -2,8 +2,9 @@
 on the device, the fine diff2 on a work queue) and its captured hipGraph, replayed.
 
 The three must give the same E-step: metadata, argmax, NR_SIGN and the per-image sums bit for bit (their
-kernels are deterministic), the weighted sums and backprojector grids to the order of their atomic
-additions (1e-12 relative for the FP64 sums, 1e-6 for the FP32 grids).  A graph is replayed several times
+kernels are deterministic), the weighted sums, the norm correction (and the NORM it gives) and the
+backprojector grids to the order of their atomic additions (1e-12 relative for the FP64 sums, 1e-6 for
+the FP32 grids).  A graph is replayed several times
 with the program's own eager work and synchronisations between the replays, and every replay is checked.
 Pools the device-driven path cannot finish (a coarse list or the significant-sample keys past their caps)
 are rerun host-synchronised and still give the same results."""
@@ -18,10 +19,11 @@ pytestmark = pytest.mark.gpu
 
 CASES = [("3d", "fp32"), ("3d_window_groups_offsets", "fp32"), ("2d_two_pass", "fp32"), ("3d_zero_prior", "fp32"),
          ("3d", "bf16"), ("3d_window_groups_offsets", "fp16"), ("3d_paired_k1", "fp16")]
-EXACT = ("meta", "argmax", "nr_sign", "min_diff2", "sum_weight", "sig_weight", "min_diff2_coarse",
-         "sig_weight_coarse")
+EXACT = ("argmax", "nr_sign", "min_diff2", "sum_weight", "sig_weight", "min_diff2_coarse", "sig_weight_coarse")
+META_EXACT = [0, 1, 2, 3, 4, 5, 6, 8]  # all but NORM (column 7), which follows the norm-correction sum
 # the coarse weight sum adds the coarse list's entries in the order the cross-term epilogue appended them
-# (atomic reservations): equal to the last bits only
+# (atomic reservations), the norm-correction sum a pool's significant samples in the order of their atomic
+# additions: equal to the last bits only
 CLOSE = ("sum_weight_coarse", "wsum_norm_correction")
 
 
@@ -38,6 +40,8 @@ def _same(a, b, what):
     rb, sb, gb, _ = b
     for k in EXACT:
         assert np.array_equal(ra[k], rb[k]), f"{what}: {k}"
+    assert np.array_equal(ra["meta"][:, META_EXACT], rb["meta"][:, META_EXACT]), f"{what}: meta"
+    np.testing.assert_allclose(ra["meta"][:, 7], rb["meta"][:, 7], rtol=1e-12, err_msg=f"{what}: NORM")
     for k in CLOSE:
         np.testing.assert_allclose(ra[k], rb[k], rtol=1e-12, err_msg=f"{what}: {k}")
     for k in sa:
@@ -97,5 +101,38 @@ def test_device_driven_overflow_reruns_host(engine):
         _same(r, ref, "key overflow rerun")
     finally:
         engine.set_coarse_mode("list", 0)
+        engine.set_significance_cap(-1)
+        engine.set_execution("graph")
+
+
+def test_work_past_the_first_launches(engine):
+    """Grids of 8 blocks and no history of earlier pools: the fine diff2's first launch covers 4 tasks per
+    image and the rest goes through its work queue, the projection and the post slices run most of their
+    items in their tails; device-driven and replayed graphs still give the host-synchronised results."""
+    # SNR 0.02: broad posteriors, 347 .. 4248 significant coarse samples per image
+    prob, batch, _ = synth.make_problem(is3d=True, ori=32, current=32, coarse=16, K=2, order=1, n=8, snr=0.02,
+                                        seed=5)
+    n = batch["n"]
+    print(f"[exec past first launches] n={n}", flush=True)
+    try:
+        engine.set_significance_cap(8192)
+        host = _pool(engine, prob, batch, "fp32", "host")
+        st = engine.batch_stats()
+        assert st["tasks"] > 4 * n and st["unique_rot"] > 4 * n, st  # the queue and the tails take part
+        engine.set_persistent_grids([8, 8, 8, 8])
+        dev = _pool(engine, prob, batch, "fp32", "device")
+        assert dev[3]["last"] == 1, dev[3]
+        _same(dev, host, "device-driven, 8-block grids")
+        engine.set_persistent_grids([8, 8, 8, 8])
+        first = _pool(engine, prob, batch, "fp32", "graph")
+        _same(first, host, "graph, first pool")
+        engine.reset_sums()
+        r = engine.expectation(batch)
+        st = engine.execution_stats()
+        assert st["last"] == 2, st
+        _same((r, engine.get_sums(), [engine.get_backprojector(k) for k in range(prob["nr_classes"])], st), host,
+              "graph replay, 8-block grids")
+    finally:
+        engine.set_persistent_grids([0, 0, 0, 0])
         engine.set_significance_cap(-1)
         engine.set_execution("graph")

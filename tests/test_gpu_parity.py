@@ -449,3 +449,30 @@ def test_config3_shape_192_hp3(engine):
     np.testing.assert_allclose(r["min_diff2"], ref["min_diff2"], rtol=1e-5)
     np.testing.assert_allclose(r["min_diff2_coarse"], ref["min_diff2_coarse"], rtol=1e-5)
     _check_sums(r, s, ref, prob, engine, "config3 shape")
+
+
+def test_class_range_single_class_pool(engine):
+    """The seeding iteration's pools run one class each (exp_iclass_min = exp_iclass_max,
+    src/map3d_optimizer.cpp:823-840): rome_set_class_range(k, k) against the oracle run with every other
+    class left out (the oracle skips classes without prior, as the reference's loop skips classes outside
+    the range); then the full range again gives the all-class E-step."""
+    prob, batch, _ = _case("3d_window_groups_offsets")
+    K = prob["nr_classes"]
+    print(f"[class range] K={K}", flush=True)
+    for k in range(K):
+        only = dict(prob, pdf_class=np.where(np.arange(K) == k, prob["pdf_class"], 0.0))
+        ref = O.estep(only, batch)
+        synth.load_engine(engine, prob)
+        engine.set_class_range(k, k)
+        try:
+            r = engine.expectation(batch)
+            s = engine.get_sums()
+        finally:
+            engine.set_class_range()
+        assert np.all(r["argmax"][:, 0] == k)
+        assert np.array_equal(r["argmax"], ref["argmax"]), k
+        np.testing.assert_allclose(r["min_diff2"], ref["min_diff2"], rtol=1e-5)
+        _check_sums(r, s, ref, only, engine, f"class range {k}")
+    synth.load_engine(engine, prob)
+    r = engine.expectation(batch)
+    np.testing.assert_allclose(r["min_diff2"], O.estep(prob, batch)["min_diff2"], rtol=1e-5)
