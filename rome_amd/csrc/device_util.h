@@ -92,9 +92,22 @@ DEV float2 gather(const float2* __restrict__ g, int pad, int padF, const Coords&
 // The same trilinear interpolation from the neighbourhood grid: record i holds the eight values
 // the flat gather reads for base voxel i (x0, x0+1 of rows (y0,z0), (y1,z0), (y0,z1), (y1,z1)),
 // so one sample reads 64 contiguous bytes instead of touching four 128-byte lines.
-DEV float2 gather_cube(const float4* __restrict__ cube, int pad, int padF, const Coords& q) {
-    const float4* rec = cube + 4 * (((size_t)q.z0 * pad + q.y0) * padF + q.x0);
-    const float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3];
+// Neighbourhood grid layout: 8 x 8 x 8 bricks of records (x fastest inside a brick, bricks x fastest),
+// nbx = ceil(padF / 8), nby = ceil(pad / 8) bricks along x and y.  Records of nearby base voxels share a
+// 32 KB brick (and a 2 MB page), where the plain z-major order puts each z plane of records 2 MB apart and
+// a slice through the grid touches a new page at almost every sample.
+DEV size_t cube_index(int x, int y, int z, int nbx, int nby) {
+    return ((((size_t)(z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) << 9) + ((z & 7) << 6) + ((y & 7) << 3) + (x & 7);
+}
+
+// record of base voxel q in the neighbourhood grid
+DEV const float4* cube_record(const float4* __restrict__ cube, int nbx, int nby, const Coords& q) {
+    return cube + 4 * cube_index(q.x0, q.y0, q.z0, nbx, nby);
+}
+
+// trilinear interpolation from a record: a = (v000, v001), b = (v010, v011), c = (v100, v101),
+// d = (v110, v111) (value index z y x), the same arithmetic as gather
+DEV float2 cube_interp(const float4& a, const float4& b, const float4& c, const float4& d, const Coords& q) {
     float dx00r = linf(q.fx, a.x, a.z), dx00i = linf(q.fx, a.y, a.w);
     float dx01r = linf(q.fx, c.x, c.z), dx01i = linf(q.fx, c.y, c.w);
     float dx10r = linf(q.fx, b.x, b.z), dx10i = linf(q.fx, b.y, b.w);
@@ -107,6 +120,7 @@ DEV float2 gather_cube(const float4* __restrict__ cube, int pad, int padF, const
     if (q.neg) out.y = -out.y;
     return out;
 }
+
 
 DEV int px_x(int xy) { return xy & 0xffff; }
 DEV int px_y(int xy) { return (int)(short)(xy >> 16); }

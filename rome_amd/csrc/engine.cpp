@@ -497,7 +497,7 @@ int ensure_ready(rome_ctx* ctx) {
         HIPCHK(ctx->Qc.reserve((size_t)ctx->Kq * ctx->Mpad));
         HIPCHK(launch_fct_norm(ctx->stream, ctx->FcT.p, ctx->Pc, ctx->Pc_pad, ctx->Mpad, ctx->Kq, ctx->Qc.p));
         if (ctx->is3d) {
-            HIPCHK(ctx->cube.reserve(4 * ctx->grid_n * ctx->K));
+            HIPCHK(ctx->cube.reserve(4 * cube_records(ctx->pad, ctx->padF) * ctx->K));
             HIPCHK(launch_build_cube(ctx->stream, ctx->grids.p, ctx->grid_n, ctx->K, ctx->pad, ctx->padF, ctx->cube.p));
         }
         if (ctx->cfg.precision != ROME_PREC_FP32) {

@@ -357,6 +357,11 @@ hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid
                                  const double* Ai, int R, int Mtot, const int* xy, int Pc, int Pc_pad, int my_r_max,
                                  int Mpad, float* FcT);
 // cube[K][grid_n] records of 4 float4: the 2x2x2 trilinear neighbourhood of every voxel of a 3D grid
+// records per class of the neighbourhood grid (8 x 8 x 8 bricks covering pad x pad x padF)
+__host__ __device__ inline size_t cube_records(int pad, int padF) {
+    const size_t nbx = (padF + 7) / 8, nby = (pad + 7) / 8;
+    return nbx * nby * nby * 512;
+}
 hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, int K, int pad, int padF, float4* cube);
 // U_dev non-null (device-driven): the unique-rotation count is read on the device (at most U, the buffer's
 // capacity); one block per item covers U_guess rotations and persist_blocks blocks take the rest from 8

@@ -69,19 +69,30 @@ __global__ void k_project_coarse(const float2* __restrict__ grids, size_t grid_s
     FcT[(size_t)(Pc_pad + p) * Mpad + m] = v.y;
 }
 
-// Neighbourhood grid of a 3D class grid: record i = the eight values the trilinear gather reads
-// for base voxel i (flat indices i, i+1, i+padF, i+padF+1, then the same one z plane on), zero
-// past the end of the class grid.  64 bytes per voxel: 8x the grid's HBM for one 64-byte read per
-// fine-projection sample instead of four scattered 128-byte lines.  blockIdx.y = class.
-__global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad, int padF, float4* __restrict__ cube) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Neighbourhood grid of a 3D class grid: the record of base voxel i = the eight values the trilinear
+// gather reads (flat indices i, i+1, i+padF, i+padF+1, then the same one z plane on), zero past the end
+// of the class grid.  64 bytes per voxel: 8x the grid's HBM for one 64-byte read per fine-projection
+// sample instead of four scattered 128-byte lines.  Records in 8 x 8 x 8 bricks (cube_index); one
+// thread per record of the brick layout, blockIdx.y = class.
+__global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad, int padF, float4* __restrict__ cube,
+                             size_t cube_n) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // record in the brick layout
+    if (j >= cube_n) return;
+    const int nbx = (padF + 7) / 8, nby = (pad + 7) / 8;
+    const size_t b = j >> 9;
+    const int x = (int)(b % nbx) * 8 + (int)(j & 7), y = (int)((b / nbx) % nby) * 8 + (int)((j >> 3) & 7),
+              z = (int)(b / ((size_t)nbx * nby)) * 8 + (int)((j >> 6) & 7);
+    float4* o = cube + 4 * (cube_n * blockIdx.y + j);
+    if (x >= padF || y >= pad || z >= pad) {  // brick padding: never read
+        o[0] = o[1] = o[2] = o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const size_t i = ((size_t)z * pad + y) * padF + x;
     const float2* g = grids + n * blockIdx.y;
     const size_t sy = (size_t)padF, sz = (size_t)pad * padF;
-    auto at = [&](size_t j) { return j < n ? g[j] : make_float2(0.f, 0.f); };
+    auto at = [&](size_t q) { return q < n ? g[q] : make_float2(0.f, 0.f); };
     const float2 v0 = at(i), v1 = at(i + 1), v2 = at(i + sy), v3 = at(i + sy + 1);
     const float2 v4 = at(i + sz), v5 = at(i + sz + 1), v6 = at(i + sz + sy), v7 = at(i + sz + sy + 1);
-    float4* o = cube + 4 * (n * blockIdx.y + i);
     o[0] = make_float4(v0.x, v0.y, v1.x, v1.y);
     o[1] = make_float4(v2.x, v2.y, v3.x, v3.y);
     o[2] = make_float4(v4.x, v4.y, v5.x, v5.y);
@@ -91,36 +102,62 @@ __global__ void k_build_cube(const float2* __restrict__ grids, size_t n, int pad
 // Fine projections for the unique rotations the batch needs: Pf[u][o][p].  3D classes read the
 // neighbourhood grid (cube, same values and arithmetic as the flat gather); 2D the flat grid.
 // U_dev (device-driven pools): the count of unique rotations is read from the device (at most U_host, the
-// capacity of Pfine; a pool with more is aborted) and the grid is a fixed set of blocks that strides over
-// the work items; otherwise one work item per block.
-__global__ void k_project_fine(const float2* __restrict__ grids, const float4* __restrict__ cube, size_t grid_stride,
-                               int pad, int padF, int c, int is3d, const double* __restrict__ Ai_fine, int R, int NOR,
-                               const int* __restrict__ ulist, int U_host, const int* __restrict__ U_dev,
-                               unsigned long long* __restrict__ xq, long long jcap, const int* __restrict__ xy, int Pf,
-                               int r_max, float2* __restrict__ Pfine) {
+// capacity of Pfine; a pool with more is aborted); the first launch covers the items of U_guess rotations,
+// a fixed tail grid the rest.  A work item is (rotation u, block of 256 pixels) with all NOR oversampled
+// orientations: each thread gathers NOR independent samples of its pixel (the loads of one item are in
+// flight together, and the chain rotation -> matrices -> pixel is paid once per item, not per sample).
+template <int NOR>
+__global__ __launch_bounds__(256) void k_project_fine(const float2* __restrict__ grids, const float4* __restrict__ cube,
+                                                      size_t grid_stride, int pad, int padF, int c, int is3d,
+                                                      const double* __restrict__ Ai_fine, int R,
+                                                      const int* __restrict__ ulist, int U_host,
+                                                      const int* __restrict__ U_dev, unsigned long long* __restrict__ xq,
+                                                      long long jcap, const int* __restrict__ xy, int Pf, int r_max,
+                                                      float2* __restrict__ Pfine) {
     // Workgroups go to the 8 XCDs round-robin by launch index; each XCD takes one contiguous range of
-    // work items w = (u * nPB + pixel block) * NOR + o, so the NOR oversampled orientations of a
-    // rotation, which read neighbouring records, share one L2.
+    // items w = u * nPB + pixel block, so the pixel blocks of a rotation share one L2.
     const int U = U_dev ? min(*U_dev, U_host) : U_host;  // device-driven: U_host is the buffer's capacity
     const int nPB = (Pf + 255) / 256;
-    const long long G = (long long)U * NOR * nPB, per = (G + 7) / 8;
+    const long long G = (long long)U * nPB, per = (G + 7) / 8;
+    const size_t cube_stride = cube ? cube_records(pad, padF) : 0;
     auto item = [&](long long w) {
         if (w >= G) return;
-        const int o = (int)(w % NOR);
-        const long long upb = w / NOR;
-        const int u = (int)(upb / nPB);
-        const int p = (int)(upb % nPB) * 256 + threadIdx.x;
+        const int u = (int)(w / nPB);
+        const int p = (int)(w % nPB) * 256 + threadIdx.x;
         if (p >= Pf) return;
-        const int m = ulist[u];
+        const int m = __builtin_amdgcn_readfirstlane(ulist[u]);  // block-uniform: matrices by scalar loads
         const int k = m / R, r = m % R;
         const int x = px_x(xy[p]), y = px_y(xy[p]);
-        float2 v = make_float2(0.f, 0.f);
-        if (x * x + y * y <= r_max * r_max) {
-            Coords q = map_coords(Ai_fine + 6 * ((size_t)r * NOR + o), x, y, c, is3d != 0);
-            v = cube ? gather_cube(cube + 4 * grid_stride * k, pad, padF, q)
-                     : gather(grids + grid_stride * k, pad, padF, q, false);
+        float2* out = Pfine + (size_t)u * NOR * Pf + p;
+        if (x * x + y * y > r_max * r_max) {
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) out[(size_t)o * Pf] = make_float2(0.f, 0.f);
+            return;
         }
-        Pfine[((size_t)u * NOR + o) * Pf + p] = v;
+        const double* A = Ai_fine + 6 * (size_t)r * NOR;
+        Coords q[NOR];
+#pragma unroll
+        for (int o = 0; o < NOR; ++o) q[o] = map_coords(A + 6 * o, x, y, c, is3d != 0);
+        float2 v[NOR];
+        if (cube) {  // every record load of the item issued before the first is used
+            const float4* cb = cube + 4 * cube_stride * k;
+            const int nbx = (padF + 7) / 8, nby = (pad + 7) / 8;
+            float4 rec[NOR][4];
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) {
+                const float4* rp = cube_record(cb, nbx, nby, q[o]);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) rec[o][e] = rp[e];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their first use
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) v[o] = cube_interp(rec[o][0], rec[o][1], rec[o][2], rec[o][3], q[o]);
+        } else {
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) v[o] = gather(grids + grid_stride * k, pad, padF, q[o], false);
+        }
+#pragma unroll
+        for (int o = 0; o < NOR; ++o) out[(size_t)o * Pf] = v[o];
     };
     if (!xq) {  // one work item per block: item j = L >> 3 of range L & 7 (a device count: j < jcap)
         const long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x;  // gridDim.x is a multiple of 8
@@ -695,26 +732,35 @@ DEV void fine_task(const FineArgs& a, int q) {
         for (int o = 0; o < NOR; ++o)
 #pragma unroll
             for (int j = 0; j < NJ; ++j) B[o][j] = 0.f;
+        const float2* St[TG];  // phase rows of this group's translations (a valid row in unused slots)
+#pragma unroll
+        for (int tt = 0; tt < TG; ++tt) St[tt] = a.Sf + (size_t)ts[g0 + min(tt, ng - 1)] * NOT * a.Pf;
         for (int p = tid; p < a.Pf; p += NT) {
+            // every load of the pixel issued before the first use: the loop is bound by memory latency,
+            // not bandwidth (pixels with Minv = 0 add zeros: no branch that the other loads wait for)
             const float minv = Mi[p];
-            if (minv == 0.f) continue;
             const float ctf = Ct[p];
             const float2 x = X[p];
+            float2 f[NOR], Sv[TG][NOT];
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) f[o] = Pf[(size_t)o * a.Pf + p];
+#pragma unroll
+            for (int tt = 0; tt < TG; ++tt)
+#pragma unroll
+                for (int ot = 0; ot < NOT; ++ot) Sv[tt][ot] = St[tt][(size_t)ot * a.Pf + p];
+            __builtin_amdgcn_sched_barrier(0);
             float2 CF[NOR];
 #pragma unroll
             for (int o = 0; o < NOR; ++o) {
-                const float2 f = Pf[(size_t)o * a.Pf + p];
-                CF[o] = make_float2(ctf * f.x, ctf * f.y);
+                CF[o] = make_float2(ctf * f[o].x, ctf * f[o].y);
                 if (g0 == 0) Aacc[o] += minv * (CF[o].x * CF[o].x + CF[o].y * CF[o].y);
             }
 #pragma unroll
             for (int tt = 0; tt < TG; ++tt) {
                 if (tt < ng) {
-                    const int t = ts[g0 + tt];
 #pragma unroll
                     for (int ot = 0; ot < NOT; ++ot) {
-                        const float2 S = a.Sf[((size_t)t * NOT + ot) * a.Pf + p];
-                        const float2 xs = cmul(x, S);
+                        const float2 xs = cmul(x, Sv[tt][ot]);
                         const float yr = minv * xs.x, yi = minv * xs.y;
 #pragma unroll
                         for (int o = 0; o < NOR; ++o) B[o][tt * NOT + ot] += CF[o].x * yr + CF[o].y * yi;
@@ -762,7 +808,7 @@ DEV void fine_task(const FineArgs& a, int q) {
 // k_fine_diff2_queue.  The body is straight-line: a loop around it keeps more registers live, and at 3
 // waves per SIMD the task body then spills.
 template <int NOR, int NOT>
-__global__ __launch_bounds__(256, 3) void k_fine_diff2(FineArgs a) {
+__global__ __launch_bounds__(256, 2) void k_fine_diff2(FineArgs a) {
     if (a.ntask_dev && (*a.abort || (long long)blockIdx.x >= *a.ntask_dev)) return;
     fine_task<NOR, NOT>(a, blockIdx.x);
 }
@@ -1285,8 +1331,9 @@ hipError_t launch_project_coarse(hipStream_t s, const float2* grids, size_t grid
 
 hipError_t launch_build_cube(hipStream_t s, const float2* grids, size_t grid_n, int K, int pad, int padF, float4* cube) {
     if (grid_n == 0 || K == 0) return hipSuccess;
-    dim3 grid((unsigned)((grid_n + 255) / 256), K);
-    k_build_cube<<<grid, 256, 0, s>>>(grids, grid_n, pad, padF, cube);
+    const size_t cn = cube_records(pad, padF);
+    dim3 grid((unsigned)((cn + 255) / 256), K);
+    k_build_cube<<<grid, 256, 0, s>>>(grids, grid_n, pad, padF, cube, cn);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1304,24 +1351,34 @@ hipError_t launch_project_fine(hipStream_t s, const float2* grids, const float4*
         gy = (unsigned)((blocks + gx - 1) / gx);
         return gy <= 65535;
     };
+    auto go = [&](dim3 grid, unsigned long long* q, long long jcap) -> hipError_t {
+        switch (NOR) {
+        case 8: k_project_fine<8><<<grid, 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, ulist, U,
+                                                       U_dev, q, jcap, xy, Pf, r_max, Pfine); break;
+        case 2: k_project_fine<2><<<grid, 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, ulist, U,
+                                                       U_dev, q, jcap, xy, Pf, r_max, Pfine); break;
+        case 1: k_project_fine<1><<<grid, 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, ulist, U,
+                                                       U_dev, q, jcap, xy, Pf, r_max, Pfine); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipSuccess;
+    };
     unsigned gx, gy;
+    hipError_t e;
     if (U_dev) {
         // device-driven: one block per item for the first U_guess rotations' items of each XCD range, then
         // a tail grid on per-XCD counters for the rest
         if (!xq || persist_blocks <= 0 || U_guess <= 0) return hipErrorInvalidValue;
-        const long long jcap = ((long long)U_guess * NOR * nPB + 7) / 8;
+        const long long jcap = ((long long)U_guess * nPB + 7) / 8;
         if (!grid_of(8 * jcap, gx, gy)) return hipErrorInvalidValue;
-        k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist,
-                                                   U, U_dev, nullptr, jcap, xy, Pf, r_max, Pfine);
-        k_project_fine<<<(unsigned)(8 * std::max(1, persist_blocks / 8)), 256, 0, s>>>(
-            grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist, U, U_dev, xq, jcap, xy, Pf, r_max, Pfine);
+        if ((e = go(dim3(gx, gy), nullptr, jcap)) != hipSuccess) return e;
+        if ((e = go(dim3((unsigned)(8 * std::max(1, persist_blocks / 8))), xq, jcap)) != hipSuccess) return e;
         LAUNCH_CHECK();
         return hipSuccess;
     }
     if (U == 0) return hipSuccess;
-    if (!grid_of((long long)U * NOR * nPB, gx, gy)) return hipErrorInvalidValue;
-    k_project_fine<<<dim3(gx, gy), 256, 0, s>>>(grids, cb, grid_stride, pad, padF, c, is3d, Ai_fine, R, NOR, ulist, U,
-                                               nullptr, nullptr, 0, xy, Pf, r_max, Pfine);
+    if (!grid_of((long long)U * nPB, gx, gy)) return hipErrorInvalidValue;
+    if ((e = go(dim3(gx, gy), nullptr, 0)) != hipSuccess) return e;
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1488,15 +1545,16 @@ hipError_t launch_fine_diff2(hipStream_t s, int ntask, int NOR, int NOT, const F
         grid = (unsigned)ntask;
     }
     const unsigned pg = (unsigned)std::max(persist_blocks, 1);
+    const FineArgs& fa = a;
     if (NOR == 8 && NOT == 4) {
-        k_fine_diff2<8, 4><<<grid, 256, 0, s>>>(a);
-        if (a.ntask_dev) k_fine_diff2_queue<8, 4><<<pg, 256, 0, s>>>(a, first);
+        k_fine_diff2<8, 4><<<grid, 256, 0, s>>>(fa);
+        if (a.ntask_dev) k_fine_diff2_queue<8, 4><<<pg, 256, 0, s>>>(fa, first);
     } else if (NOR == 2 && NOT == 4) {
-        k_fine_diff2<2, 4><<<grid, 256, 0, s>>>(a);
-        if (a.ntask_dev) k_fine_diff2_queue<2, 4><<<pg, 256, 0, s>>>(a, first);
+        k_fine_diff2<2, 4><<<grid, 256, 0, s>>>(fa);
+        if (a.ntask_dev) k_fine_diff2_queue<2, 4><<<pg, 256, 0, s>>>(fa, first);
     } else if (NOR == 1 && NOT == 1) {
-        k_fine_diff2<1, 1><<<grid, 256, 0, s>>>(a);
-        if (a.ntask_dev) k_fine_diff2_queue<1, 1><<<pg, 256, 0, s>>>(a, first);
+        k_fine_diff2<1, 1><<<grid, 256, 0, s>>>(fa);
+        if (a.ntask_dev) k_fine_diff2_queue<1, 1><<<pg, 256, 0, s>>>(fa, first);
     } else {
         return hipErrorInvalidValue;
     }
