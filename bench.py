@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--adaptive-batch", type=int, default=250,
                     help="particles per pool of that leg (a pool's fine projections and backprojection slices "
                          "fit the device-driven capacities up to about 400)")
+    ap.add_argument("--only-adaptive", action="store_true",
+                    help="run only the batched adaptive fine-pass leg (--adaptive-n particles, 20000 if unset)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -344,6 +346,14 @@ def main():
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
+        return
+    if args.only_adaptive:
+        # the batched adaptive fine-pass leg alone (profiling runs of that workload)
+        def log(*a):
+            print(*a, file=sys.stderr, flush=True)
+        if args.adaptive_n <= 0:
+            args.adaptive_n = 20000
+        print(json.dumps({"adaptive_fine_pass": adaptive_leg(args, log)}))
         return
     import rome_amd
     from rome_amd import synth

@@ -802,6 +802,93 @@ DEV void fine_task(const FineArgs& a, int q) {
     }
 }
 
+// The same task with the four waves on different translations (NOR = 8): wave w takes translations
+// w*TGW .. w*TGW+TGW-1 of each pass of 4*TGW, every wave runs over all pixels with all 8 orientations, so
+// a pass reads the task's projections once from L2 / HBM (the other three waves find them in L1) and a
+// task of up to 4*TGW translations makes one pass; each wave sums its own outputs (FP32 products and
+// per-lane partials, FP64 wave reduction) and computes the norms |CF|^2 itself.
+template <int NOR, int NOT, int TGW>
+DEV void fine_task_waves(const FineArgs& a, int q) {
+    static_assert(NOR == 8, "four waves of two orientations' outputs");
+    constexpr int NJW = TGW * NOT, TGB = 4 * TGW;
+    __shared__ int ts[kMaxT];
+    __shared__ double asum[4][NOR];  // per wave, lane 0 (the lane that writes the outputs)
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int img = a.task_img[q], m = a.task_m[q], nt = a.task_nt[q];
+    const long long b0 = a.task_blk0[q];
+    const int u = a.m2u[m];
+    for (int j = tid; j < nt; j += 256) ts[j] = a.blk_t[b0 + j];
+    __syncthreads();
+    const float2* Pf = a.Pfine + (size_t)u * NOR * a.Pf;
+    const float2* X = a.Xf + (size_t)img * a.Pf;
+    const float* Mi = a.Minvf + (size_t)img * a.Pf;
+    const float* Ct = a.Ctff + (size_t)img * a.Pf;
+    const double cimg = a.C2f[img];
+    for (int g0 = 0; g0 < nt; g0 += TGB) {
+        const int tw0 = g0 + wid * TGW;
+        const int ngw = min(TGW, nt - tw0);  // translations of this wave in this pass (wave-uniform)
+        if (ngw <= 0) break;                  // later passes have none either
+        float B[NOR][NJW], Aacc[NOR];
+#pragma unroll
+        for (int o = 0; o < NOR; ++o) {
+            Aacc[o] = 0.f;
+#pragma unroll
+            for (int j = 0; j < NJW; ++j) B[o][j] = 0.f;
+        }
+        const float2* St[TGW];
+#pragma unroll
+        for (int tt = 0; tt < TGW; ++tt) St[tt] = a.Sf + (size_t)ts[tw0 + min(tt, ngw - 1)] * NOT * a.Pf;
+        for (int p = lane; p < a.Pf; p += 64) {
+            const float minv = Mi[p];
+            const float ctf = Ct[p];
+            const float2 x = X[p];
+            float2 f[NOR], Sv[TGW][NOT];
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) f[o] = Pf[(size_t)o * a.Pf + p];
+#pragma unroll
+            for (int tt = 0; tt < TGW; ++tt)
+#pragma unroll
+                for (int ot = 0; ot < NOT; ++ot) Sv[tt][ot] = St[tt][(size_t)ot * a.Pf + p];
+            __builtin_amdgcn_sched_barrier(0);
+            float2 CF[NOR];
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) {
+                CF[o] = make_float2(ctf * f[o].x, ctf * f[o].y);
+                if (g0 == 0) Aacc[o] += minv * (CF[o].x * CF[o].x + CF[o].y * CF[o].y);
+            }
+#pragma unroll
+            for (int tt = 0; tt < TGW; ++tt) {
+                if (tt < ngw) {
+#pragma unroll
+                    for (int ot = 0; ot < NOT; ++ot) {
+                        const float2 xs = cmul(x, Sv[tt][ot]);
+                        const float yr = minv * xs.x, yi = minv * xs.y;
+#pragma unroll
+                        for (int o = 0; o < NOR; ++o) B[o][tt * NOT + ot] += CF[o].x * yr + CF[o].y * yi;
+                    }
+                }
+            }
+        }
+        if (g0 == 0) {
+#pragma unroll
+            for (int o = 0; o < NOR; ++o) {
+                const double v = wave_sum((double)Aacc[o]);
+                if (lane == 0) asum[wid][o] = v;
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < NOR; ++o)
+#pragma unroll
+            for (int j = 0; j < NJW; ++j) {
+                if (j / NOT >= ngw) continue;  // translation slot unused in this pass (wave-uniform)
+                const double bsum = wave_sum((double)B[o][j]);
+                if (lane == 0)
+                    a.D2[(size_t)(b0 + tw0 + j / NOT) * (NOR * NOT) + o * NOT + j % NOT] =
+                        0.5 * (asum[wid][o] - 2.0 * bsum) + cimg;
+            }
+    }
+}
+
 // One task (image, rotation) per block.  Device-driven pools: the grid is sized from the task totals of
 // earlier pools (the hardware dispatcher balances tasks of 1 .. T translations as blocks retire), task
 // totals on the device, blocks past the total return at once; the tasks past the grid are left to
@@ -810,7 +897,8 @@ DEV void fine_task(const FineArgs& a, int q) {
 template <int NOR, int NOT>
 __global__ __launch_bounds__(256, 2) void k_fine_diff2(FineArgs a) {
     if (a.ntask_dev && (*a.abort || (long long)blockIdx.x >= *a.ntask_dev)) return;
-    fine_task<NOR, NOT>(a, blockIdx.x);
+    if constexpr (NOR == 8) fine_task_waves<NOR, NOT, 2>(a, blockIdx.x);
+    else fine_task<NOR, NOT>(a, blockIdx.x);
 }
 
 // Device-driven pools, after k_fine_diff2 over the first `first` tasks: a persistent work-queue kernel
@@ -829,7 +917,8 @@ __global__ __launch_bounds__(256) void k_fine_diff2_queue(FineArgs a, long long 
         __syncthreads();
         const long long q = s_q;
         if (q >= ntask) break;  // the same value in every thread
-        fine_task<NOR, NOT>(a, (int)q);
+        if constexpr (NOR == 8) fine_task_waves<NOR, NOT, 2>(a, (int)q);
+        else fine_task<NOR, NOT>(a, (int)q);
         __syncthreads();  // s_q and the translation list are rewritten next
     }
 }
