@@ -348,7 +348,7 @@ struct rome_ctx {
         return (int)std::max<long long>(1, std::min<long long>(cap, (long long)g));
     }
     int persist_blocks(int i) const {
-        static const int per_cu[4] = {2, 8, 2, 8};
+        static const int per_cu[4] = {2, 8, 2, 32};
         return persist[i] > 0 ? persist[i] : n_cu * per_cu[i];
     }
     // unique rotations a device-driven pool holds fine projections for (a pool that needs more aborts)
@@ -2348,7 +2348,7 @@ int rome_autotune(rome_ctx* ctx, const rome_device_images* im, const int* sig_ke
     HIPCHK(hipSetDevice(ctx->device));
     // per tuned kernel: its stage and the candidate grids in blocks per CU
     static const int stage_of[4] = {6, 5, 8, 10};
-    static const int per_cu[4][kTuneCands] = {{1, 2, 3, 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}};
+    static const int per_cu[4][kTuneCands] = {{1, 2, 3, 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {8, 16, 32, 64}};
     const bool prof = ctx->profiling;
     const int mode = ctx->exec_mode;
     int saved[4];
