@@ -284,12 +284,12 @@ int rome_get_execution_stats(rome_ctx* ctx, long long* out6);
  * `images` (with sig_keys / sig_cnt / cap: its fine pass from that significance, as
  * rome_expectation_fine_device; NULL: the whole E-step) device-driven with 4 candidate grids for each of the fine diff2's work-queue kernel, the
  * fine projection, the per-slice post kernel and the binned backprojection (blocks per CU: {1, 2, 3,
- * 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}), twice each, times their stages with HIP events and
+ * 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {8, 16, 32, 64}), twice each, times their stages with HIP events and
  * keeps the fastest grid of each kernel for the context.  chosen [4] receives the grids (blocks),
  * stage_ms [4][4] the measured stage times per candidate.  The runs add to the E-step sums: call
  * rome_reset_sums afterwards.  rome_set_persistent_grids sets the four grids directly (0: the default,
- * 2, 8, 2 and 8 blocks per CU) and forgets the task and rotation totals of earlier pools, from which
- * the first launches of the fine diff2, the projection, the post slices and the backprojection are sized (the next pool's
+ * 2, 8, 2 and 32 blocks per CU) and forgets the task and rotation totals of earlier pools, from which
+ * the first launches of the fine diff2, the projection and the post slices are sized (the next pool's
  * are then sized from 4 per image, and what exceeds them goes through the work queue / the tails). */
 int rome_autotune(rome_ctx* ctx, const rome_device_images* images, const int* sig_keys, const int* sig_cnt, int cap,
                   int* chosen, double* stage_ms);

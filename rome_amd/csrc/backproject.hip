@@ -105,15 +105,14 @@ DEV void bp_count_one(const BpArgs& a, long long s, int pb, unsigned* keys, unsi
 // blocks stride over the slices of the slice total on the device.  No work counter: a device-scope
 // atomic on one address serialises about a million items a pool.
 template <bool Q>
-__global__ __launch_bounds__(256) void k_bp_count(BpArgs a, long long S_host, long long s_first) {
+__global__ __launch_bounds__(256) void k_bp_count(BpArgs a, long long S_host) {
     __shared__ unsigned keys[kHash], cnt[kHash];
     if constexpr (!Q) {
-        if (a.ntask_dev && (long long)blockIdx.x >= bp_slices(a, 0)) return;  // past the device total
         bp_count_one(a, blockIdx.x, blockIdx.y, keys, cnt);
         return;
     }
     const long long S = bp_slices(a, S_host);
-    for (long long sl = s_first + blockIdx.x; sl < S; sl += gridDim.x) {  // slices strided over the grid
+    for (long long sl = blockIdx.x; sl < S; sl += gridDim.x) {  // slices strided over the grid
         bp_count_one(a, sl, (int)blockIdx.y, keys, cnt);
         __syncthreads();  // the hash table of the next slice
     }
@@ -204,15 +203,14 @@ DEV void bp_fill_one(const BpArgs& a, long long s, int pb, unsigned* keys, unsig
 }
 
 template <bool Q>
-__global__ __launch_bounds__(256) void k_bp_fill(BpArgs a, long long S_host, long long s_first) {
+__global__ __launch_bounds__(256) void k_bp_fill(BpArgs a, long long S_host) {
     __shared__ unsigned keys[kHash], cnt[kHash], base[kHash];
     if constexpr (!Q) {
-        if (a.ntask_dev && (long long)blockIdx.x >= bp_slices(a, 0)) return;  // past the device total
         bp_fill_one(a, blockIdx.x, blockIdx.y, keys, cnt, base);
         return;
     }
     const long long S = bp_slices(a, S_host);
-    for (long long sl = s_first + blockIdx.x; sl < S; sl += gridDim.x) {  // slices strided over the grid
+    for (long long sl = blockIdx.x; sl < S; sl += gridDim.x) {  // slices strided over the grid
         bp_fill_one(a, sl, (int)blockIdx.y, keys, cnt, base);
         __syncthreads();  // the hash table of the next slice
     }
@@ -346,14 +344,14 @@ DEV void bp_brick_one(const BpArgs& a, int nb, unsigned ck) {
 // One block per chunk (the grid sized for the most chunks there can be), or (device-driven pools) a
 // fixed grid striding over the chunks, up to the total the scan left in coff[nb].
 template <bool Q>
-__global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb, unsigned ck_first) {
+__global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb) {
     if constexpr (!Q) {
-        if (blockIdx.x < a.coff[nb]) bp_brick_one(a, nb, blockIdx.x);  // coff[nb]: 0 for an aborted pool
+        if (blockIdx.x < a.coff[nb]) bp_brick_one(a, nb, blockIdx.x);
         return;
     }
     if (*a.abort) return;
     const unsigned nck = a.coff[nb];
-    for (unsigned ck = ck_first + blockIdx.x; ck < nck; ck += gridDim.x) {  // chunks strided over the grid
+    for (unsigned ck = blockIdx.x; ck < nck; ck += gridDim.x) {  // chunks strided over the grid
         bp_brick_one(a, nb, ck);
         __syncthreads();  // the box of the next chunk
     }
@@ -361,40 +359,31 @@ __global__ __launch_bounds__(256) void k_bp_brick(BpArgs a, int nb, unsigned ck_
 
 }  // namespace
 
-hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks,
-                                     long long S_guess) {
+hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks) {
     const int nb = a.nb_class * a.K;
     hipError_t e;
-    if (a.ntask_dev) {
-        // device-driven: one block per item up to the expected slice total (S_guess, from earlier pools;
-        // blocks past the device total return), then fixed grids striding over whatever lies beyond it
-        if (!a.abort || persist_blocks <= 0 || S_guess <= 0) return hipErrorInvalidValue;
+    if (a.ntask_dev) {  // device-driven: fixed grids striding over the device counts
+        if (!a.abort || persist_blocks <= 0) return hipErrorInvalidValue;
         if ((unsigned long long)a.slice_cap * a.Pf >= (1ULL << 32)) return hipErrorInvalidValue;
-        const long long Sg = std::min<long long>(S_guess, a.slice_cap);
         if ((e = launch_fill32(s, a.count, 0u, (size_t)nb)) != hipSuccess) return e;
         const int nPB = (a.Pf + 255) / 256;
-        const dim3 g1((unsigned)Sg, (unsigned)nPB);
         const dim3 g2((unsigned)std::max(1, persist_blocks / nPB), (unsigned)nPB);
-        k_bp_count<false><<<g1, 256, 0, s>>>(a, 0, 0);
-        k_bp_count<true><<<g2, 256, 0, s>>>(a, 0, Sg);
+        k_bp_count<true><<<g2, 256, 0, s>>>(a, 0);
         k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
-        k_bp_fill<false><<<g1, 256, 0, s>>>(a, 0, 0);
-        k_bp_fill<true><<<g2, 256, 0, s>>>(a, 0, Sg);
-        const unsigned long long chunks = (unsigned long long)nb + ((unsigned long long)Sg * a.Pf + kChunk - 1) / kChunk;
-        k_bp_brick<false><<<(unsigned)chunks, 256, 0, s>>>(a, nb, 0);
-        k_bp_brick<true><<<persist_blocks, 256, 0, s>>>(a, nb, (unsigned)chunks);
+        k_bp_fill<true><<<g2, 256, 0, s>>>(a, 0);
+        k_bp_brick<true><<<persist_blocks, 256, 0, s>>>(a, nb);
         return hipGetLastError();
     }
     if (S <= 0) return hipSuccess;
     if (S > 0x7fffffffLL || (unsigned long long)S * a.Pf >= (1ULL << 32)) return hipErrorInvalidValue;
     if ((e = launch_fill32(s, a.count, 0u, (size_t)nb)) != hipSuccess) return e;
     const dim3 grid((unsigned)S, (a.Pf + 255) / 256);
-    k_bp_count<false><<<grid, 256, 0, s>>>(a, S, 0);
+    k_bp_count<false><<<grid, 256, 0, s>>>(a, S);
     k_bp_scan<<<1, 1024, 0, s>>>(a.count, nb, a.off, a.coff, a.cursor);
-    k_bp_fill<false><<<grid, 256, 0, s>>>(a, S, 0);
+    k_bp_fill<false><<<grid, 256, 0, s>>>(a, S);
     // at most one chunk per brick plus one per kChunk records
     const unsigned long long chunks = (unsigned long long)nb + ((unsigned long long)S * a.Pf + kChunk - 1) / kChunk;
-    k_bp_brick<false><<<(unsigned)chunks, 256, 0, s>>>(a, nb, 0);
+    k_bp_brick<false><<<(unsigned)chunks, 256, 0, s>>>(a, nb);
     return hipGetLastError();
 }
 

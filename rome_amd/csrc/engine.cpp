@@ -331,10 +331,10 @@ struct rome_ctx {
     unsigned long long* work_counters(int first) const {
         return reinterpret_cast<unsigned long long*>(pool_flags.p + 4) + first;
     }
-    // Device-driven launches: the first launch of the fine diff2, the projection, the post slices and the
-    // backprojection covers the counts of the pools before (10 % above the largest of them, decaying; the
-    // blocks past a pool's own count return at once), a tail kernel or the work queue the rest.  Without
-    // history: 4 tasks / rotations per image.  Bounded by the capacities.
+    // Device-driven launches: the first launch of the fine diff2, the projection and the post slices
+    // covers the counts of the pools before (10 % above the largest of them, decaying; the blocks past a
+    // pool's own count return at once), a tail kernel or the work queue the rest.  Without history: 4
+    // tasks / rotations per image.  Bounded by the capacities.
     double hist_tasks = 0., hist_U = 0.;
     void note_counts(long long tasks, long long u) {
         hist_tasks = std::max((double)tasks, 0.9 * hist_tasks);
@@ -349,7 +349,7 @@ struct rome_ctx {
         return (int)std::max<long long>(1, std::min<long long>(cap, (long long)g));
     }
     int persist_blocks(int i) const {
-        static const int per_cu[4] = {2, 8, 2, 8};
+        static const int per_cu[4] = {2, 8, 2, 32};
         return persist[i] > 0 ? persist[i] : n_cu * per_cu[i];
     }
     // unique rotations a device-driven pool holds fine projections for (a pool that needs more aborts)
@@ -1513,7 +1513,7 @@ int run_pool(rome_ctx* ctx, int n, rome_results* res, int pool_mode, PoolOut* do
                 ba.abort = ctx->pool_flags.p;
             }
             stage_begin(ctx, 10);
-            HIPCHK(launch_backproject_binned(s, S, ba, ctx->persist_blocks(3), ctx->guess_tasks(n, ntasks) * NOR));
+            HIPCHK(launch_backproject_binned(s, S, ba, ctx->persist_blocks(3)));
             stage_end(ctx, 10);
         }
     } else {
@@ -2349,7 +2349,7 @@ int rome_autotune(rome_ctx* ctx, const rome_device_images* im, const int* sig_ke
     HIPCHK(hipSetDevice(ctx->device));
     // per tuned kernel: its stage and the candidate grids in blocks per CU
     static const int stage_of[4] = {6, 5, 8, 10};
-    static const int per_cu[4][kTuneCands] = {{1, 2, 3, 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {4, 8, 12, 16}};
+    static const int per_cu[4][kTuneCands] = {{1, 2, 3, 4}, {4, 8, 12, 16}, {1, 2, 3, 4}, {8, 16, 32, 64}};
     const bool prof = ctx->profiling;
     const int mode = ctx->exec_mode;
     int saved[4];

@@ -241,7 +241,7 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
     const int h = lane >> 5;
     int sl_j[NJ];
     bool ok_j[NJ];
-    float s_j[NJ], emin_j[NJ], vmin_j[NJ];
+    float s_j[NJ], emin_j[NJ], vmin_ij[NI][NJ];  // vmin per accumulator block: the count skips whole blocks
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         const int col = c0 + j * 32 + (lane & 31);
@@ -250,7 +250,8 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
         sl_j[j] = img - img0;
         s_j[j] = SCALED ? inv_scale[img] : 1.f;
         emin_j[j] = INFINITY;
-        vmin_j[j] = INFINITY;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) vmin_ij[i][j] = INFINITY;
     }
     // 1. E in place, minima
     if constexpr (Rows::kPaired) {
@@ -279,7 +280,8 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
                     acc[1][j][4 * q + u] = e1;
                     // the staged At is +inf on rows without a prior, so is their E: out of the minimum
                     emin_j[j] = fminf(fminf(emin_j[j], e0), e1);  // v_min3
-                    vmin_j[j] = fminf(fminf(vmin_j[j], e0 - lp0[u]), e1 - lp1[u]);
+                    vmin_ij[0][j] = fminf(vmin_ij[0][j], e0 - lp0[u]);
+                    vmin_ij[1][j] = fminf(vmin_ij[1][j], e1 - lp1[u]);
                 }
             }
         }
@@ -307,7 +309,7 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
                                                                     : INFINITY;
                     acc[i][j][4 * q + u] = e;
                     emin_j[j] = fminf(emin_j[j], e + pen[u]);  // rows with a prior
-                    vmin_j[j] = fminf(vmin_j[j], e - lpd[u]);  // +inf without a prior
+                    vmin_ij[i][j] = fminf(vmin_ij[i][j], e - lpd[u]);  // +inf without a prior
                 }
             }
         }
@@ -316,9 +318,11 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         // branch-free: padding columns and infinite minima contribute INT_MAX (no change)
+        float vmin = vmin_ij[0][j];
+#pragma unroll
+        for (int i = 1; i < NI; ++i) vmin = fminf(vmin, vmin_ij[i][j]);
         const int kmin = ok_j[j] && emin_j[j] < INFINITY ? float_order_key(emin_j[j]) : INT_MAX;
-        const int kvb = ok_j[j] && vmin_j[j] < INFINITY
-                            ? float_order_key(vmin_j[j] + 1e-6f * fabsf(vmin_j[j]) + 1e-4f) : INT_MAX;
+        const int kvb = ok_j[j] && vmin < INFINITY ? float_order_key(vmin + 1e-6f * fabsf(vmin) + 1e-4f) : INT_MAX;
         atomicMin(&slot_min[sl_j[j]], kmin);
         atomicMin(&slot_vb[sl_j[j]], kvb);
     }
@@ -356,8 +360,9 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
             int cnt = 0;
-            // vmin_j is the minimum of the same differences over the lane's rows: none passes unless it does
-            if (vmin_j[j] < lim_j[j]) {
+            // vmin_ij is the minimum of the same differences over the lane's rows of this block: none passes
+            // unless it does
+            if (vmin_ij[i][j] < lim_j[j]) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) cnt += acc[i][j][r] - lpd[r] < lim_j[j];
             }

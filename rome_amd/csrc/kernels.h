@@ -505,11 +505,9 @@ struct BpImageArgs {
 };
 hipError_t launch_bp_images(hipStream_t s, int n, const BpImageArgs& a);
 hipError_t launch_store_sums(hipStream_t s, const StoreArgs& a);
-// With a.ntask_dev (device-driven pools) S is read on the device: one block per (slice, pixel block) and
-// per brick chunk up to S_guess slices (the expected total), then persist_blocks blocks stride over the
-// slices and chunks beyond it.
-hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks = 0,
-                                     long long S_guess = 0);
+// With a.ntask_dev (device-driven pools) S is read on the device and persist_blocks blocks stride over
+// the slices (one pixel block per grid row) and over the brick chunks.
+hipError_t launch_backproject_binned(hipStream_t s, long long S, const BpArgs& a, int persist_blocks = 0);
 // M-step reconstruction (reconstruct.hip)
 hipError_t rc_fold(hipStream_t s, double2* data, double* weight, int Dz, int P, int PF);
 hipError_t rc_decenter_real(hipStream_t s, const double* cen, double* out, int Dz, int P, int PF, int max_r2);
