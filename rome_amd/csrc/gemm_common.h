@@ -15,8 +15,12 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // blockIdx -> (bm, bn).  Workgroups are dispatched round-robin over the 8 XCDs (XCD = bid % 8).
 // When the m-tiles split evenly, XCD x owns the band [x*band, (x+1)*band) of m-tiles and walks it in
 // sub-bands of `sband` tiles, n-tile after n-tile: the sub-band's rows of the rotation operand stay
-// in that XCD's L2 while the image operand streams past once per sub-band.  A bijection on
-// [0, nbm*nbn) in both branches.
+// in that XCD's L2 while the image operand streams past once per sub-band.  When the band is one
+// sub-band (one pass over the n-tiles), XCD x starts its pass x/8 of the way along the n-tiles, so
+// that the 8 bands reach an image's columns at 8 different times: the later ones find the image's
+// list bound set by the earlier ones and keep fewer entries (in the same order, all 8 bands of an
+// n-tile run at once and each sees only its own minimum).  A bijection on [0, nbm*nbn) in both
+// branches.
 DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
     const int band = nbm >> 3;
     if ((nbm & 7) == 0 && sband > 0 && band % sband == 0) {
@@ -25,6 +29,7 @@ DEV void gemm_tile(int bid, int nbm, int nbn, int sband, int& bm, int& bn) {
         const int sb = i / per, rem = i - sb * per;
         bm = x * band + sb * sband + rem % sband;
         bn = rem / sband;
+        if (sband == band) bn = (bn + x * ((nbn + 7) >> 3)) % nbn;
     } else {
         bm = bid % nbm;
         bn = bid / nbm;

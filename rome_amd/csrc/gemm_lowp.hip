@@ -299,6 +299,11 @@ DEV void gemm_epilogue_e(const f16v (&acc)[NI][NJ], const Rows& rows, int c0, in
 // 40 KB LDS stage, the next chunk waiting in registers; two blocks per CU.  The epilogue turns P, Q
 // into the two rotations' cross terms P + Q, P - Q and writes the list (LIST) or E.
 constexpr int kPqBN = 128;         // columns per block: 64 or 128
+// one 16-byte global load per lane straight into LDS: lane L writes lds + 16 L (lds wave-uniform)
+DEV void lds_load16(const void* g, void* lds) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+}
+DEV void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0), expcnt / lgkmcnt untouched
 constexpr int kPqBlocksPerCU = 2;  // blocks per CU the register budget is sized for
 template <typename T, bool LIST>
 __global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmArgs a) {
@@ -317,6 +322,19 @@ __global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmAr
     if (!gemm_tile_grouped(blockIdx.x, a.Mh / BM, a.Npad / BN, a.sband, bm, bn)) return;
     const int m0 = bm * BM, n0 = bn * BN;
     if (n0 >= a.Ncols) return;  // a tile of padding columns only (block-uniform, before any barrier)
+    // the list epilogue's rows of At (one per image slot of the tile) and of ln pdf_direction: with at most
+    // three image slots, wave w loads row w straight into LDS before the main loop (no registers held,
+    // no global round trip after it); otherwise they are staged after the loop through the operand stage
+    __shared__ __attribute__((aligned(16))) float AtD[4][256];
+    const int img0 = n0 / a.T, nsl = (min(n0 + BN, a.Ncols) - 1) / a.T - img0 + 1;
+    const bool at_early = LIST && nsl < 4;
+    if (at_early && wm <= nsl) {
+        const int lr = lane * 4, hh = m0 + (lr & 127);
+        const int m = (hh / a.psih) * 2 * a.psih + hh % a.psih + (lr >> 7) * a.psih;
+        lds_load16(wm < nsl ? static_cast<const void*>(a.At + (size_t)(img0 + wm) * a.Mpad + m)
+                            : static_cast<const void*>(a.list.lpd + m),
+                   &AtD[wm][0]);
+    }
     const T* A = reinterpret_cast<const T*>(a.A);
     const T* B = reinterpret_cast<const T*>(a.B);
     f16v acc[2][TJ];  // [0]: P = sum Re Re, [1]: Q = sum Im Im
@@ -386,10 +404,23 @@ __global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmAr
     }
 #undef PQ_LOAD
 #undef PQ_STORE
-    // the tile's 256 rotations (128 pairs): At of its images and ln pdf_direction into LDS (the operand
+    // the tile's 256 rotations (128 pairs): At of its images and ln pdf_direction in LDS (the operand
     // stage is free: the loop ended with a barrier)
-    float* Ats = reinterpret_cast<float*>(&As[0][0][0]);
-    const int img0 = n0 / a.T, nsl = (min(n0 + BN, a.Ncols) - 1) / a.T - img0 + 1;
+    float* Ats = at_early ? &AtD[0][0] : reinterpret_cast<float*>(&As[0][0][0]);
+    if (at_early) {
+        wait_vmem_all();  // this wave's row has landed; the barrier makes every wave's visible
+        __syncthreads();
+        for (int v = tid; v < nsl * 64; v += 256) {  // +inf on rows without a prior
+            const int sl = v >> 6, lr = (v & 63) * 4;
+            float4 x = *reinterpret_cast<const float4*>(Ats + sl * 256 + lr);
+            const float4 l = *reinterpret_cast<const float4*>(Ats + nsl * 256 + lr);
+            x.x += l.x > -INFINITY ? 0.f : INFINITY;
+            x.y += l.y > -INFINITY ? 0.f : INFINITY;
+            x.z += l.z > -INFINITY ? 0.f : INFINITY;
+            x.w += l.w > -INFINITY ? 0.f : INFINITY;
+            *reinterpret_cast<float4*>(Ats + sl * 256 + lr) = x;
+        }
+    } else
     for (int v = tid; v < (nsl + 1) * 64; v += 256) {
         const int sl = v >> 6, lr = (v & 63) * 4, hh = m0 + (lr & 127);
         const int m = (hh / a.psih) * 2 * a.psih + hh % a.psih + (lr >> 7) * a.psih;
