@@ -14,6 +14,7 @@ backprojection, per-image sums) over one batch of particles already resident in 
 Two arms run on the same particles: FP32 (coarse cross term on FP32 MFMA) and mixed (coarse cross term
 on BF16 or FP16 MFMA; fine pass, weights and sums FP32/FP64).  The headline value is the mixed arm's
 when its max-posterior poses agree with the FP32 arm's on >= 99.5 % of the particles, else the FP32 arm's.
+The batched adaptive fine-pass leg (192^2, K=8, hp3, its own engine) runs after them and is reported beside them.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n-images N] [--precision bf16|fp16|fp32]
 """
@@ -322,7 +323,7 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--execution", choices=["host", "device", "graph"], default="graph",
                     help="pool execution: host-synchronised, device-driven, or device-driven captured as hipGraphs")
-    ap.add_argument("--adaptive-n", type=int, default=0,
+    ap.add_argument("--adaptive-n", type=int, default=20000,
                     help="particles of the batched adaptive fine-pass leg (192^2, K=8, hp3; 0: not run)")
     ap.add_argument("--adaptive-batch", type=int, default=250,
                     help="particles per pool of that leg (a pool's fine projections and backprojection slices "
