@@ -251,7 +251,7 @@ DEV void gemm_epilogue_list(f16v (&acc)[NI][NJ], const Rows& rows, int c0, int n
     for (int j = 0; j < NJ; ++j) {
         const int col = c0 + j * 32 + (lane & 31);
         ok_j[j] = col < Ncols;
-        const int img = min(col, Ncols - 1) / T;  // clamped: padding columns are dropped
+        const int img = (int)((unsigned)min(col, Ncols - 1) / (unsigned)T);  // clamped: padding columns are dropped
         sl_j[j] = img - img0;
         s_j[j] = SCALED ? inv_scale[img] : 1.f;
         emin_j[j] = INFINITY;

@@ -241,7 +241,8 @@ struct PairRows {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int hh = hbase + lrow + 8 * q;
-            m[q] = (hh / ph) * 2 * ph + hh % ph;
+            const unsigned uh = (unsigned)hh, up = (unsigned)ph;  // non-negative: the shorter unsigned division
+            m[q] = (int)((uh / up) * 2 * up + uh % up);
         }
     }
     DEV int operator()(int i, int q) const { return m[q] + i * psih; }
@@ -330,7 +331,8 @@ __global__ __launch_bounds__(256, kPqBlocksPerCU) void k_gemm_lowp_pq(LowpGemmAr
     const bool at_early = LIST && nsl < 4;
     if (at_early && wm <= nsl) {
         const int lr = lane * 4, hh = m0 + (lr & 127);
-        const int m = (hh / a.psih) * 2 * a.psih + hh % a.psih + (lr >> 7) * a.psih;
+        const unsigned uh = (unsigned)hh, up = (unsigned)a.psih;
+        const int m = (int)((uh / up) * 2 * up + uh % up) + (lr >> 7) * a.psih;
         lds_load16(wm < nsl ? static_cast<const void*>(a.At + (size_t)(img0 + wm) * a.Mpad + m)
                             : static_cast<const void*>(a.list.lpd + m),
                    &AtD[wm][0]);
